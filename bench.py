@@ -1,0 +1,200 @@
+"""Benchmark: DG right-hand-side throughput on one MI355X.
+
+Workload (default): 3-D compressible Navier-Stokes Taylor-Green vortex,
+periodic cube [0, 2 pi]^3 of 24^3 cells x 6 tets = 82,944 tets, p=3
+(20 nodes/element, 1,658,880 nodes x 5 fields), Re=1600, Ma=0.1, Pr=0.71,
+constant viscosity, ideal gas gamma=1.4, Rusanov inviscid flux and central
+viscous flux.  One "step" is one full RHS evaluation (``ns_operator``: the
+gradient kernel and the divergence kernel) with the state resident in HBM.
+
+Prints ONE JSON line with the metric (DOF-RHS/s = nodes x fields / RHS wall
+time), the roofline fraction of the dominant kernel (algorithmic HBM bytes
+per launch / its average event-timed duration / 8 TB/s) and the CPU baseline
+(the FP64 NumPy oracle on a bounded sample of the same workload, host cores).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=24, help="cells per axis")
+    ap.add_argument("--order", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-n", type=int, default=12,
+                    help="cells per axis of the CPU-baseline sample")
+    return ap.parse_args()
+
+
+def alg_bytes(K, np_, nfp, dim, nfld):
+    """Algorithmic HBM bytes per launch of the two NS kernels: every nodal
+    field read or written once, per-element geometry (rx, normals, face
+    Jacobian ratios) and the int32 neighbour map once."""
+    n = K * np_
+    nf = dim + 1
+    geo = K * (8 * (dim * dim + nf * dim + nf) + 4 * nf * nfp)
+    grad = 8 * n * (nfld + (nfld + 1) * dim) + geo
+    rhs = 8 * n * (nfld + (nfld + 1) * dim + nfld) + geo
+    return {"mg_ns_gradient": grad, "mg_ns_rhs": rhs}
+
+
+def alg_flops(K, np_, nfp, dim, nfld):
+    """Contraction flops per launch (element-local matrix applications)."""
+    nfn = (dim + 1) * nfp
+    grad = K * (nfld + 1) * (2 * dim * np_ * np_ + 2 * np_ * nfn)
+    rhs = K * nfld * (2 * dim * np_ * np_ + 2 * np_ * nfn)
+    return {"mg_ns_gradient": grad, "mg_ns_rhs": rhs}
+
+
+def cpu_baseline(n, order):
+    """The FP64 oracle (NumPy) on a TGV mesh of n^3 x 6 tets, same order."""
+    import numpy as np
+
+    from oracle import initializers as oini, operators as oop, physics as ph
+    from tests.cases import Problem
+
+    cores = len(os.sched_getaffinity(0))
+    pb = Problem(3, order, (n,) * 3, 0.0, 2 * np.pi, (True,) * 3)
+    q0 = oini.taylor_green(pb.x)
+    mu, kappa = _tgv_transport()
+    ogm = ph.GasModel(ph.IdealSingleGas(),
+                      ph.SimpleTransport(viscosity=mu, thermal_conductivity=kappa))
+    reps = 0
+    t0 = time.perf_counter()
+    while True:
+        st = ph.make_fluid_state(q0, ogm, 3)
+        oop.ns_operator(pb.odisc, pb.box, st, ogm, {})
+        reps += 1
+        el = time.perf_counter() - t0
+        if el > 10.0 or reps >= 20:
+            break
+    K = pb.mesh.nelements
+    dofs = K * pb.dcoll.refel.nnodes * 5
+    return {"value": dofs * reps / el, "unit": "DOF-RHS/s", "cores": cores,
+            "kind": "port",
+            "sample": f"FP64 NumPy oracle, TGV {n}^3x6={K} tets p={order}, "
+                      f"{reps} RHS evaluations in {el:.1f} s "
+                      f"(NumPy; BLAS threads <= {cores})"}
+
+
+def _tgv_transport():
+    from mirgecom_amd.initializers import TaylorGreenVortex
+    return TaylorGreenVortex(mach=0.1).transport_for(
+        reynolds=1600.0, prandtl=0.71, gamma=1.4, gas_const=287.1)
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU path is a design note, not built)",
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from mirgecom_amd import _lib
+    from mirgecom_amd.discretization import (
+        create_discretization_collection, generate_box_mesh)
+    from mirgecom_amd.eos import IdealSingleGas
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from mirgecom_amd.initializers import TaylorGreenVortex
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+    from mirgecom_amd.navierstokes import ns_operator
+    from mirgecom_amd.transport import SimpleTransport
+
+    import math
+    n, order = args.n, args.order
+    t0 = time.perf_counter()
+    mesh = generate_box_mesh(3, 0.0, 2 * math.pi, (n,) * 3, periodic=(True,) * 3)
+    dcoll = create_discretization_collection(mesh, order)
+    print(f"# setup: {mesh.nelements} tets p={order} in "
+          f"{time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    tgv = TaylorGreenVortex(mach=0.1)
+    mu, kappa = _tgv_transport()
+    gm = GasModel(eos=IdealSingleGas(gamma=1.4, gas_const=287.1),
+                  transport=SimpleTransport(viscosity=mu,
+                                            thermal_conductivity=kappa))
+    x = dcoll.nodes()
+    cv = tgv(x, eos=gm.eos)
+    K, npe = mesh.nelements, dcoll.refel.nnodes
+    nfld = 5
+    dofs = K * npe * nfld
+
+    def step():
+        st = make_fluid_state(cv, gm)
+        return ns_operator(dcoll, gm, st, {},
+                           inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _lib.timing = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    timing = _lib.timing
+    _lib.timing = None
+    kern_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
+               for k, v in timing.items()}
+    ms = wall / args.steps * 1e3
+    abytes = alg_bytes(K, npe, dcoll.refel.nface_nodes, 3, nfld)
+    aflops = alg_flops(K, npe, dcoll.refel.nface_nodes, 3, nfld)
+    dom = max(kern_ms, key=kern_ms.get)
+    achieved = abytes[dom] / (kern_ms[dom] * 1e-3) / 1e9
+    per_kernel = {k: {"ms": round(kern_ms[k], 4),
+                      "alg_bytes": abytes[k],
+                      "GB_s": round(abytes[k] / (kern_ms[k] * 1e-3) / 1e9, 1),
+                      "TFLOP_s_contractions": round(
+                          aflops[k] / (kern_ms[k] * 1e-3) / 1e12, 2)}
+                  for k in kern_ms}
+    print("# kernels: " + json.dumps(per_kernel), file=sys.stderr, flush=True)
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(args.cpu_n, order)
+    out = {
+        "metric": "DOF-RHS/s (3D NS p=3) mixed vs FP64 at <=1e-6 rel err; chem-surrogate speedup",
+        "value": dofs / (ms * 1e-3),
+        "unit": "DOF-RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Taylor-Green vortex initial state)",
+        "config": {"workload": f"3D NS Taylor-Green, periodic [0,2pi]^3, "
+                               f"{n}^3x6={K} tets, p={order}, Re=1600, "
+                               f"Ma=0.1, Pr=0.71, Rusanov + central viscous",
+                   "nodes": K * npe, "fields": nfld, "dofs": dofs,
+                   "kernels": per_kernel},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,133 @@
+/* C ABI of the MI355X DG right-hand-side library (libmgdg.so).
+ *
+ * Every entry point takes device pointers (HBM, caller-allocated), sizes and
+ * a hipStream_t (passed as void*).  Nothing allocates except where stated,
+ * nothing synchronises the stream, and nothing throws across the ABI: a
+ * nonzero mg_status is returned and mg_last_error() gives the message.
+ *
+ * Layouts (N = nelements * np nodes, F = dim + 2 + nspecies fields):
+ *   conserved state q         [F][N]    field order mass, energy,
+ *                                       momentum[dim], species[nspecies]
+ *                                       (mirgecom/fluid.py:311-333)
+ *   gradient g                [(F+1)][dim][N]   grad of each conserved field,
+ *                                       then grad T
+ *   boundary exterior state   [F][nbnd_nodes], temperature [nbnd_nodes]
+ *
+ * The reference's operator interface is Python-level; the Python host layer
+ * (mirgecom_amd.euler / mirgecom_amd.navierstokes) binds these entry points
+ * with ctypes behind the reference's signatures.
+ */
+#ifndef MGDG_H
+#define MGDG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t mg_status;
+#define MG_OK 0
+#define MG_ERR_ARG 1
+#define MG_ERR_UNSUPPORTED 2
+#define MG_ERR_HIP 3
+
+/* Discretization: reference-element operators, affine geometry and face
+ * connectivity, all resident in HBM (built once on the host; see
+ * create_discretization_collection, mirgecom/discretization.py:41-83). */
+typedef struct mg_mesh {
+  int32_t dim, order, np, nfp, nfaces, pad0;
+  int64_t nelements;
+  int64_t nbnd_nodes;
+  const double* rx;         /* [K][dim][dim]  dr_a/dx_i                    */
+  const double* normals;    /* [K][nfaces][dim] outward unit normals       */
+  const double* fscale;     /* [K][nfaces]  face Jacobian / volume Jacobian */
+  const int32_t* vmap_p;    /* [K][nfaces][nfp] neighbour node, or -(b+1)  */
+  const double* weak_diff;  /* [dim][np][np]  M^-1 D_r^T M                 */
+  const double* lift;       /* [np][nfaces*nfp]  M^-1 E                    */
+  const int32_t* face_mask; /* [nfaces][nfp] volume node of each face node */
+} mg_mesh;
+
+/* Gas model: equation of state and transport (mirgecom/gas_model.py:82-97,
+ * mirgecom/eos.py, mirgecom/transport.py). */
+typedef struct mg_gas {
+  int32_t eos_kind;        /* 0 IdealSingleGas, 1 uiuc_7sp mixture          */
+  int32_t transport_kind;  /* 0 none, 1 SimpleTransport, 2 PowerLawTransport*/
+  int32_t nspecies;
+  int32_t has_lewis;       /* power law: species_diffusivity holds Lewis nos */
+  double gamma, gas_const; /* IdealSingleGas (eos.py:329)                   */
+  double mu, mu_b, kappa;  /* SimpleTransport (transport.py:143-202)        */
+  double pl_alpha, pl_beta, pl_sigma, pl_n; /* PowerLaw (transport.py:205-321)*/
+  double species_diffusivity[8];
+} mg_gas;
+
+/* Numerical inviscid flux kinds: inviscid_facial_flux_rusanov,
+ * inviscid_facial_flux_hll, inviscid_facial_flux_central
+ * (mirgecom/inviscid.py:105-287). */
+#define MG_FLUX_RUSANOV 0
+#define MG_FLUX_HLL 1
+#define MG_FLUX_CENTRAL 2
+
+/* Euler RHS, -div F_I with the numerical flux on faces
+ * (replaces euler_operator, mirgecom/euler.py:313-416). temperature is
+ * read for mixtures only (may be NULL otherwise); q_bnd/t_bnd hold the
+ * exterior state of every boundary node (t_bnd may be NULL for ideal gas). */
+mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas,
+                       const double* q, const double* temperature,
+                       const double* q_bnd, const double* t_bnd,
+                       int32_t flux_kind, double* rhs, void* stream);
+
+/* DG gradients of the conserved fields and of T with the central flux on
+ * interior faces and the exterior value on boundary faces
+ * (replaces grad_cv_operator + grad_t_operator,
+ * mirgecom/navierstokes.py:129-347). */
+mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas,
+                         const double* q, const double* temperature,
+                         const double* q_bnd, const double* t_bnd,
+                         double* grad, void* stream);
+
+/* Navier-Stokes RHS div(F_V - F_I) from q and its gradients (replaces the
+ * divergence part of ns_operator, mirgecom/navierstokes.py:533-575). */
+mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas,
+                    const double* q, const double* temperature,
+                    const double* grad, const double* q_bnd,
+                    const double* t_bnd, int32_t flux_kind,
+                    int32_t inviscid_on, double* rhs, void* stream);
+
+/* Mixture temperature by a fixed number of Newton iterations from a seed
+ * (replaces PyrometheusMixture.temperature, mirgecom/eos.py:777-803, and
+ * get_temperature, mirgecom/thermochemistry.py:126-167). q is [F][n]. */
+mg_status mg_mixture_temperature(const mg_gas* gas, int32_t dim, int64_t n,
+                                 const double* q, const double* t_seed,
+                                 int32_t niter, double* t_out, void* stream);
+
+/* Species mass sources W_k * omega_k into src [nspecies][n]
+ * (replaces get_species_source_terms, mirgecom/eos.py:858-875). */
+mg_status mg_species_source(const mg_gas* gas, int32_t dim, int64_t n,
+                            const double* q, const double* temperature,
+                            double* src, void* stream);
+
+/* Classical RK4 stage helpers over n values (mirgecom/integrators/
+ * explicit_rk.py:31-38): out = y + a*k ; acc = acc + w*k (acc may be NULL). */
+mg_status mg_rk_axpy(int64_t n, const double* y, const double* k, double a,
+                     double* out, double* acc, double w, void* stream);
+
+/* Upload the uiuc_7sp mechanism constants (NASA7 tables, molecular weights,
+ * gas constant, standard pressure, SI Arrhenius parameters; 130 doubles in
+ * the order of mirgecom_amd.mechanisms.packed_mechanism) to the device.
+ * Needed once per process before any mixture call; it replaces the
+ * pyrometheus code generation of the mechanism class
+ * (mirgecom/thermochemistry.py:186-219). */
+mg_status mg_load_mechanism(const double* packed, int64_t count);
+
+/* Message of the last failed call on this thread. */
+const char* mg_last_error(void);
+
+/* Library ABI version. */
+int32_t mg_abi_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MGDG_H */

@@ -1,0 +1,145 @@
+"""ctypes binding of ``libmgdg.so`` (C ABI in include/mgdg.h).
+
+The library is built in-tree by ``mirgecom_amd.build``.  There is no fallback:
+if the library is missing, loading raises, and every operator that needs it
+fails loudly.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmgdg.so")
+
+MG_FLUX_RUSANOV = 0
+MG_FLUX_HLL = 1
+MG_FLUX_CENTRAL = 2
+
+
+class MgMesh(ctypes.Structure):
+    _fields_ = [
+        ("dim", ctypes.c_int32), ("order", ctypes.c_int32),
+        ("np", ctypes.c_int32), ("nfp", ctypes.c_int32),
+        ("nfaces", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("nelements", ctypes.c_int64), ("nbnd_nodes", ctypes.c_int64),
+        ("rx", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+        ("fscale", ctypes.c_void_p), ("vmap_p", ctypes.c_void_p),
+        ("weak_diff", ctypes.c_void_p), ("lift", ctypes.c_void_p),
+        ("face_mask", ctypes.c_void_p),
+    ]
+
+
+class MgGas(ctypes.Structure):
+    _fields_ = [
+        ("eos_kind", ctypes.c_int32), ("transport_kind", ctypes.c_int32),
+        ("nspecies", ctypes.c_int32), ("has_lewis", ctypes.c_int32),
+        ("gamma", ctypes.c_double), ("gas_const", ctypes.c_double),
+        ("mu", ctypes.c_double), ("mu_b", ctypes.c_double),
+        ("kappa", ctypes.c_double),
+        ("pl_alpha", ctypes.c_double), ("pl_beta", ctypes.c_double),
+        ("pl_sigma", ctypes.c_double), ("pl_n", ctypes.c_double),
+        ("species_diffusivity", ctypes.c_double * 8),
+    ]
+
+
+# exported symbol -> (restype, argtypes)
+_P = ctypes.c_void_p
+_I32 = ctypes.c_int32
+_I64 = ctypes.c_int64
+_D = ctypes.c_double
+SIGNATURES = {
+    "mg_euler_rhs": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                            _P, _P, _P, _P, _I32, _P, _P]),
+    "mg_ns_gradient": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                              _P, _P, _P, _P, _P, _P]),
+    "mg_ns_rhs": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                         _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
+    "mg_mixture_temperature": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P,
+                                      _P, _I32, _P, _P]),
+    "mg_species_source": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,
+                                 _P, _P]),
+    "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
+    "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),
+    "mg_last_error": (ctypes.c_char_p, []),
+    "mg_abi_version": (_I32, []),
+}
+
+_lib = None
+
+
+class MgError(RuntimeError):
+    pass
+
+
+def load():
+    """Load and type the library (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is not built; run `python -m mirgecom_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status):
+    if status != 0:
+        msg = load().mg_last_error().decode()
+        raise MgError(f"libmgdg error {status}: {msg}")
+
+
+_mech_loaded = False
+
+
+def ensure_mechanism():
+    global _mech_loaded
+    if _mech_loaded:
+        return
+    from mirgecom_amd.mechanisms import packed_mechanism
+    arr = np.ascontiguousarray(packed_mechanism())
+    check(load().mg_load_mechanism(
+        arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), arr.size))
+    _mech_loaded = True
+
+
+# Optional per-entry-point timing: when ``timing`` is a dict, ``call`` records
+# a pair of events on the current stream around every launch and appends it
+# to ``timing[name]`` (used by the benchmark; off by default).
+timing = None
+
+
+def call(name, *args):
+    """Call entry point ``name`` and raise ``MgError`` on a nonzero status."""
+    fn = getattr(load(), name)
+    if timing is None:
+        check(fn(*args))
+        return
+    import torch
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record()
+    check(fn(*args))
+    stop.record()
+    timing.setdefault(name, []).append((start, stop))
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (None -> NULL)."""
+    if t is None:
+        return None
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def stream_handle():
+    import torch
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)

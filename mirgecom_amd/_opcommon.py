@@ -1,0 +1,112 @@
+"""Shared host-side preparation of the GPU operator calls."""
+
+from __future__ import annotations
+
+from warnings import warn
+
+import torch
+
+from mirgecom_amd import _lib
+from mirgecom_amd.device import get_device_discretization
+from mirgecom_amd.dof_desc import (
+    BTAG_ALL, DISCR_TAG_BASE, BoundaryDomainTag, DOFDesc, VolumeDomainTag)
+from mirgecom_amd.fluid import ConservedVars
+from mirgecom_amd.gas_model import FluidState, _LazyDV
+from mirgecom_amd.utils import normalize_boundaries
+
+
+def check_dd(dd):
+    """mirgecom/euler.py:369-372"""
+    if not isinstance(dd.domain_tag, VolumeDomainTag):
+        raise TypeError("dd must represent a volume")
+    if dd.discretization_tag != DISCR_TAG_BASE:
+        raise ValueError("dd must belong to the base discretization")
+
+
+def check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
+                      limiter_func, entropy_min, comm_tag=None):
+    if use_esdg:
+        raise NotImplementedError("the entropy-stable (ESDG) operator is not "
+                                  "part of the device path")
+    if quadrature_tag != DISCR_TAG_BASE:
+        raise NotImplementedError("overintegration is not part of the device "
+                                  "path")
+    if operator_states_quad is not None:
+        raise NotImplementedError("operator_states_quad is not supported")
+    if limiter_func is not None or entropy_min is not None:
+        raise NotImplementedError("limiters are not part of the device path")
+
+
+class Prepared:
+    """Device pointers and boundary exterior states for one operator call."""
+
+    def __init__(self, dcoll, state, gas_model, boundaries, time):
+        cv = state.cv
+        q = cv.q
+        if not q.is_cuda:
+            raise RuntimeError("the GPU operators need the state on a HIP "
+                               "device")
+        if q.dtype != torch.float64:
+            raise TypeError("the state must be float64")
+        self.dev = get_device_discretization(dcoll, q.device)
+        dev = self.dev
+        host = dcoll
+        K, npe = host.nelements, host.refel.nnodes
+        if tuple(q.shape[1:]) != (K, npe):
+            raise ValueError(f"state shape {tuple(q.shape)} does not match the "
+                             f"discretization ({K}, {npe})")
+        self.q = q.contiguous()
+        self.dim = cv.dim
+        self.nspecies = cv.nspecies
+        self.mix = bool(getattr(gas_model.eos, "is_mixture", False))
+        self.temp = None
+        if self.mix:
+            self.temp = state.temperature.contiguous()
+        self.gas = gas_model.device_params(cv.nspecies)
+        boundaries = normalize_boundaries(boundaries)
+        nb = host.nbnd_nodes
+        self.qb = None
+        self.tb = None
+        if nb > 0:
+            self.qb = torch.empty((q.shape[0], nb), dtype=q.dtype,
+                                  device=q.device)
+            if self.mix:
+                self.tb = torch.empty(nb, dtype=q.dtype, device=q.device)
+            covered = torch.zeros(nb, dtype=torch.int32, device=q.device)
+            flat_q = self.q.reshape(q.shape[0], -1)
+            for btag, bdry in boundaries.items():
+                tag = btag.tag
+                idx = dev.boundary_index(tag)
+                if idx.numel() == 0:
+                    continue
+                covered[idx] += 1
+                vidx = dev.bnd_vol_index[idx]
+                cv_m = ConservedVars(self.dim, flat_q[:, vidx])
+                t_m = (self.temp.reshape(-1)[vidx] if self.mix else None)
+                state_m = FluidState(cv_m, _LazyDV(cv_m, gas_model.eos, t_m),
+                                     gas_model)
+                dd_bdry = DOFDesc(BoundaryDomainTag(tag), DISCR_TAG_BASE)
+                ext = bdry.exterior_state(dcoll, dd_bdry, gas_model, state_m,
+                                          time=time)
+                self.qb[:, idx] = ext.cv.q.to(q.dtype)
+                if self.mix:
+                    self.tb[idx] = ext.temperature
+            if int(covered.min()) != 1 or int(covered.max()) != 1:
+                raise ValueError("every boundary face needs exactly one "
+                                 "boundary condition")
+        elif boundaries:
+            unknown = [b.tag for b in boundaries
+                       if b.tag != BTAG_ALL
+                       and b.tag not in host.boundary_tags()]
+            if unknown:
+                raise ValueError(f"unknown boundary tags {unknown}")
+        self.stream = _lib.stream_handle()
+
+
+def warn_mixture_limiter(state, limiter_func):
+    if state.is_mixture and limiter_func is None:
+        warn("Mixtures often require species limiting, and a non-limited "
+             "state is being created for this operator. For mixtures, "
+             "one should pass the operator_states_quad argument with "
+             "limited states or pass a limiter_func to this operator.",
+             stacklevel=3)

@@ -1,0 +1,313 @@
+// Pointwise gas physics evaluated inside the DG kernels (FP64).
+//
+// Formulas follow the reference:
+//   ideal gas EOS           mirgecom/eos.py:302-537
+//   uiuc_7sp mixture EOS    mirgecom/eos.py:540-888, thermochemistry.py:87-167
+//   transport               mirgecom/transport.py:143-321
+//   inviscid flux           mirgecom/inviscid.py:65-102
+//   Rusanov / central / HLL mirgecom/inviscid.py:105-287, flux.py:45-177
+//   viscous flux            mirgecom/viscous.py:72-293
+// The operation order follows the reference expressions where it costs
+// nothing, so that the FP64 results agree with the oracle to a few ulp.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace mgdg {
+
+constexpr int binom(int n, int k) {
+  int r = 1;
+  for (int i = 1; i <= k; ++i) r = r * (n - k + i) / i;
+  return r;
+}
+
+// Gas model parameters passed by value to every kernel.
+struct GasDev {
+  int eos_kind;        // 0 ideal single gas, 1 uiuc_7sp mixture
+  int transport_kind;  // 0 none, 1 constant (SimpleTransport), 2 power law
+  double gamma, gas_const;
+  double mu, mu_b, kappa;
+  double pl_alpha, pl_beta, pl_sigma, pl_n;
+  int has_lewis;
+  double diff[8];      // species diffusivities, or Lewis numbers if has_lewis
+};
+
+// {{{ uiuc_7sp thermochemistry (species order C2H4, O2, CO2, CO, H2O, H2, N2)
+
+constexpr int MIX_NSP = 7;
+
+struct MechDev {
+  double nasa_lo[MIX_NSP][7];
+  double nasa_hi[MIX_NSP][7];
+  double t_mid[MIX_NSP];
+  double wt[MIX_NSP];
+  double inv_wt[MIX_NSP];
+  double gas_constant;   // J/(kmol K)
+  double one_atm;        // Pa
+  // Arrhenius: k = exp(log_a + b log T - ea_r / T)
+  double log_a[3], b[3], ea_r[3];
+};
+
+extern __constant__ MechDev c_mech;
+
+__device__ __forceinline__ void nasa7_cp_h(int k, double t, double& cp_r,
+                                           double& h_rt) {
+  const double* a = (t > c_mech.t_mid[k]) ? c_mech.nasa_hi[k] : c_mech.nasa_lo[k];
+  cp_r = a[0] + a[1] * t + a[2] * t * t + a[3] * t * t * t + a[4] * t * t * t * t;
+  h_rt = a[0] + a[1] * 0.5 * t + a[2] / 3.0 * t * t + a[3] * 0.25 * t * t * t +
+         a[4] * 0.2 * t * t * t * t + a[5] / t;
+}
+
+__device__ __forceinline__ double nasa7_s_r(int k, double t) {
+  const double* a = (t > c_mech.t_mid[k]) ? c_mech.nasa_hi[k] : c_mech.nasa_lo[k];
+  return a[0] * log(t) + a[1] * t + a[2] * 0.5 * t * t + a[3] / 3.0 * t * t * t +
+         a[4] * 0.25 * t * t * t * t + a[6];
+}
+
+// Mixture internal energy and cv (mass basis) at T.
+__device__ __forceinline__ void mix_e_cv(double t, const double* y, double& e,
+                                         double& cv) {
+  double se = 0.0, sc = 0.0;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    double cp_r, h_rt;
+    nasa7_cp_h(k, t, cp_r, h_rt);
+    se += c_mech.inv_wt[k] * (h_rt - 1.0) * y[k];
+    sc += c_mech.inv_wt[k] * (cp_r - 1.0) * y[k];
+  }
+  e = c_mech.gas_constant * t * se;
+  cv = c_mech.gas_constant * sc;
+}
+
+// Fixed-iteration Newton solve for T from mass-specific internal energy
+// (thermochemistry.py:126-167, update at :99-102).
+__device__ __forceinline__ double mix_temperature(double e_in, double t_seed,
+                                                  const double* y, int niter) {
+  double t = t_seed;
+  for (int it = 0; it < niter; ++it) {
+    double e, cv;
+    mix_e_cv(t, y, e, cv);
+    t = t + (e_in - e) / cv;
+  }
+  return t;
+}
+
+// }}}
+
+template <int DIM, int NSP>
+struct Node {
+  static constexpr int NFLD = 2 + DIM + NSP;
+  static constexpr int NS1 = NSP > 0 ? NSP : 1;
+  double q[NFLD];
+  double v[DIM];
+  double y[NS1];
+  double hk[NS1];
+  double p, t, c, gamma, speed;
+  // transport
+  double mu, mu_b, kappa;
+  double d[NS1];
+};
+
+// Fill the dependent variables of a node from q (and T for mixtures).
+template <int DIM, int NSP, bool MIX>
+__device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
+                                           double t_mix) {
+  const double rho = s.q[0];
+  double mm = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) mm += s.q[2 + d] * s.q[2 + d];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) s.v[d] = s.q[2 + d] / rho;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) s.y[k] = s.q[2 + DIM + k] / rho;
+  double vv = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) vv += s.v[d] * s.v[d];
+  s.speed = sqrt(vv);
+  const double e_int = s.q[1] - 0.5 * mm / rho;
+  if constexpr (!MIX) {
+    s.gamma = g.gamma;
+    s.p = e_int * (g.gamma - 1.0);
+    s.t = ((g.gamma - 1.0) / g.gas_const) * e_int / rho;
+    s.c = sqrt(g.gamma / rho * s.p);
+  } else {
+    static_assert(NSP == MIX_NSP, "mixture path is the 7-species mechanism");
+    s.t = t_mix;
+    double rsum = 0.0, cpsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < MIX_NSP; ++k) {
+      double cp_r, h_rt;
+      nasa7_cp_h(k, t_mix, cp_r, h_rt);
+      rsum += s.y[k] * c_mech.inv_wt[k];
+      cpsum += c_mech.inv_wt[k] * cp_r * s.y[k];
+      s.hk[k] = c_mech.gas_constant * c_mech.inv_wt[k] * t_mix * h_rt;
+    }
+    const double rmix = c_mech.gas_constant * rsum;
+    const double cp = c_mech.gas_constant * cpsum;
+    s.p = rho * rmix * t_mix;
+    s.gamma = cp / (cp - rmix);
+    s.c = sqrt(s.gamma * s.p / rho);
+  }
+  // transport
+  if (g.transport_kind == 1) {
+    s.mu = g.mu;
+    s.mu_b = g.mu_b;
+    s.kappa = g.kappa;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) s.d[k] = g.diff[k];
+  } else if (g.transport_kind == 2) {
+    const double mu = g.pl_beta * pow(s.t, g.pl_n);
+    s.mu = mu;
+    s.mu_b = g.pl_alpha * mu;
+    double cv;
+    if constexpr (!MIX) {
+      cv = g.gas_const / (g.gamma - 1.0);
+    } else {
+      double e;
+      mix_e_cv(s.t, s.y, e, cv);
+    }
+    s.kappa = g.pl_sigma * mu * cv;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k)
+      s.d[k] = g.has_lewis ? g.pl_sigma * mu / (rho * g.diff[k] * s.gamma)
+                           : g.diff[k];
+  }
+}
+
+// Inviscid flux F[f][d] (inviscid.py:65-102).
+template <int DIM, int NSP>
+__device__ __forceinline__ void inviscid_flux(const Node<DIM, NSP>& s,
+                                              double (*F)[DIM]) {
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    F[0][d] = s.q[2 + d];
+    F[1][d] = s.v[d] * (s.q[1] + s.p);
+#pragma unroll
+    for (int i = 0; i < DIM; ++i)
+      F[2 + i][d] = s.q[0] * (s.v[i] * s.v[d]) + (i == d ? s.p : 0.0);
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) F[2 + DIM + k][d] = s.v[d] * s.q[2 + DIM + k];
+  }
+}
+
+// Normal inviscid flux F.n
+template <int DIM, int NSP>
+__device__ __forceinline__ void inviscid_flux_n(const Node<DIM, NSP>& s,
+                                                const double* n, double* fn) {
+  double vn = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) vn += s.v[d] * n[d];
+  double mn = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) mn += s.q[2 + d] * n[d];
+  fn[0] = mn;
+  fn[1] = vn * (s.q[1] + s.p);
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) fn[2 + i] = s.q[0] * s.v[i] * vn + s.p * n[i];
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) fn[2 + DIM + k] = vn * s.q[2 + DIM + k];
+}
+
+// Numerical inviscid flux: 0 Rusanov, 1 HLL, 2 central.
+template <int DIM, int NSP>
+__device__ __forceinline__ void inviscid_numflux(const Node<DIM, NSP>& a,
+                                                 const Node<DIM, NSP>& b,
+                                                 const double* n, int kind,
+                                                 double* out) {
+  constexpr int NF = 2 + DIM + NSP;
+  double fa[NF], fb[NF];
+  inviscid_flux_n<DIM, NSP>(a, n, fa);
+  inviscid_flux_n<DIM, NSP>(b, n, fb);
+  if (kind == 1) {
+    double u_a = 0.0, u_b = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      u_a += a.v[d] * n[d];
+      u_b += b.v[d] * n[d];
+    }
+    const double p_star = 0.5 * (a.p + b.p) +
+                          (1. / 8.) * (u_a - u_b) * (a.q[0] + b.q[0]) * (a.c + b.c);
+    const double qa = 1 + (a.gamma + 1) / (2 * a.gamma) * (p_star / a.p - 1);
+    const double qb = 1 + (b.gamma + 1) / (2 * b.gamma) * (p_star / b.p - 1);
+    const double qa_rt = sqrt(p_star > a.p ? qa : 1.0);
+    const double qb_rt = sqrt(p_star > b.p ? qb : 1.0);
+    const double s_m = u_a - a.c * qa_rt;
+    const double s_p = u_b + b.c * qb_rt;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) {
+      double fs = (s_p * fa[f] - s_m * fb[f] + s_p * s_m * (b.q[f] - a.q[f])) /
+                  (s_p - s_m);
+      if (s_m >= 0.0) fs = fa[f];
+      if (s_p <= 0.0) fs = fb[f];
+      out[f] = fs;
+    }
+  } else {
+    const double lam = kind == 0 ? fmax(a.speed + a.c, b.speed + b.c) : 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) out[f] = (fa[f] + fb[f] + lam * (a.q[f] - b.q[f])) / 2;
+  }
+}
+
+// Viscous flux F_V[f][d] (viscous.py:232-293). gq[f][d] = grad of cv field
+// f, gt[d] = grad T.
+template <int DIM, int NSP, bool MIX>
+__device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
+                                             const double (*gq)[DIM],
+                                             const double* gt, double (*FV)[DIM]) {
+  const double rho = s.q[0];
+  double gv[DIM][DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) gv[i][j] = (gq[2 + i][j] - s.v[i] * gq[0][j]) / rho;
+  double divv = 0.0;
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) divv += gv[i][i];
+  const double lam = (s.mu_b - 2 * s.mu / 3) * divv;
+  double tau[DIM][DIM];
+#pragma unroll
+  for (int i = 0; i < DIM; ++i)
+#pragma unroll
+    for (int j = 0; j < DIM; ++j)
+      tau[i][j] = s.mu * (gv[i][j] + gv[j][i]) + (i == j ? lam : 0.0);
+  double jd[Node<DIM, NSP>::NS1][DIM];
+  if constexpr (NSP > 0) {
+    double gy[NSP][DIM];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d)
+        gy[k][d] = (gq[2 + DIM + k][d] - s.y[k] * gq[0][d]) / rho;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) {
+      double sum = 0.0;
+      if constexpr (MIX) {
+#pragma unroll
+        for (int k = 0; k < NSP; ++k) sum += s.d[k] * gy[k][d];
+      }
+#pragma unroll
+      for (int k = 0; k < NSP; ++k)
+        jd[k][d] = -rho * (s.d[k] * gy[k][d] - s.y[k] * sum);
+    }
+  }
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    FV[0][d] = 0.0;
+    double tv = 0.0;
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) tv += tau[d][j] * s.v[j];
+    double qd = 0.0;
+    if constexpr (MIX) {
+#pragma unroll
+      for (int k = 0; k < NSP; ++k) qd += s.hk[k] * jd[k][d];
+    }
+    FV[1][d] = tv - qd + s.kappa * gt[d];
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) FV[2 + i][d] = tau[i][d];
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) FV[2 + DIM + k][d] = -jd[k][d];
+  }
+}
+
+}  // namespace mgdg

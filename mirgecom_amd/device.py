@@ -1,0 +1,79 @@
+"""HBM-resident copy of a discretization and its ``mg_mesh`` descriptor."""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from mirgecom_amd import _lib
+from mirgecom_amd.discretization import BTAG_ALL, Discretization
+
+
+class DeviceDiscretization:
+    """Geometry, connectivity and reference operators uploaded once.
+
+    Besides the kernel inputs it keeps the node coordinates and, for every
+    boundary tag, the boundary-node indices and coordinates that boundary
+    callbacks need."""
+
+    def __init__(self, dcoll: Discretization, device):
+        self.host = dcoll
+        self.device = torch.device(device)
+        f64 = dict(dtype=torch.float64, device=self.device)
+        re = dcoll.refel
+
+        def t64(a):
+            return torch.as_tensor(np.ascontiguousarray(a), **f64)
+
+        def t32(a):
+            return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32),
+                                   device=self.device)
+
+        self.rx = t64(dcoll.rx)
+        self.normals = t64(dcoll.normals)
+        self.fscale = t64(dcoll.fscale)
+        self.vmap_p = t32(dcoll.vmap_p)
+        self.weak_diff = t64(re.weak_diff)
+        self.lift = t64(re.lift)
+        self.face_mask = t32(re.face_mask)
+        self.nodes = t64(dcoll.host_nodes)
+        nfp = re.nface_nodes
+        bf = dcoll.bnd_faces
+        # flat volume node index of every boundary node
+        if len(bf):
+            vol_idx = (bf[:, 0:1] * re.nnodes + re.face_mask[bf[:, 1]]).reshape(-1)
+        else:
+            vol_idx = np.zeros(0, dtype=np.int64)
+        self.bnd_vol_index = torch.as_tensor(vol_idx, dtype=torch.int64,
+                                             device=self.device)
+        self.bnd_nodes = t64(dcoll.bnd_nodes)
+        self.bnd_normals = t64(dcoll.bnd_normals)
+        self._tag_index = {}
+        self.mesh = _lib.MgMesh(
+            dim=dcoll.dim, order=re.order, np=re.nnodes, nfp=nfp,
+            nfaces=re.nfaces, pad0=0, nelements=dcoll.nelements,
+            nbnd_nodes=dcoll.nbnd_nodes,
+            rx=self.rx.data_ptr(), normals=self.normals.data_ptr(),
+            fscale=self.fscale.data_ptr(), vmap_p=self.vmap_p.data_ptr(),
+            weak_diff=self.weak_diff.data_ptr(), lift=self.lift.data_ptr(),
+            face_mask=self.face_mask.data_ptr())
+
+    def boundary_index(self, tag):
+        """Indices (into the boundary-node arrays) of a boundary tag."""
+        if tag not in self._tag_index:
+            mask = self.host.boundary_node_slice_mask(tag)
+            self._tag_index[tag] = torch.as_tensor(
+                np.nonzero(mask)[0], dtype=torch.int64, device=self.device)
+        return self._tag_index[tag]
+
+
+def get_device_discretization(dcoll: Discretization, device=None):
+    if device is None:
+        device = torch.device("cuda", torch.cuda.current_device())
+    key = str(torch.device(device))
+    if key not in dcoll._device_cache:
+        dcoll._device_cache[key] = DeviceDiscretization(dcoll, device)
+    return dcoll._device_cache[key]
+
+
+__all__ = ["DeviceDiscretization", "get_device_discretization", "BTAG_ALL"]

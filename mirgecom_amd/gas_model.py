@@ -1,0 +1,183 @@
+"""Gas model and model-consistent fluid states.
+
+Call surface of mirgecom/gas_model.py:82-491: ``GasModel(eos, transport)``,
+``FluidState(cv, dv)``, ``ViscousFluidState(cv, dv, tv)`` and
+``make_fluid_state(cv, gas_model, temperature_seed=None, ...)``.
+
+The dependent variables are evaluated lazily: the DG kernels recompute
+pressure, sound speed and transport coefficients in registers from the
+conserved state (and, for mixtures, the temperature), so only the mixture
+temperature -- the FP64 Newton solve ``mg_mixture_temperature`` -- is computed
+eagerly.  The other fields are produced on first access with the reference's
+formulas.
+"""
+
+from __future__ import annotations
+
+import torch
+
+from mirgecom_amd import _lib
+
+
+class GasModel:
+    """mirgecom/gas_model.py:82-97"""
+
+    def __init__(self, eos, transport=None):
+        self.eos = eos
+        self.transport = transport
+
+    def device_params(self, nspecies: int) -> "_lib.MgGas":
+        """The ``mg_gas`` struct of this model for a state with ``nspecies``."""
+        eos = self.eos
+        g = _lib.MgGas()
+        g.nspecies = nspecies
+        if getattr(eos, "is_mixture", False):
+            if nspecies != 7:
+                raise ValueError("the uiuc_7sp mixture has 7 species")
+            g.eos_kind = 1
+        else:
+            g.eos_kind = 0
+            g.gamma = eos.gamma()
+            g.gas_const = eos.gas_const()
+        if nspecies > 8:
+            raise NotImplementedError("at most 8 species")
+        if self.transport is not None:
+            prm = self.transport.device_params(nspecies)
+            diff = prm.pop("diff")
+            for k, v in prm.items():
+                setattr(g, k, v)
+            for k in range(8):
+                g.species_diffusivity[k] = float(diff[k])
+        return g
+
+
+class _LazyDV:
+    """Dependent variables (mirgecom/eos.py:66-100) computed on access."""
+
+    def __init__(self, cv, eos, temperature=None):
+        self._cv = cv
+        self._eos = eos
+        self._t = temperature
+        self._cache = {}
+
+    def _get(self, name, fn):
+        if name not in self._cache:
+            self._cache[name] = fn()
+        return self._cache[name]
+
+    @property
+    def temperature(self):
+        if self._t is None:
+            self._t = self._eos.temperature(self._cv)
+        return self._t
+
+    @property
+    def pressure(self):
+        return self._get("p", lambda: self._eos.pressure(self._cv, self.temperature))
+
+    @property
+    def speed_of_sound(self):
+        return self._get("c", lambda: self._eos.sound_speed(self._cv,
+                                                            self.temperature))
+
+    @property
+    def species_enthalpies(self):
+        if not getattr(self._eos, "is_mixture", False):
+            raise AttributeError("species enthalpies need a mixture EOS")
+        return self._get("h", lambda: self._eos.species_enthalpies(
+            self._cv, self.temperature))
+
+    @property
+    def smoothness_mu(self):
+        return self._get("smu", lambda: torch.zeros_like(self._cv.mass))
+
+    smoothness_kappa = smoothness_mu
+    smoothness_d = smoothness_mu
+    smoothness_beta = smoothness_mu
+
+
+class FluidState:
+    """mirgecom/gas_model.py:100-248"""
+
+    def __init__(self, cv, dv, gas_model=None):
+        self.cv = cv
+        self.dv = dv
+        self.gas_model = gas_model
+
+    dim = property(lambda self: self.cv.dim)
+    nspecies = property(lambda self: self.cv.nspecies)
+    pressure = property(lambda self: self.dv.pressure)
+    temperature = property(lambda self: self.dv.temperature)
+    mass_density = property(lambda self: self.cv.mass)
+    momentum_density = property(lambda self: self.cv.momentum)
+    energy_density = property(lambda self: self.cv.energy)
+    species_mass_density = property(lambda self: self.cv.species_mass)
+    velocity = property(lambda self: self.cv.velocity)
+    speed = property(lambda self: self.cv.speed)
+    species_mass_fractions = property(lambda self: self.cv.species_mass_fractions)
+    speed_of_sound = property(lambda self: self.dv.speed_of_sound)
+    species_enthalpies = property(lambda self: self.dv.species_enthalpies)
+
+    @property
+    def wavespeed(self):
+        """mirgecom/gas_model.py:223-226"""
+        return self.cv.speed + self.dv.speed_of_sound
+
+    @property
+    def is_viscous(self):
+        return False
+
+    @property
+    def is_mixture(self):
+        return bool(getattr(self.gas_model.eos, "is_mixture", False)) \
+            if self.gas_model is not None else False
+
+
+class ViscousFluidState(FluidState):
+    """mirgecom/gas_model.py:251-286"""
+
+    def __init__(self, cv, dv, gas_model):
+        super().__init__(cv, dv, gas_model)
+        self._tv = None
+
+    @property
+    def tv(self):
+        if self._tv is None:
+            self._tv = self.gas_model.transport.transport_vars(
+                self.cv, self.dv, self.gas_model.eos)
+        return self._tv
+
+    viscosity = property(lambda self: self.tv.viscosity)
+    bulk_viscosity = property(lambda self: self.tv.bulk_viscosity)
+    thermal_conductivity = property(lambda self: self.tv.thermal_conductivity)
+    species_diffusivity = property(lambda self: self.tv.species_diffusivity)
+
+    @property
+    def is_viscous(self):
+        return True
+
+
+def make_fluid_state(cv, gas_model, temperature_seed=None, smoothness_mu=None,
+                     smoothness_kappa=None, smoothness_d=None,
+                     smoothness_beta=None, entropy_min=None,
+                     material_densities=None, limiter_func=None,
+                     limiter_dd=None):
+    """mirgecom/gas_model.py:302-431 (GasModel branch).
+
+    Artificial-viscosity smoothness fields, limiters and porous media are
+    outside this build and raise ``NotImplementedError`` when given."""
+    if not isinstance(gas_model, GasModel):
+        raise NotImplementedError("only GasModel (no porous media)")
+    if limiter_func is not None:
+        raise NotImplementedError("limiter_func is not supported")
+    if any(x is not None for x in (smoothness_mu, smoothness_kappa,
+                                   smoothness_d, smoothness_beta)):
+        raise NotImplementedError("artificial-viscosity fields not supported")
+    temperature = None
+    if getattr(gas_model.eos, "is_mixture", False):
+        temperature = gas_model.eos.temperature(cv, temperature_seed)
+    dv = _LazyDV(cv, gas_model.eos, temperature)
+    if gas_model.transport is not None:
+        return ViscousFluidState(cv, dv, gas_model)
+    return FluidState(cv, dv, gas_model)
+

@@ -1,0 +1,88 @@
+"""Inviscid fluxes (call surface of mirgecom/inviscid.py:65-376).
+
+``inviscid_flux`` and the facial fluxes are tensor functions of fluid states
+(with a TracePair-like ``state_pair.int``/``state_pair.ext``); inside
+``euler_operator``/``ns_operator`` the same fluxes are evaluated by the HIP
+kernels, selected by identity of the function passed in."""
+
+from __future__ import annotations
+
+import torch
+
+from mirgecom_amd import _lib
+from mirgecom_amd.flux import num_flux_hll, num_flux_lfr
+from mirgecom_amd.fluid import ConservedVars
+
+
+def inviscid_flux(state):
+    """mirgecom/inviscid.py:65-102: ``(F, dim, ...)`` as ConservedVars."""
+    dim = state.dim
+    v = state.velocity
+    p = state.pressure
+    rows = [state.momentum_density, v * (state.energy_density + p)]
+    rho = state.mass_density
+    for i in range(dim):
+        rows.append(torch.stack([rho * (v[i] * v[j]) + (p if i == j else 0 * p)
+                                 for j in range(dim)]))
+    for k in range(state.nspecies):
+        rows.append(v * state.species_mass_density[k])
+    return ConservedVars(dim, torch.stack(rows))
+
+
+def _fn(flux, normal):
+    return torch.einsum("fi...,i...->f...", flux.q, normal)
+
+
+def inviscid_facial_flux_rusanov(state_pair, gas_model, normal):
+    """mirgecom/inviscid.py:105-152"""
+    s_i, s_e = state_pair.int, state_pair.ext
+    lam = torch.maximum(s_i.wavespeed, s_e.wavespeed)
+    return ConservedVars(s_i.dim, num_flux_lfr(
+        _fn(inviscid_flux(s_i), normal), _fn(inviscid_flux(s_e), normal),
+        s_i.cv.q, s_e.cv.q, lam))
+
+
+def inviscid_facial_flux_central(state_pair, gas_model, normal):
+    """mirgecom/inviscid.py:155-199"""
+    s_i, s_e = state_pair.int, state_pair.ext
+    return ConservedVars(s_i.dim, num_flux_lfr(
+        _fn(inviscid_flux(s_i), normal), _fn(inviscid_flux(s_e), normal),
+        s_i.cv.q, s_e.cv.q, 0))
+
+
+def inviscid_facial_flux_hll(state_pair, gas_model, normal):
+    """mirgecom/inviscid.py:202-287"""
+    s_i, s_e = state_pair.int, state_pair.ext
+    p_i, p_e = s_i.pressure, s_e.pressure
+    u_i = torch.einsum("i...,i...->...", s_i.velocity, normal)
+    u_e = torch.einsum("i...,i...->...", s_e.velocity, normal)
+    c_i, c_e = s_i.speed_of_sound, s_e.speed_of_sound
+    p_star = (0.5 * (p_i + p_e) + (1. / 8.) * (u_i - u_e)
+              * (s_i.mass_density + s_e.mass_density) * (c_i + c_e))
+    g_i = gas_model.eos.gamma(s_i.cv, s_i.temperature)
+    g_e = gas_model.eos.gamma(s_e.cv, s_e.temperature)
+    q_i = 1 + (g_i + 1) / (2 * g_i) * (p_star / p_i - 1)
+    q_e = 1 + (g_e + 1) / (2 * g_e) * (p_star / p_e - 1)
+    one = torch.ones_like(p_i)
+    s_minus = u_i - c_i * torch.sqrt(torch.where(p_star > p_i, q_i, one))
+    s_plus = u_e + c_e * torch.sqrt(torch.where(p_star > p_e, q_e, one))
+    return ConservedVars(s_i.dim, num_flux_hll(
+        _fn(inviscid_flux(s_i), normal), _fn(inviscid_flux(s_e), normal),
+        s_i.cv.q, s_e.cv.q, s_minus, s_plus))
+
+
+_FLUX_KIND = {
+    inviscid_facial_flux_rusanov: _lib.MG_FLUX_RUSANOV,
+    inviscid_facial_flux_hll: _lib.MG_FLUX_HLL,
+    inviscid_facial_flux_central: _lib.MG_FLUX_CENTRAL,
+}
+
+
+def flux_kind(numerical_flux_func):
+    """Device flux selector of a facial flux function."""
+    try:
+        return _FLUX_KIND[numerical_flux_func]
+    except (KeyError, TypeError):
+        raise NotImplementedError(
+            f"numerical flux {numerical_flux_func!r} has no device kernel; "
+            "use inviscid_facial_flux_rusanov, _hll or _central") from None

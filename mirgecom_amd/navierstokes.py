@@ -1,0 +1,111 @@
+"""``ns_operator``, ``grad_cv_operator``, ``grad_t_operator`` on the GPU.
+
+Call surface of mirgecom/navierstokes.py:129-575.  One RHS is two HIP
+launches: ``mg_ns_gradient`` (the DG gradients of all conserved fields and
+of T, central flux inside, exterior value on boundaries) and ``mg_ns_rhs``
+(viscous and inviscid volume fluxes, face fluxes with the gathered
+neighbour traces and gradients, weak divergence, lift, inverse mass)."""
+
+from __future__ import annotations
+
+import torch
+
+from mirgecom_amd import _lib
+from mirgecom_amd._opcommon import (
+    Prepared, check_dd, check_unsupported, warn_mixture_limiter)
+from mirgecom_amd.dof_desc import DD_VOLUME_ALL, DISCR_TAG_BASE
+from mirgecom_amd.flux import num_flux_central
+from mirgecom_amd.fluid import ConservedVars
+from mirgecom_amd.inviscid import flux_kind, inviscid_facial_flux_rusanov
+from mirgecom_amd.viscous import viscous_facial_flux_central
+
+
+def _check_grad_flux(numerical_flux_func):
+    if numerical_flux_func is not num_flux_central:
+        raise NotImplementedError("the device gradient uses num_flux_central")
+
+
+def _gradients(dcoll, gas_model, boundaries, state, time):
+    prep = Prepared(dcoll, state, gas_model, boundaries, time)
+    nf = prep.q.shape[0]
+    grad = torch.empty((nf + 1, prep.dim) + tuple(prep.q.shape[1:]),
+                       dtype=prep.q.dtype, device=prep.q.device)
+    _lib.call("mg_ns_gradient", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+              _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb),
+              _lib.ptr(grad), prep.stream)
+    return prep, grad
+
+
+def grad_cv_operator(dcoll, gas_model, boundaries, state, *, time=0.0,
+                     numerical_flux_func=num_flux_central,
+                     quadrature_tag=DISCR_TAG_BASE, dd=DD_VOLUME_ALL,
+                     comm_tag=None, operator_states_quad=None,
+                     limiter_func=None, entropy_min=None, use_esdg=False):
+    """mirgecom/navierstokes.py:129-235: ConservedVars of ``(F, dim, K, Np)``."""
+    check_dd(dd)
+    check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
+                      limiter_func, entropy_min)
+    _check_grad_flux(numerical_flux_func)
+    _, grad = _gradients(dcoll, gas_model, boundaries, state, time)
+    return ConservedVars(state.dim, grad[:-1])
+
+
+def grad_t_operator(dcoll, gas_model, boundaries, state, *, time=0.0,
+                    numerical_flux_func=num_flux_central,
+                    quadrature_tag=DISCR_TAG_BASE, dd=DD_VOLUME_ALL,
+                    comm_tag=None, operator_states_quad=None,
+                    limiter_func=None, entropy_min=None, use_esdg=False):
+    """mirgecom/navierstokes.py:238-347: ``(dim, K, Np)``."""
+    check_dd(dd)
+    check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
+                      limiter_func, entropy_min)
+    _check_grad_flux(numerical_flux_func)
+    _, grad = _gradients(dcoll, gas_model, boundaries, state, time)
+    return grad[-1]
+
+
+def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
+                inviscid_fluid_operator=None,
+                inviscid_numerical_flux_func=inviscid_facial_flux_rusanov,
+                gradient_numerical_flux_func=num_flux_central,
+                viscous_numerical_flux_func=viscous_facial_flux_central,
+                return_gradients=False, quadrature_tag=DISCR_TAG_BASE,
+                dd=DD_VOLUME_ALL, comm_tag=None, limiter_func=None,
+                entropy_min=None, operator_states_quad=None, use_esdg=False,
+                grad_cv=None, grad_t=None, inviscid_terms_on=True,
+                entropy_conserving_flux_func=None):
+    r"""RHS of the Navier-Stokes equations,
+    :math:`\partial_t Q = \nabla\cdot(F_V - F_I)` (mirgecom/navierstokes.py:
+    350-575, fused inviscid path)."""
+    if not state.is_viscous:
+        raise ValueError("Navier-Stokes operator expects viscous gas model.")
+    check_dd(dd)
+    check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
+                      limiter_func, entropy_min)
+    if inviscid_fluid_operator is not None:
+        raise NotImplementedError("a separate inviscid operator is not "
+                                  "supported; the inviscid terms are fused")
+    if viscous_numerical_flux_func is not viscous_facial_flux_central:
+        raise NotImplementedError("the device path uses "
+                                  "viscous_facial_flux_central")
+    _check_grad_flux(gradient_numerical_flux_func)
+    warn_mixture_limiter(state, limiter_func)
+    kind = flux_kind(inviscid_numerical_flux_func)
+    if grad_cv is None or grad_t is None:
+        prep, grad = _gradients(dcoll, gas_model, boundaries, state, time)
+        if grad_cv is not None:
+            grad[:-1] = grad_cv.q
+        if grad_t is not None:
+            grad[-1] = grad_t
+    else:
+        prep = Prepared(dcoll, state, gas_model, boundaries, time)
+        grad = torch.cat([grad_cv.q, grad_t[None]], dim=0).contiguous()
+    rhs = torch.empty_like(prep.q)
+    _lib.call("mg_ns_rhs", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+              _lib.ptr(prep.temp), _lib.ptr(grad), _lib.ptr(prep.qb),
+              _lib.ptr(prep.tb), kind, 1 if inviscid_terms_on else 0,
+              _lib.ptr(rhs), prep.stream)
+    out = ConservedVars(state.dim, rhs)
+    if return_gradients:
+        return out, ConservedVars(state.dim, grad[:-1]), grad[-1]
+    return out

@@ -1,0 +1,279 @@
+"""Oracle DG operators: Euler and Navier-Stokes right-hand sides.
+
+Restated from the reference in NumPy FP64:
+
+* ``div_operator`` / ``grad_operator`` -- mirgecom/operators.py:34-95:
+  ``-M^-1 (weak_local_{div,grad}(u) - face_mass(flux))``
+* ``euler_operator`` -- mirgecom/euler.py:313-416 (non-ESDG path):
+  ``-div_operator(inviscid_flux(vol), inviscid_flux_on_element_boundary)``
+* ``inviscid_flux_on_element_boundary`` -- mirgecom/inviscid.py:290-376
+* ``make_operator_fluid_states`` -- mirgecom/gas_model.py:728-904: the EOS is
+  re-evaluated on both sides of every face from the traced conserved state
+  (mixtures seed the Newton solve with the traced temperature)
+* ``grad_cv_operator`` / ``grad_t_operator`` -- mirgecom/navierstokes.py:
+  118-347 (central flux on interior faces, boundary hooks otherwise)
+* ``ns_operator`` -- mirgecom/navierstokes.py:350-575 (fused default path:
+  ``div_operator(F_V - F_I, f_V* - f_I*)``)
+* ``PrescribedFluidBoundary`` / ``DummyBoundary`` -- mirgecom/boundary.py:
+  796-1049 (gradient flux takes the exterior value, viscous flux is central
+  with identical gradients)
+
+Parity: pinned by the reference's invariant and convergence tests restated in
+tests/ (uniform state => RHS = 0, test/test_euler.py:80-232 and
+test/test_navierstokes.py:81-235; isentropic-vortex convergence,
+test/test_euler.py:405-480 and :816-887; polynomial exactness of the gradient,
+test/test_operators.py:237-416).  grudge's floating-point association order is
+not reproducible here (parity unpinned at the ulp level).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from oracle import physics as ph
+
+
+# {{{ boundaries
+
+class DummyBoundary:
+    """mirgecom/boundary.py:1012-1049"""
+
+    kind = "dummy"
+
+
+class PrescribedFluidBoundary:
+    """mirgecom/boundary.py:796-1009 with a user ``boundary_state_func``
+    ``f(x_bnd, state_minus, time) -> State``."""
+
+    kind = "prescribed"
+
+    def __init__(self, boundary_state_func):
+        self.state_func = boundary_state_func
+
+
+def _face_tags(disc, box):
+    a, b = box[0], box[1]
+    fx = disc.face_x.mean(axis=2)            # (K, Nf, d)
+    tags = np.full(fx.shape[:2], "", dtype=object)
+    scale = max(b[i] - a[i] for i in range(disc.dim))
+    for i in range(disc.dim):
+        lo = np.abs(fx[..., i] - a[i]) < 1e-9 * scale
+        hi = np.abs(fx[..., i] - b[i]) < 1e-9 * scale
+        tags[lo & (tags == "") & disc.is_bnd[..., 0]] = f"-{i + 1}"
+        tags[hi & (tags == "") & disc.is_bnd[..., 0]] = f"+{i + 1}"
+    return tags
+
+
+def _boundary_groups(disc, box, boundaries):
+    """List of (boundary, face-node flat indices) for every boundary."""
+    tags = _face_tags(disc, box)
+    nfp = disc.el.nfp
+    groups = []
+    covered = np.zeros(disc.is_bnd.shape[:2], dtype=bool)
+    for tag, bdry in boundaries.items():
+        if tag == "all":
+            fmask = disc.is_bnd[..., 0]
+        else:
+            fmask = (tags == tag) & disc.is_bnd[..., 0]
+        if np.any(covered & fmask):
+            raise ValueError("boundary faces covered twice")
+        covered |= fmask
+        faces = np.argwhere(fmask)
+        idx = ((faces[:, 0] * disc.nf + faces[:, 1])[:, None] * nfp
+               + np.arange(nfp)[None]).reshape(-1)
+        groups.append((bdry, idx))
+    if np.any(disc.is_bnd[..., 0] & ~covered):
+        raise ValueError("boundary faces without a boundary condition")
+    return groups
+
+# }}}
+
+
+def _flat(a):
+    """``(..., K, Nf, Nfp)`` -> ``(..., K*Nf*Nfp)``"""
+    return a.reshape(a.shape[:-3] + (-1,))
+
+
+def _sub_state(state: ph.State, idx):
+    tv = None
+    if state.tv is not None:
+        tv = ph.TransportVars(
+            bulk_viscosity=state.tv.bulk_viscosity[..., idx],
+            viscosity=state.tv.viscosity[..., idx],
+            thermal_conductivity=state.tv.thermal_conductivity[..., idx],
+            species_diffusivity=state.tv.species_diffusivity[..., idx])
+    return ph.State(
+        cv=state.cv[..., idx], dim=state.dim,
+        temperature=state.temperature[..., idx],
+        pressure=state.pressure[..., idx],
+        speed_of_sound=state.speed_of_sound[..., idx],
+        species_enthalpies=(None if state.species_enthalpies is None
+                            else state.species_enthalpies[..., idx]),
+        tv=tv)
+
+
+class OperatorStates:
+    """Volume state and face-node states on both sides of every face (flat
+    face-node axis); mirgecom/gas_model.py:728-904."""
+
+    def __init__(self, disc, box, state, gas_model, boundaries, time):
+        self.disc = disc
+        self.vol = state
+        dim = state.dim
+        cv_m = _flat(disc.trace(state.cv))
+        t_m = _flat(disc.trace(state.temperature[None]))[0]
+        self.minus = ph.make_fluid_state(cv_m, gas_model, dim,
+                                         temperature_seed=t_m)
+        cv_p = _flat(disc.exterior(state.cv))
+        t_p = _flat(disc.exterior(state.temperature[None]))[0]
+        self.interior = np.nonzero(~disc.is_bnd.reshape(-1))[0]
+        self.plus_int = ph.make_fluid_state(
+            cv_p[:, self.interior], gas_model, dim,
+            temperature_seed=t_p[self.interior])
+        self.normal = np.moveaxis(
+            np.repeat(disc.normal[:, :, None, :], disc.el.nfp, axis=2)
+            .reshape(-1, dim), 1, 0)                  # (dim, Nfn)
+        self.x = np.moveaxis(disc.face_x.reshape(-1, dim), 1, 0)
+        self.groups = []
+        for bdry, idx in _boundary_groups(disc, box, boundaries):
+            sm = _sub_state(self.minus, idx)
+            ext = None
+            if bdry.kind == "prescribed":
+                ext = bdry.state_func(self.x[:, idx], sm, time)
+            self.groups.append((bdry, idx, sm, ext))
+        self.nfn = cv_m.shape[-1]
+
+    def unflat(self, a):
+        d = self.disc
+        return a.reshape(a.shape[:-1] + (d.K, d.nf, d.el.nfp))
+
+
+def inviscid_flux_on_element_boundary(ops: OperatorStates, gas_model,
+                                      numerical_flux_func):
+    """mirgecom/inviscid.py:290-376 -> ``(F, Nfn)``."""
+    nf = ops.minus.cv.shape[0]
+    out = np.zeros((nf, ops.nfn))
+    ii = ops.interior
+    out[:, ii] = numerical_flux_func(_sub_state(ops.minus, ii), ops.plus_int,
+                                     gas_model, ops.normal[:, ii])
+    for bdry, idx, sm, ext in ops.groups:
+        n = ops.normal[:, idx]
+        if bdry.kind == "dummy":
+            out[:, idx] = ph._dot_normal(ph.inviscid_flux(sm), n)
+        else:
+            out[:, idx] = numerical_flux_func(sm, ext, gas_model, n)
+    return out
+
+
+def div_operator(disc, vol_flux, face_flux):
+    """``-M^-1 (weak_local_div(v) - face_mass(flux))``; ``vol_flux`` is
+    ``(F, dim, K, Np)`` and ``face_flux`` ``(F, K, Nf, Nfp)``."""
+    v = np.moveaxis(vol_flux, 1, 0)
+    return -disc.inverse_mass(disc.weak_local_div(v) - disc.face_mass(face_flux))
+
+
+def grad_operator(disc, u, face_flux):
+    """``-M^-1 (weak_local_grad(u) - face_mass(flux))``; ``u`` is
+    ``(..., K, Np)``, ``face_flux`` ``(..., dim, K, Nf, Nfp)``; returns
+    ``(..., dim, K, Np)``."""
+    wg = np.moveaxis(disc.weak_local_grad(u), 0, -3)     # (..., dim, K, Np)
+    return -disc.inverse_mass(wg - disc.face_mass(face_flux))
+
+
+def euler_operator(disc, box, state, gas_model, boundaries, time=0.0,
+                   numerical_flux_func=ph.inviscid_facial_flux_rusanov):
+    """mirgecom/euler.py:313-416"""
+    ops = OperatorStates(disc, box, state, gas_model, boundaries, time)
+    vol = ph.inviscid_flux(state)
+    bnd = ops.unflat(inviscid_flux_on_element_boundary(
+        ops, gas_model, numerical_flux_func))
+    return -div_operator(disc, vol, bnd)
+
+
+def _gradient_face_flux(ops: OperatorStates, u_m, u_p_int, bnd_values):
+    """Central flux on interior faces, boundary value on boundary faces,
+    times the normal: returns ``(..., dim, Nfn)``."""
+    shp = u_m.shape[:-1]
+    dim = ops.normal.shape[0]
+    out = np.zeros(shp + (dim, ops.nfn))
+    ii = ops.interior
+    cen = ph.num_flux_central(u_m[..., ii], u_p_int)
+    out[..., ii] = cen[..., None, :] * ops.normal[:, ii]
+    for (bdry, idx, sm, ext), bval in zip(ops.groups, bnd_values):
+        out[..., idx] = bval[..., None, :] * ops.normal[:, idx]
+    return out
+
+
+def grad_cv_operator(ops: OperatorStates):
+    """mirgecom/navierstokes.py:129-235 -> ``(F, dim, K, Np)``."""
+    bvals = []
+    for bdry, idx, sm, ext in ops.groups:
+        # Prescribed: exterior value (boundary.py:70-72, 895-908);
+        # Dummy: interior value (boundary.py:1031-1035)
+        bvals.append(sm.cv if bdry.kind == "dummy" else ext.cv)
+    flux = _gradient_face_flux(ops, ops.minus.cv, ops.plus_int.cv, bvals)
+    return grad_operator(ops.disc, ops.vol.cv, ops.unflat(flux))
+
+
+def grad_t_operator(ops: OperatorStates):
+    """mirgecom/navierstokes.py:238-347 -> ``(dim, K, Np)``."""
+    bvals = []
+    for bdry, idx, sm, ext in ops.groups:
+        bvals.append(sm.temperature if bdry.kind == "dummy"
+                     else ext.temperature)
+    flux = _gradient_face_flux(ops, ops.minus.temperature,
+                               ops.plus_int.temperature, bvals)
+    return grad_operator(ops.disc, ops.vol.temperature, ops.unflat(flux))
+
+
+def viscous_flux_on_element_boundary(ops: OperatorStates, grad_cv, grad_t):
+    """mirgecom/viscous.py:431-544 with the central facial flux."""
+    disc = ops.disc
+    gcv_m = _flat(disc.trace(grad_cv))            # (F, dim, Nfn)
+    gcv_p = _flat(disc.exterior(grad_cv))
+    gt_m = _flat(disc.trace(grad_t))              # (dim, Nfn)
+    gt_p = _flat(disc.exterior(grad_t))
+    nf = grad_cv.shape[0]
+    out = np.zeros((nf, ops.nfn))
+    ii = ops.interior
+    out[:, ii] = ph.viscous_facial_flux_central(
+        _sub_state(ops.minus, ii), ops.plus_int, gcv_m[..., ii], gcv_p[..., ii],
+        gt_m[..., ii], gt_p[..., ii], ops.normal[:, ii])
+    for bdry, idx, sm, ext in ops.groups:
+        n = ops.normal[:, idx]
+        if bdry.kind == "dummy":
+            out[:, idx] = ph._dot_normal(
+                ph.viscous_flux(sm, gcv_m[..., idx], gt_m[..., idx]), n)
+        else:
+            out[:, idx] = ph.viscous_facial_flux_central(
+                sm, ext, gcv_m[..., idx], gcv_m[..., idx], gt_m[..., idx],
+                gt_m[..., idx], n)
+    return out
+
+
+def ns_operator(disc, box, state, gas_model, boundaries, time=0.0,
+                inviscid_numerical_flux_func=ph.inviscid_facial_flux_rusanov,
+                return_gradients=False, inviscid_terms_on=True):
+    """mirgecom/navierstokes.py:350-575 (fused inviscid path)."""
+    ops = OperatorStates(disc, box, state, gas_model, boundaries, time)
+    grad_cv = grad_cv_operator(ops)
+    grad_t = grad_t_operator(ops)
+    vol = ph.viscous_flux(state, grad_cv, grad_t)
+    bnd = viscous_flux_on_element_boundary(ops, grad_cv, grad_t)
+    if inviscid_terms_on:
+        vol = vol - ph.inviscid_flux(state)
+        bnd = bnd - inviscid_flux_on_element_boundary(
+            ops, gas_model, inviscid_numerical_flux_func)
+    rhs = div_operator(disc, vol, ops.unflat(bnd))
+    if return_gradients:
+        return rhs, grad_cv, grad_t
+    return rhs
+
+
+def rk4_step(state, t, dt, rhs):
+    """mirgecom/integrators/explicit_rk.py:31-38"""
+    k1 = rhs(t, state)
+    k2 = rhs(t + dt / 2, state + dt / 2 * k1)
+    k3 = rhs(t + dt / 2, state + dt / 2 * k2)
+    k4 = rhs(t + dt, state + dt * k3)
+    return state + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)

@@ -1,0 +1,225 @@
+"""Oracle uiuc_7sp thermochemistry (NumPy FP64).
+
+Restates what the reference obtains from pyrometheus' code-generated
+mechanism class (pyrometheus @ tulio/entropy, generated from Cantera 3.0.0
+for mirgecom/mechanisms/uiuc_7sp.yaml) plus mirgecom's PyroWrapper overrides
+(mirgecom/thermochemistry.py:87-167) and PyrometheusMixture
+(mirgecom/eos.py:540-888):
+
+* NASA7 ``cp/R``, ``h/RT``, ``s/R`` with the range switch at ``T_mid``;
+* mixture ``cp``, ``cv``, ``e``, ``R`` on a mass basis, ``p = rho R_mix T``;
+* ``get_temperature``: fixed-iteration Newton ``T += (e - e(T))/cv(T)``;
+* ``get_concentrations``: ``rho Y_k / W_k`` pinned to 0 below ``zero_level``;
+* Arrhenius forward rates in SI (kmol, m^3) with the yaml's explicit orders,
+  ``1/K_c = exp(dG/RT - dnu ln(p0/RT))`` for the reversible R2, and
+  ``omega_k = sum_r nu_kr r_r``.
+
+The raw numbers (NASA7 coefficients, A, b, Ea in cm/mol/cal, compositions,
+element masses) come from ``mirgecom_amd.mechanisms``' transcription of the
+yaml (checked against the yaml in tests/test_mechanism_tables.py); the unit
+conversions and formulas below are the oracle's own.
+
+Parity: no Cantera/pyrometheus here, so against the reference this is pinned
+only through closed forms and invariants (tests/test_oracle_thermochem.py):
+NASA7 self-consistency, Newton round trips, mass and element conservation,
+detailed balance of R2, zero-level pinning and 0-D autoignition bounds
+(test/test_chemistry.py:258-260).  The exact pyrometheus FP sequence is
+parity unpinned.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from mirgecom_amd import mechanisms as raw
+
+R_U = 8314.46261815324          # J/(kmol K)
+P_ATM = 101325.0
+SPECIES = list(raw.SPECIES)
+NSP = len(SPECIES)
+
+W = np.array([sum(raw._ATOMIC[el] * n for el, n in raw._COMPOSITION[s].items())
+              for s in SPECIES])
+LO = np.array([raw._NASA7[s][0] for s in SPECIES])
+HI = np.array([raw._NASA7[s][1] for s in SPECIES])
+T_MID = 1000.0
+
+# stoichiometry: nu[k, r] (products positive)
+_IDX = {s: i for i, s in enumerate(SPECIES)}
+NU = np.zeros((NSP, 3))
+for sp, c in (("C2H4", -1), ("O2", -1), ("CO", 2), ("H2", 2)):
+    NU[_IDX[sp], 0] = c
+for sp, c in (("CO", -1), ("O2", -0.5), ("CO2", 1)):
+    NU[_IDX[sp], 1] = c
+for sp, c in (("H2", -1), ("O2", -0.5), ("H2O", 1)):
+    NU[_IDX[sp], 2] = c
+# forward orders
+ORD = np.zeros((NSP, 3))
+ORD[_IDX["C2H4"], 0], ORD[_IDX["O2"], 0] = 0.5, 0.65
+ORD[_IDX["CO"], 1], ORD[_IDX["O2"], 1] = 1.0, 0.5
+ORD[_IDX["H2"], 2], ORD[_IDX["O2"], 2] = 0.75, 0.5
+REVERSIBLE = [False, True, False]
+
+# Arrhenius in SI: concentration unit mol/cm^3 -> kmol/m^3 is a factor 1e3;
+# r[kmol/m^3/s] = 1e3 * A (C/1e3)^order  =>  A_SI = A * 1e3^(1 - order)
+_A_CGS = np.array([r[0] for r in raw._REACTIONS])
+B = np.array([r[1] for r in raw._REACTIONS])
+_EA_CAL = np.array([r[2] for r in raw._REACTIONS])
+A_SI = _A_CGS * 1e3**(1.0 - ORD.sum(axis=0))
+EA_R = _EA_CAL * 4.184 * 1e3 / R_U           # cal/mol -> J/kmol, / R
+
+
+def _a(t):
+    """Coefficient arrays ``a[i]`` of shape ``(NSP, ...)``."""
+    t = np.asarray(t, dtype=np.float64)
+    shp = (NSP,) + (1,) * t.ndim
+    high = (t > T_MID)[None]
+    return [np.where(high, HI[:, i].reshape(shp), LO[:, i].reshape(shp))
+            for i in range(7)]
+
+
+def cp_r(t):
+    a = _a(t)
+    return a[0] + a[1] * t + a[2] * t**2 + a[3] * t**3 + a[4] * t**4
+
+
+def h_rt(t):
+    a = _a(t)
+    return (a[0] + a[1] / 2 * t + a[2] / 3 * t**2 + a[3] / 4 * t**3
+            + a[4] / 5 * t**4 + a[5] / t)
+
+
+def s_r(t):
+    a = _a(t)
+    return (a[0] * np.log(t) + a[1] * t + a[2] / 2 * t**2 + a[3] / 3 * t**3
+            + a[4] / 4 * t**4 + a[6])
+
+
+def g_rt(t):
+    return h_rt(t) - s_r(t)
+
+
+def _w(t):
+    return W.reshape((NSP,) + (1,) * np.ndim(t))
+
+
+def specific_gas_constant(y):
+    return R_U * np.sum(y / _w(y[0]), axis=0)
+
+
+def mixture_cp_mass(t, y):
+    return R_U * np.sum(cp_r(t) * y / _w(t), axis=0)
+
+
+def mixture_cv_mass(t, y):
+    return R_U * np.sum((cp_r(t) - 1.0) * y / _w(t), axis=0)
+
+
+def mixture_energy_mass(t, y):
+    return R_U * t * np.sum((h_rt(t) - 1.0) * y / _w(t), axis=0)
+
+
+def mixture_enthalpy_mass(t, y):
+    return R_U * t * np.sum(h_rt(t) * y / _w(t), axis=0)
+
+
+def pressure(rho, t, y):
+    return rho * specific_gas_constant(y) * t
+
+
+def density(p, t, y):
+    return p / (specific_gas_constant(y) * t)
+
+
+def get_temperature(e, t_guess, y, niter=5):
+    """mirgecom/thermochemistry.py:126-167 (energy form)."""
+    t = np.asarray(t_guess, dtype=np.float64) * np.ones_like(e)
+    for _ in range(niter):
+        t = t + (e - mixture_energy_mass(t, y)) / mixture_cv_mass(t, y)
+    return t
+
+
+def concentrations(rho, y, zero_level=0.0):
+    """mirgecom/thermochemistry.py:87-94"""
+    c = rho * y / _w(rho)
+    return np.where(c < zero_level, 0.0, c)
+
+
+def fwd_rate_coefficients(t):
+    shp = (3,) + (1,) * np.ndim(t)
+    return A_SI.reshape(shp) * t**B.reshape(shp) * np.exp(
+        -EA_R.reshape(shp) / t)
+
+
+def equilibrium_constants(t):
+    """``K_c`` per reaction (NaN-free; irreversible ones unused)."""
+    dg = np.einsum("kr,k...->r...", NU, g_rt(t))
+    dnu = NU.sum(axis=0).reshape((3,) + (1,) * np.ndim(t))
+    return np.exp(-dg) * (P_ATM / (R_U * t))**dnu
+
+
+def rates_of_progress(rho, t, y):
+    c = concentrations(rho, y)
+    kf = fwd_rate_coefficients(t)
+    kc = equilibrium_constants(t)
+    out = []
+    for r in range(3):
+        fwd = kf[r].copy()
+        for k in range(NSP):
+            if ORD[k, r] != 0:
+                fwd = fwd * c[k]**ORD[k, r]
+        if REVERSIBLE[r]:
+            rev = kf[r] / kc[r]
+            for k in range(NSP):
+                if NU[k, r] > 0:
+                    rev = rev * c[k]**NU[k, r]
+            fwd = fwd - rev
+        out.append(fwd)
+    return np.stack(out)
+
+
+def net_production_rates(rho, t, y):
+    """``omega_k`` [kmol/m^3/s]"""
+    return np.einsum("kr,r...->k...", NU, rates_of_progress(rho, t, y))
+
+
+def species_source_terms(rho, t, y):
+    """``W_k omega_k`` [kg/m^3/s] (mirgecom/eos.py:858-875)."""
+    return _w(t) * net_production_rates(rho, t, y)
+
+
+class PyrometheusMixture:
+    """Oracle mixture EOS with the ``oracle.physics`` EOS interface
+    (``dim`` is implied by the field count ``2 + dim + 7``)."""
+
+    def __init__(self, temperature_niter=5):
+        self.niter = temperature_niter
+
+    @staticmethod
+    def _y(cv):
+        return cv[cv.shape[0] - NSP:] / cv[0]
+
+    def temperature(self, cv, dim, temperature_seed=None):
+        if temperature_seed is None:
+            raise ValueError("mixture temperature needs a seed")
+        mom = cv[2:2 + dim]
+        e = (cv[1] - 0.5 * np.sum(mom * mom, axis=0) / cv[0]) / cv[0]
+        return get_temperature(e, temperature_seed, self._y(cv), self.niter)
+
+    def pressure(self, cv, dim, temperature):
+        return pressure(cv[0], temperature, self._y(cv))
+
+    def gamma(self, cv, temperature):
+        y = self._y(cv)
+        cp = mixture_cp_mass(temperature, y)
+        return cp / (cp - specific_gas_constant(y))
+
+    def sound_speed(self, cv, dim, temperature):
+        return np.sqrt(self.gamma(cv, temperature)
+                       * self.pressure(cv, dim, temperature) / cv[0])
+
+    def heat_capacity_cv(self, cv, temperature):
+        return mixture_cv_mass(temperature, self._y(cv))
+
+    def species_enthalpies(self, cv, temperature):
+        return R_U / _w(temperature) * temperature * h_rt(temperature)
