@@ -37,27 +37,32 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=12,
                     help="cells per axis of the CPU-baseline sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain 256 MiB device copy (3x), the "
+                         "reference kernel for HBM-traffic counter scaling")
     return ap.parse_args()
 
 
 def alg_bytes(K, np_, nfp, dim, nfld):
-    """Algorithmic HBM bytes per launch of the two NS kernels: every nodal
-    field read or written once, per-element geometry (rx, normals, face
-    Jacobian ratios) and the int32 neighbour map once."""
+    """Algorithmic HBM bytes per launch of the two fused NS kernels: every
+    nodal array read or written once, plus per-element geometry (rx,
+    normals, face Jacobian ratios) and the int32 neighbour map once.
+      mg_ns_flux: read q (F), write nodal net flux (F*dim)
+      mg_ns_div:  read q (F) and nodal net flux (F*dim), write RHS (F)"""
     n = K * np_
     nf = dim + 1
     geo = K * (8 * (dim * dim + nf * dim + nf) + 4 * nf * nfp)
-    grad = 8 * n * (nfld + (nfld + 1) * dim) + geo
-    rhs = 8 * n * (nfld + (nfld + 1) * dim + nfld) + geo
-    return {"mg_ns_gradient": grad, "mg_ns_rhs": rhs}
+    flux = 8 * n * (nfld + nfld * dim) + geo
+    div = 8 * n * (nfld + nfld * dim + nfld) + geo
+    return {"mg_ns_flux": flux, "mg_ns_div": div}
 
 
 def alg_flops(K, np_, nfp, dim, nfld):
     """Contraction flops per launch (element-local matrix applications)."""
     nfn = (dim + 1) * nfp
-    grad = K * (nfld + 1) * (2 * dim * np_ * np_ + 2 * np_ * nfn)
-    rhs = K * nfld * (2 * dim * np_ * np_ + 2 * np_ * nfn)
-    return {"mg_ns_gradient": grad, "mg_ns_rhs": rhs}
+    flux = K * (nfld + 1) * (2 * dim * np_ * np_ + 2 * np_ * nfn)
+    div = K * nfld * (2 * dim * np_ * np_ + 2 * np_ * nfn)
+    return {"mg_ns_flux": flux, "mg_ns_div": div}
 
 
 def cpu_baseline(n, order):
@@ -167,6 +172,14 @@ def main():
                           aflops[k] / (kern_ms[k] * 1e-3) / 1e12, 2)}
                   for k in kern_ms}
     print("# kernels: " + json.dumps(per_kernel), file=sys.stderr, flush=True)
+    if args.copy_calibration:
+        src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print("# copy calibration: 3 x copy of 268435456 bytes", file=sys.stderr,
+              flush=True)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n, order)

@@ -46,7 +46,20 @@ typedef struct mg_mesh {
   const double* weak_diff;  /* [dim][np][np]  M^-1 D_r^T M                 */
   const double* lift;       /* [np][nfaces*nfp]  M^-1 E                    */
   const int32_t* face_mask; /* [nfaces][nfp] volume node of each face node */
+  const double* workspace;  /* packed operators, set by mg_mesh_prepare     */
 } mg_mesh;
+
+/* Bytes of the packed-operator workspace of a mesh (-1 if dim/order are
+ * unsupported). */
+int64_t mg_mesh_workspace_bytes(const mg_mesh* mesh);
+
+/* Pack weak_diff and lift into the caller-allocated workspace in the layout
+ * the contraction phase reads (rows of each wave contiguous, read with
+ * wave-uniform loads) and store the pointer in mesh->workspace.  Once per
+ * mesh; asynchronous on the stream.  There is no reference counterpart: it
+ * is the device-side part of building the discretization
+ * (mirgecom/discretization.py:41-83). */
+mg_status mg_mesh_prepare(mg_mesh* mesh, double* workspace, void* stream);
 
 /* Gas model: equation of state and transport (mirgecom/gas_model.py:82-97,
  * mirgecom/eos.py, mirgecom/transport.py). */
@@ -92,6 +105,31 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas,
                     const double* q, const double* temperature,
                     const double* grad, const double* q_bnd,
                     const double* t_bnd, int32_t flux_kind,
+                    int32_t inviscid_on, double* rhs, void* stream);
+
+/* Fused first half of the Navier-Stokes RHS: the DG gradients of the
+ * conserved fields and of T (as mg_ns_gradient) feed, in the same kernel,
+ * the pointwise net flux F_I - F_V at every node, written to fnet
+ * [F*dim][N] (F_I omitted if inviscid_on == 0).  The gradients themselves
+ * are written only if grad != NULL.  For boundary face nodes fnet_bnd
+ * [F][nbnd_nodes] receives (F_I - F_V)(q_ext, grad q-) . n, the exterior side
+ * of the boundary fluxes (mirgecom/boundary.py:944-970).  Replaces
+ * grad_cv_operator, grad_t_operator and the volume viscous_flux of
+ * ns_operator (mirgecom/navierstokes.py:494-541). */
+mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                     const double* temperature, const double* q_bnd,
+                     const double* t_bnd, int32_t inviscid_on, double* fnet,
+                     double* fnet_bnd, double* grad, void* stream);
+
+/* Second half: divergence of the nodal net fluxes with the numerical face
+ * flux (Rusanov/central applied to the net normal fluxes, HLL to the
+ * inviscid part plus the central viscous part), lift and inverse mass.
+ * Replaces viscous_flux_on_element_boundary, inviscid_flux_on_element_boundary
+ * and div_operator in ns_operator (mirgecom/navierstokes.py:544-563). */
+mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                    const double* temperature, const double* fnet,
+                    const double* q_bnd, const double* t_bnd,
+                    const double* fnet_bnd, int32_t flux_kind,
                     int32_t inviscid_on, double* rhs, void* stream);
 
 /* Mixture temperature by a fixed number of Newton iterations from a seed

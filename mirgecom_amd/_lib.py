@@ -30,6 +30,7 @@ class MgMesh(ctypes.Structure):
         ("fscale", ctypes.c_void_p), ("vmap_p", ctypes.c_void_p),
         ("weak_diff", ctypes.c_void_p), ("lift", ctypes.c_void_p),
         ("face_mask", ctypes.c_void_p),
+        ("workspace", ctypes.c_void_p),
     ]
 
 
@@ -58,12 +59,18 @@ SIGNATURES = {
                               _P, _P, _P, _P, _P, _P]),
     "mg_ns_rhs": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                          _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
+    "mg_ns_flux": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                          _P, _P, _P, _P, _I32, _P, _P, _P, _P]),
+    "mg_ns_div": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                         _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "mg_mixture_temperature": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P,
                                       _P, _I32, _P, _P]),
     "mg_species_source": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,
                                  _P, _P]),
     "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
     "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),
+    "mg_mesh_workspace_bytes": (_I64, [ctypes.POINTER(MgMesh)]),
+    "mg_mesh_prepare": (_I32, [ctypes.POINTER(MgMesh), _P, _P]),
     "mg_last_error": (ctypes.c_char_p, []),
     "mg_abi_version": (_I32, []),
 }

@@ -52,6 +52,11 @@ struct KArgs {
   const double* dw;
   const double* lift;
   const int32_t* fmask;
+  const double* ops;   // packed operators (mg_mesh_prepare)
+  const double* fnet;  // nodal net flux [F*DIM][N] (kFlux)
+  const double* fnb;   // exterior net normal flux [F][nbnd] (kFlux)
+  double* grad_out;    // optional gradient output of the fused flux kernel
+  double* fnb_out;     // exterior net normal flux output of the flux kernel
   int64_t K;
   int64_t nbnd;
   GasDev gas;
@@ -99,39 +104,131 @@ __device__ __forceinline__ void load_grad(double (*gq)[DIM], double* gt,
   for (int d = 0; d < DIM; ++d) gt[d] = grad[(NFLD * DIM + d) * stride + g];
 }
 
-// {{{ divergence (Euler / Navier-Stokes RHS)
+// {{{ packed element operators
+//
+// The contraction phase assigns one (element, field) column per lane and
+// splits the element's output rows over the kWaves waves of a workgroup.
+// Every lane of a wave then needs the same operator entry at the same time,
+// so the operators are read with wave-uniform addresses (scalar loads into
+// SGPRs) from a packed copy in which each wave's rows are contiguous:
+//   dvol [w][j][r][rr]     Dw_r[i][j]       (divergence and gradient volume)
+//   lift [w][m][rr]        LIFT[i][m]
+// with i = w*RPW + rr (rows >= NP are zero).
 
-template <int DIM, int P, int NSP, bool MIX, bool VISC>
+constexpr int kWaves = kBlock / 64;
+
+template <int DIM, int P>
+struct Ops {
+  using E = Elem<DIM, P>;
+  static constexpr int RPW = (E::NP + kWaves - 1) / kWaves;
+  static constexpr int DVOL = kWaves * E::NP * DIM * RPW;
+  static constexpr int LIFT = kWaves * E::NFN * RPW;
+  static constexpr int TOTAL = DVOL + LIFT;
+};
+
+template <int DIM, int P>
+__global__ __launch_bounds__(kBlock) void pack_ops_kernel(const double* dw,
+                                                          const double* lift,
+                                                          double* ws) {
+  using E = Elem<DIM, P>;
+  using O = Ops<DIM, P>;
+  constexpr int NP = E::NP, NFN = E::NFN, RPW = O::RPW;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < O::DVOL) {
+    const int rr = t % RPW;
+    const int r = (t / RPW) % DIM;
+    const int j = (t / (RPW * DIM)) % NP;
+    const int w = t / (RPW * DIM * NP);
+    const int i = w * RPW + rr;
+    ws[t] = i < NP ? dw[(r * NP + i) * NP + j] : 0.0;
+  } else if (t < O::TOTAL) {
+    const int u = t - O::DVOL;
+    const int rr = u % RPW;
+    const int m = (u / RPW) % NFN;
+    const int w = u / (RPW * NFN);
+    const int i = w * RPW + rr;
+    ws[t] = i < NP ? lift[i * NFN + m] : 0.0;
+  }
+}
+
+// Coalesced write-out of a block's staged results: stage[plane][le*NP + i]
+// -> out[plane*N + (e0 + le)*NP + i].
+template <int NP>
+__device__ __forceinline__ void write_staged(const double* stage, int nplanes,
+                                             int epb, int64_t e0, int64_t K,
+                                             double* __restrict__ out) {
+  const int64_t N = K * NP;
+  const int64_t nvalid = (K - e0 < epb ? K - e0 : epb) * NP;
+  const int per = epb * NP;
+  for (int t = threadIdx.x; t < nplanes * per; t += blockDim.x) {
+    const int plane = t / per;
+    const int idx = t - plane * per;
+    if (idx < nvalid) out[plane * N + e0 * NP + idx] = stage[t];
+  }
+}
+
+// }}}
+
+// {{{ divergence (Euler / Navier-Stokes RHS)
+//
+// Net flux F = F_I - F_V per node, by mode:
+//   kEuler  F_I(q) evaluated here
+//   kGrad   F_I(q) - F_V(q, grad q, grad T) from given gradients
+//   kFlux   read from the nodal-flux array written by flux_kernel
+// Face flux from the net normal fluxes Fn = F.n of both sides:
+//   Rusanov / central  (Fn- + Fn+ + lam (q- - q+)) / 2
+//   HLL                HLL(F_I- . n, F_I+ . n) - (F_V- . n + F_V+ . n) / 2
+
+constexpr int kEuler = 0, kGrad = 1, kFlux = 2;
+
+template <int DIM, int P, int NSP, bool MIX, int MODE>
 struct RhsCfg {
   using E = Elem<DIM, P>;
   static constexpr int NFLD = 2 + DIM + NSP;
-  static constexpr int PER_ELEM = NFLD * DIM * E::NP + NFLD * E::NFN;
-  static constexpr int EPB_LDS = (kLdsBudget / 8 - E::MAT - E::NFN) / PER_ELEM;
-  static constexpr int EPB_THR = kBlock / E::NP;
-  static constexpr int EPB = EPB_LDS < EPB_THR ? (EPB_LDS > 0 ? EPB_LDS : 1) : EPB_THR;
-  static constexpr size_t LDS = sizeof(double) * (E::MAT + EPB * PER_ELEM) +
+  static constexpr int EPB = 64 / NFLD;           // columns fit one wave
+  static constexpr int COLS = 64;
+  static constexpr int NODEF = NFLD * DIM;        // net flux
+  static constexpr int NODEQ = NFLD + 2;          // q, T, wavespeed
+  static constexpr int SF = EPB * E::NP * NODEF;
+  static constexpr int SQ = EPB * E::NP * NODEQ;
+  static constexpr int SFACE = E::NFN * COLS;
+  static constexpr int SGEO = EPB * DIM * DIM;
+  static constexpr int STAGE = NFLD * EPB * E::NP;
+  static_assert(STAGE <= SF, "staging aliases the flux buffer");
+  static constexpr size_t LDS = sizeof(double) * (SF + SQ + SFACE + SGEO) +
                                 sizeof(int) * E::NFN;
+  static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
 
-template <int DIM, int P, int NSP, bool MIX, bool VISC>
-__global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a) {
+template <int DIM, int P, int NSP, bool MIX, int MODE>
+__global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
+                                                     const double* __restrict__ ops,
+                                                     double* __restrict__ out) {
   using E = Elem<DIM, P>;
-  using C = RhsCfg<DIM, P, NSP, MIX, VISC>;
+  using O = Ops<DIM, P>;
+  using C = RhsCfg<DIM, P, NSP, MIX, MODE>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
-  constexpr int NFLD = C::NFLD, EPB = C::EPB;
+  constexpr int NFLD = C::NFLD, EPB = C::EPB, COLS = C::COLS;
+  constexpr int RPW = O::RPW, NODEF = C::NODEF, NODEQ = C::NODEQ;
+  constexpr bool VISC = MODE == kGrad;
   extern __shared__ double smem[];
-  double* s_dw = smem;
-  double* s_lift = s_dw + DIM * NP * NP;
-  double* s_vol = s_lift + NFN * NP;             // [EPB][NFLD][DIM][NP]
-  double* s_face = s_vol + EPB * NFLD * DIM * NP; // [EPB][NFLD][NFN]
-  int* s_fmask = reinterpret_cast<int*>(s_face + EPB * NFLD * NFN);
+  double* s_f = smem;                       // [EPB][NP][NFLD][DIM]
+  double* s_q = s_f + C::SF;                // [EPB][NP][NODEQ]
+  double* s_face = s_q + C::SQ;             // [NFN][COLS]
+  double* s_geo = s_face + C::SFACE;        // [EPB][DIM][DIM]
+  int* s_fmask = reinterpret_cast<int*>(s_geo + C::SGEO);
 
   const int64_t N = a.K * NP;
   const int64_t e0 = int64_t(blockIdx.x) * EPB;
-  load_matrices<DIM, P>(s_dw, s_lift, s_fmask, a);
+  const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
+  const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
+  const bool inv = MODE == kEuler || a.inviscid_on;
+  for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
+  for (int t = threadIdx.x; t < EPB * DIM * DIM; t += blockDim.x)
+    s_geo[t] = t < nel * DIM * DIM ? a.rx[e0 * DIM * DIM + t] : 0.0;
 
-  // phase 1: volume fluxes
+  // phase 1: state, wavespeed and net flux at the volume nodes (coalesced)
   for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
     const int le = t / NP;
     const int i = t - le * NP;
@@ -140,133 +237,231 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a) {
     const int64_t g = e * NP + i;
     Node<DIM, NSP> s;
     load_q<DIM, NSP>(s, a.q, N, g);
-    node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
-    double F[NFLD][DIM];
-    if (a.inviscid_on) {
-      inviscid_flux<DIM, NSP>(s, F);
+    const double tn = MIX ? a.temp[g] : 0.0;
+    node_state<DIM, NSP, MIX, VISC>(s, a.gas, tn);
+    double* fdst = s_f + t * NODEF;
+    if constexpr (MODE == kFlux) {
+#pragma unroll
+      for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet[k * N + g];
     } else {
+      double F[NFLD][DIM];
+      if (inv) {
+        inviscid_flux<DIM, NSP>(s, F);
+      } else {
 #pragma unroll
-      for (int f = 0; f < NFLD; ++f)
+        for (int f = 0; f < NFLD; ++f)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) F[f][d] = 0.0;
-    }
-    if constexpr (VISC) {
-      double gq[NFLD][DIM], gt[DIM], FV[NFLD][DIM];
-      load_grad<DIM, NSP>(gq, gt, a.grad, N, g);
-      viscous_flux<DIM, NSP, MIX>(s, gq, gt, FV);
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) F[f][d] -= FV[f][d];
-    }
-    double rxe[DIM][DIM];
-#pragma unroll
-    for (int r = 0; r < DIM; ++r)
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) rxe[r][d] = a.rx[(e * DIM + r) * DIM + d];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-      for (int r = 0; r < DIM; ++r) {
-        double gsum = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) gsum += rxe[r][d] * F[f][d];
-        s_vol[((le * NFLD + f) * DIM + r) * NP + i] = gsum;
+          for (int d = 0; d < DIM; ++d) F[f][d] = 0.0;
       }
+      if constexpr (VISC) {
+        double gq[NFLD][DIM], gt[DIM], FV[NFLD][DIM];
+        load_grad<DIM, NSP>(gq, gt, a.grad, N, g);
+        viscous_flux<DIM, NSP, MIX>(s, gq, gt, FV);
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) F[f][d] -= FV[f][d];
+      }
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) fdst[f * DIM + d] = F[f][d];
+    }
+    double* qdst = s_q + t * NODEQ;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) qdst[f] = s.q[f];
+    qdst[NFLD] = tn;
+    qdst[NFLD + 1] = s.speed + s.c;
   }
   __syncthreads();
 
-  // phase 2: face fluxes
+  // phase 2: numerical face fluxes.  The own side and neighbours inside this
+  // workgroup come from LDS; other neighbours are gathered here.
+  const bool hll = inv && a.flux_kind == 1;
   for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
     const int le = t / NFN;
     const int fn = t - le * NFN;
     const int64_t e = e0 + le;
     if (e >= a.K) continue;
     const int f = fn / NFP;
-    const int64_t gm = e * NP + s_fmask[fn];
+    const int im = s_fmask[fn];
     const int32_t nb = a.vmap_p[e * NFN + fn];
     double n[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + f) * DIM + d];
     const double fs = a.fscale[e * NF + f];
-    Node<DIM, NSP> sm, sp;
-    load_q<DIM, NSP>(sm, a.q, N, gm);
-    double tp = 0.0;
-    const bool bnd = nb < 0;
-    const int64_t b = bnd ? int64_t(-nb - 1) : 0;
-    if (!bnd) {
-      load_q<DIM, NSP>(sp, a.q, N, nb);
-      if (MIX) tp = a.temp[nb];
-    } else {
-      load_q<DIM, NSP>(sp, a.qb, a.nbnd, b);
-      if (MIX) tp = a.tb[b];
-    }
-    node_state<DIM, NSP, MIX>(sm, a.gas, MIX ? a.temp[gm] : 0.0);
-    node_state<DIM, NSP, MIX>(sp, a.gas, tp);
-    double fl[NFLD];
-    if (a.inviscid_on) {
-      inviscid_numflux<DIM, NSP>(sm, sp, n, a.flux_kind, fl);
-    } else {
+    const int lm = le * NP + im;
+    double fnm[NFLD], fnp[NFLD], qm[NFLD], qp[NFLD];
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) fl[k] = 0.0;
+    for (int k = 0; k < NFLD; ++k) {
+      double c = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) c += s_f[lm * NODEF + k * DIM + d] * n[d];
+      fnm[k] = c;
+      qm[k] = s_q[lm * NODEQ + k];
     }
-    if constexpr (VISC) {
-      double gq[NFLD][DIM], gt[DIM], fvm[NFLD][DIM], fvp[NFLD][DIM];
-      load_grad<DIM, NSP>(gq, gt, a.grad, N, gm);
-      viscous_flux<DIM, NSP, MIX>(sm, gq, gt, fvm);
-      if (!bnd) load_grad<DIM, NSP>(gq, gt, a.grad, N, nb);
-      viscous_flux<DIM, NSP, MIX>(sp, gq, gt, fvp);
+    const double tm = s_q[lm * NODEQ + NFLD];
+    const double wsm = s_q[lm * NODEQ + NFLD + 1];
+    double tp, wsp;
+    if (nb >= g_lo && nb < g_hi) {
+      const int lp = int(nb - g_lo);
 #pragma unroll
       for (int k = 0; k < NFLD; ++k) {
         double c = 0.0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) c += (fvp[k][d] + fvm[k][d]) / 2 * n[d];
-        fl[k] -= c;
+        for (int d = 0; d < DIM; ++d) c += s_f[lp * NODEF + k * DIM + d] * n[d];
+        fnp[k] = c;
+        qp[k] = s_q[lp * NODEQ + k];
+      }
+      tp = s_q[lp * NODEQ + NFLD];
+      wsp = s_q[lp * NODEQ + NFLD + 1];
+    } else {
+      const bool bnd = nb < 0;
+      const int64_t b = bnd ? int64_t(-nb - 1) : 0;
+      Node<DIM, NSP> sp;
+      if (!bnd) {
+        load_q<DIM, NSP>(sp, a.q, N, nb);
+        tp = MIX ? a.temp[nb] : 0.0;
+      } else {
+        load_q<DIM, NSP>(sp, a.qb, a.nbnd, b);
+        tp = MIX ? a.tb[b] : 0.0;
+      }
+      node_state<DIM, NSP, MIX, VISC>(sp, a.gas, tp);
+      wsp = sp.speed + sp.c;
+#pragma unroll
+      for (int k = 0; k < NFLD; ++k) qp[k] = sp.q[k];
+      if constexpr (MODE == kFlux) {
+        if (!bnd) {
+#pragma unroll
+          for (int k = 0; k < NFLD; ++k) {
+            double c = 0.0;
+#pragma unroll
+            for (int d = 0; d < DIM; ++d) c += a.fnet[(k * DIM + d) * N + nb] * n[d];
+            fnp[k] = c;
+          }
+        } else {
+#pragma unroll
+          for (int k = 0; k < NFLD; ++k) fnp[k] = a.fnb[k * a.nbnd + b];
+        }
+      } else {
+        if (inv) {
+          inviscid_flux_n<DIM, NSP>(sp, n, fnp);
+        } else {
+#pragma unroll
+          for (int k = 0; k < NFLD; ++k) fnp[k] = 0.0;
+        }
+        if constexpr (VISC) {
+          double gq[NFLD][DIM], gt[DIM], fv[NFLD];
+          load_grad<DIM, NSP>(gq, gt, a.grad, N, bnd ? e * NP + im : int64_t(nb));
+          viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
+#pragma unroll
+          for (int k = 0; k < NFLD; ++k) fnp[k] -= fv[k];
+        }
       }
     }
+    double fl[NFLD];
+    if (hll) {
+      Node<DIM, NSP> sm, sp;
 #pragma unroll
-    for (int k = 0; k < NFLD; ++k) s_face[(le * NFLD + k) * NFN + fn] = fs * fl[k];
+      for (int k = 0; k < NFLD; ++k) {
+        sm.q[k] = qm[k];
+        sp.q[k] = qp[k];
+      }
+      node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
+      node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
+      double fim[NFLD], fip[NFLD];
+      inviscid_flux_n<DIM, NSP>(sm, n, fim);
+      inviscid_flux_n<DIM, NSP>(sp, n, fip);
+      inviscid_numflux<DIM, NSP>(sm, sp, n, 1, fl);
+#pragma unroll
+      for (int k = 0; k < NFLD; ++k)
+        fl[k] -= ((fim[k] - fnm[k]) + (fip[k] - fnp[k])) / 2;
+    } else {
+      const double lam = (inv && a.flux_kind == 0) ? fmax(wsm, wsp) : 0.0;
+#pragma unroll
+      for (int k = 0; k < NFLD; ++k) fl[k] = (fnm[k] + fnp[k] + lam * (qm[k] - qp[k])) / 2;
+    }
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) s_face[fn * COLS + le * NFLD + k] = -(fs * fl[k]);
   }
   __syncthreads();
 
-  // phase 3: weak divergence + lift
-  for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
-    const int le = t / NP;
-    const int i = t - le * NP;
-    const int64_t e = e0 + le;
-    if (e >= a.K) continue;
-    const int64_t g = e * NP + i;
-#pragma unroll 1
-    for (int f = 0; f < NFLD; ++f) {
-      const double* gv = s_vol + (le * NFLD + f) * DIM * NP;
-      double acc = 0.0;
+  // phase 3: rhs[i] = sum_j sum_r Dw_r[i][j] G_r[j] + sum_m LIFT[i][m] B[m]
+  // per (element, field) column, G_r = sum_d rx[r][d] F_d; rows split over
+  // the waves, operator entries wave-uniform
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = threadIdx.x & 63;
+  const int cle = c < EPB * NFLD ? c / NFLD : 0;
+  const int cf = c < EPB * NFLD ? c - cle * NFLD : 0;
+  double rxl[DIM][DIM];
 #pragma unroll
-      for (int r = 0; r < DIM; ++r)
+  for (int r = 0; r < DIM; ++r)
 #pragma unroll
-        for (int j = 0; j < NP; ++j) acc += s_dw[(r * NP + j) * NP + i] * gv[r * NP + j];
-      const double* fv = s_face + (le * NFLD + f) * NFN;
-      double lf = 0.0;
+    for (int d = 0; d < DIM; ++d) rxl[r][d] = s_geo[cle * DIM * DIM + r * DIM + d];
+  const double* DV = ops + w * (NP * DIM * RPW);
+  const double* LF = ops + O::DVOL + w * (NFN * RPW);
+  double acc[RPW];
 #pragma unroll
-      for (int m = 0; m < NFN; ++m) lf += s_lift[m * NP + i] * fv[m];
-      a.out[f * N + g] = acc - lf;
+  for (int rr = 0; rr < RPW; ++rr) acc[rr] = 0.0;
+#pragma unroll 2
+  for (int j = 0; j < NP; ++j) {
+    const double* fj = s_f + (cle * NP + j) * NODEF + cf * DIM;
+    double fd[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) fd[d] = fj[d];
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) {
+      double gr = 0.0;
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) gr += rxl[r][d] * fd[d];
+#pragma unroll
+      for (int rr = 0; rr < RPW; ++rr) acc[rr] += DV[(j * DIM + r) * RPW + rr] * gr;
     }
   }
+#pragma unroll 4
+  for (int m = 0; m < NFN; ++m) {
+    const double bv = s_face[m * COLS + c];
+#pragma unroll
+    for (int rr = 0; rr < RPW; ++rr) acc[rr] += LF[m * RPW + rr] * bv;
+  }
+  __syncthreads();
+  // phase 4: stage in LDS (over s_f), write coalesced
+  double* stage = s_f;                       // [NFLD][EPB*NP]
+  if (c < EPB * NFLD) {
+#pragma unroll
+    for (int rr = 0; rr < RPW; ++rr) {
+      const int i = w * RPW + rr;
+      if (i < NP) stage[cf * (EPB * NP) + cle * NP + i] = acc[rr];
+    }
+  }
+  __syncthreads();
+  write_staged<NP>(stage, NFLD, EPB, e0, a.K, out);
 }
 
 // }}}
 
-// {{{ gradient
+// {{{ gradient (and the fused nodal net flux)
+//
+// grad_kernel computes the DG gradients of the conserved fields and of T
+// (central flux inside, exterior value on boundary faces).  With FLUX set it
+// then evaluates, at every node, the net flux F_I - F_V from the state and
+// the just-computed gradients (viscous stress, heat and species fluxes are
+// pointwise) and writes that instead of (or besides) the gradients; for
+// boundary face nodes it also writes (F_I - F_V)(q_ext, grad q-) . n, the
+// exterior side of the central viscous boundary flux.
 
 template <int DIM, int P, int NSP, bool MIX>
 struct GradCfg {
   using E = Elem<DIM, P>;
   static constexpr int NS = 3 + DIM + NSP;  // conserved fields + T
-  static constexpr int PER_ELEM = NS * E::NP + NS * E::NFN;
-  static constexpr int EPB_LDS = (kLdsBudget / 8 - E::MAT - E::NFN) / PER_ELEM;
-  static constexpr int EPB_THR = kBlock / E::NP;
-  static constexpr int EPB = EPB_LDS < EPB_THR ? (EPB_LDS > 0 ? EPB_LDS : 1) : EPB_THR;
-  static constexpr size_t LDS = sizeof(double) * (E::MAT + EPB * PER_ELEM) +
+  static constexpr int EPB = 64 / NS;
+  static constexpr int COLS = 64;
+  static constexpr int KG = E::NP + E::NFN;
+  static constexpr int GEO = DIM * DIM + E::NF * DIM;  // rx and normals
+  static constexpr int STAGE = NS * DIM * EPB * E::NP;
+  static constexpr size_t LDS = sizeof(double) * (KG * COLS + STAGE + EPB * GEO) +
                                 sizeof(int) * E::NFN;
+  static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
 
@@ -279,28 +474,44 @@ __device__ __forceinline__ double node_temperature(const double* qv,
     double mm = 0.0;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) mm += qv[2 + d] * qv[2 + d];
-    const double e_int = qv[1] - 0.5 * mm / qv[0];
-    return ((g.gamma - 1.0) / g.gas_const) * e_int / qv[0];
+    const double irho = 1.0 / qv[0];
+    const double e_int = qv[1] - 0.5 * mm * irho;
+    return ((g.gamma - 1.0) / g.gas_const) * e_int * irho;
   }
 }
 
-template <int DIM, int P, int NSP, bool MIX>
-__global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a) {
+template <int DIM, int P, int NSP, bool MIX, bool FLUX>
+__global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
+                                                      const double* __restrict__ ops,
+                                                      double* __restrict__ out) {
   using E = Elem<DIM, P>;
+  using O = Ops<DIM, P>;
   using C = GradCfg<DIM, P, NSP, MIX>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
-  constexpr int NFLD = 2 + DIM + NSP, NS = C::NS, EPB = C::EPB;
+  constexpr int NFLD = 2 + DIM + NSP, NS = C::NS, EPB = C::EPB, COLS = C::COLS;
+  constexpr int RPW = O::RPW, GEO = C::GEO;
   extern __shared__ double smem[];
-  double* s_dw = smem;
-  double* s_lift = s_dw + DIM * NP * NP;
-  double* s_u = s_lift + NFN * NP;           // [EPB][NS][NP]
-  double* s_us = s_u + EPB * NS * NP;        // [EPB][NS][NFN]
-  int* s_fmask = reinterpret_cast<int*>(s_us + EPB * NS * NFN);
+  double* s_b = smem;                           // [NP + NFN][COLS]
+  double* stage = s_b + C::KG * COLS;           // [NS][DIM][EPB*NP]
+  double* s_geo = stage + C::STAGE;             // [EPB][GEO]
+  int* s_fmask = reinterpret_cast<int*>(s_geo + EPB * GEO);
 
   const int64_t N = a.K * NP;
   const int64_t e0 = int64_t(blockIdx.x) * EPB;
-  load_matrices<DIM, P>(s_dw, s_lift, s_fmask, a);
+  const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
+  const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
+  for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
+  for (int t = threadIdx.x; t < EPB * GEO; t += blockDim.x) {
+    const int le = t / GEO;
+    const int u = t - le * GEO;
+    const int64_t e = e0 + le;
+    double v = 0.0;
+    if (e < a.K)
+      v = u < DIM * DIM ? a.rx[e * DIM * DIM + u] : a.normals[e * NF * DIM + u - DIM * DIM];
+    s_geo[t] = v;
+  }
 
+  // phase 1: conserved fields and T at the volume nodes
   for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
     const int le = t / NP;
     const int i = t - le * NP;
@@ -311,12 +522,14 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a) {
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) qv[f] = a.q[f * N + g];
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_u[(le * NS + f) * NP + i] = qv[f];
-    s_u[(le * NS + NFLD) * NP + i] =
+    for (int f = 0; f < NFLD; ++f) s_b[i * COLS + le * NS + f] = qv[f];
+    s_b[i * COLS + le * NS + NFLD] =
         node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g] : 0.0);
   }
   __syncthreads();
 
+  // phase 2: central (interior) / exterior-value (boundary) traces x fscale;
+  // neighbours inside this workgroup are read from LDS
   for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
     const int le = t / NFN;
     const int fn = t - le * NFN;
@@ -326,16 +539,22 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a) {
     const int im = s_fmask[fn];
     const int32_t nb = a.vmap_p[e * NFN + fn];
     const double fs = a.fscale[e * NF + f];
-    if (nb >= 0) {
+    double* dst = s_b + (NP + fn) * COLS + le * NS;
+    const double* own = s_b + im * COLS + le * NS;
+    if (nb >= g_lo && nb < g_hi) {
+      const int lp = int(nb - g_lo);
+      const int ple = lp / NP;
+      const double* nbv = s_b + (lp - ple * NP) * COLS + ple * NS;
+#pragma unroll
+      for (int k = 0; k <= NFLD; ++k) dst[k] = fs * ((nbv[k] + own[k]) / 2);
+    } else if (nb >= 0) {
       double qp[NFLD];
 #pragma unroll
       for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
       const double tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.temp[nb] : 0.0);
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k)
-        s_us[(le * NS + k) * NFN + fn] = fs * ((qp[k] + s_u[(le * NS + k) * NP + im]) / 2);
-      s_us[(le * NS + NFLD) * NFN + fn] =
-          fs * ((tp + s_u[(le * NS + NFLD) * NP + im]) / 2);
+      for (int k = 0; k < NFLD; ++k) dst[k] = fs * ((qp[k] + own[k]) / 2);
+      dst[NFLD] = fs * ((tp + own[NFLD]) / 2);
     } else {
       const int64_t b = -int64_t(nb) - 1;
       double qp[NFLD];
@@ -343,55 +562,136 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a) {
       for (int k = 0; k < NFLD; ++k) qp[k] = a.qb[k * a.nbnd + b];
       const double tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.tb[b] : 0.0);
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) s_us[(le * NS + k) * NFN + fn] = fs * qp[k];
-      s_us[(le * NS + NFLD) * NFN + fn] = fs * tp;
+      for (int k = 0; k < NFLD; ++k) dst[k] = fs * qp[k];
+      dst[NFLD] = fs * tp;
     }
   }
   __syncthreads();
 
-  for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
-    const int le = t / NP;
-    const int i = t - le * NP;
-    const int64_t e = e0 + le;
-    if (e >= a.K) continue;
-    const int64_t g = e * NP + i;
-    double rxe[DIM][DIM], nrm[NF][DIM];
+  // phase 3: per (element, scalar) column: W_r = Dw_r u, L_f = LIFT_f (fs u*),
+  // grad_d = sum_f n_f[d] L_f - sum_r rx[r][d] W_r
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int c = threadIdx.x & 63;
+  const double* GV = ops + w * (NP * DIM * RPW);
+  const double* GL = ops + O::DVOL + w * (NFN * RPW);
+  double accw[RPW][DIM], accl[RPW][NF];
+#pragma unroll
+  for (int rr = 0; rr < RPW; ++rr) {
+#pragma unroll
+    for (int r = 0; r < DIM; ++r) accw[rr][r] = 0.0;
+#pragma unroll
+    for (int f = 0; f < NF; ++f) accl[rr][f] = 0.0;
+  }
+#pragma unroll 2
+  for (int j = 0; j < NP; ++j) {
+    const double bv = s_b[j * COLS + c];
 #pragma unroll
     for (int r = 0; r < DIM; ++r)
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) rxe[r][d] = a.rx[(e * DIM + r) * DIM + d];
+      for (int rr = 0; rr < RPW; ++rr) accw[rr][r] += GV[(j * DIM + r) * RPW + rr] * bv;
+  }
 #pragma unroll
-    for (int f = 0; f < NF; ++f)
+  for (int f = 0; f < NF; ++f) {
+#pragma unroll 2
+    for (int m = 0; m < NFP; ++m) {
+      const double bv = s_b[(NP + f * NFP + m) * COLS + c];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) nrm[f][d] = a.normals[(e * NF + f) * DIM + d];
-#pragma unroll 1
-    for (int s = 0; s < NS; ++s) {
-      const double* u = s_u + (le * NS + s) * NP;
-      const double* us = s_us + (le * NS + s) * NFN;
-      double dr[DIM];
+      for (int rr = 0; rr < RPW; ++rr) accl[rr][f] += GL[(f * NFP + m) * RPW + rr] * bv;
+    }
+  }
+  // phase 4: stage the gradients [NS][DIM][EPB*NP]
+  if (c < EPB * NS) {
+    const int le = c / NS;
+    const int sc = c - le * NS;
+    const double* geo = s_geo + le * GEO;
 #pragma unroll
-      for (int r = 0; r < DIM; ++r) {
-        double acc = 0.0;
+    for (int d = 0; d < DIM; ++d) {
 #pragma unroll
-        for (int j = 0; j < NP; ++j) acc += s_dw[(r * NP + j) * NP + i] * u[j];
-        dr[r] = acc;
-      }
-      double lf[NF];
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        double acc = 0.0;
-#pragma unroll
-        for (int m = 0; m < NFP; ++m) acc += s_lift[(f * NFP + m) * NP + i] * us[f * NFP + m];
-        lf[f] = acc;
-      }
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
+      for (int rr = 0; rr < RPW; ++rr) {
+        const int i = w * RPW + rr;
+        if (i >= NP) continue;
         double vsum = 0.0, fsum = 0.0;
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) vsum += rxe[r][d] * dr[r];
+        for (int r = 0; r < DIM; ++r) vsum += geo[r * DIM + d] * accw[rr][r];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) fsum += nrm[f][d] * lf[f];
-        a.out[(s * DIM + d) * N + g] = fsum - vsum;
+        for (int f = 0; f < NF; ++f) fsum += geo[DIM * DIM + f * DIM + d] * accl[rr][f];
+        stage[(sc * DIM + d) * (EPB * NP) + le * NP + i] = fsum - vsum;
+      }
+    }
+  }
+  __syncthreads();
+  if (!FLUX || a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, e0, a.K,
+                                             FLUX ? a.grad_out : out);
+  if constexpr (FLUX) {
+    // phase 5: net flux F_I - F_V at every node, written coalesced
+    for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
+      const int le = t / NP;
+      const int i = t - le * NP;
+      const int64_t e = e0 + le;
+      if (e >= a.K) continue;
+      const int64_t g = e * NP + i;
+      Node<DIM, NSP> s;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s.q[f] = s_b[i * COLS + le * NS + f];
+      node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
+      double gq[NFLD][DIM], gt[DIM], F[NFLD][DIM];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) gq[f][d] = stage[(f * DIM + d) * (EPB * NP) + t];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) gt[d] = stage[(NFLD * DIM + d) * (EPB * NP) + t];
+      viscous_flux<DIM, NSP, MIX>(s, gq, gt, F);
+      double FI[NFLD][DIM];
+      if (a.inviscid_on) {
+        inviscid_flux<DIM, NSP>(s, FI);
+      } else {
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) FI[f][d] = 0.0;
+      }
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) out[(f * DIM + d) * N + g] = FI[f][d] - F[f][d];
+    }
+    // phase 6: exterior net normal flux at boundary face nodes, with the
+    // interior gradients (boundary.py:944-970)
+    if (a.nbnd > 0) {
+      for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
+        const int le = t / NFN;
+        const int fn = t - le * NFN;
+        const int64_t e = e0 + le;
+        if (e >= a.K) continue;
+        const int32_t nb = a.vmap_p[e * NFN + fn];
+        if (nb >= 0) continue;
+        const int64_t b = -int64_t(nb) - 1;
+        const int f = fn / NFP;
+        const int im = s_fmask[fn];
+        double n[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) n[d] = s_geo[le * GEO + DIM * DIM + f * DIM + d];
+        Node<DIM, NSP> s;
+        load_q<DIM, NSP>(s, a.qb, a.nbnd, b);
+        node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.tb[b] : 0.0);
+        double gq[NFLD][DIM], gt[DIM], fv[NFLD], fi[NFLD];
+        const int ti = le * NP + im;
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k)
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) gq[k][d] = stage[(k * DIM + d) * (EPB * NP) + ti];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) gt[d] = stage[(NFLD * DIM + d) * (EPB * NP) + ti];
+        viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
+        if (a.inviscid_on) {
+          inviscid_flux_n<DIM, NSP>(s, n, fi);
+        } else {
+#pragma unroll
+          for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
+        }
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) a.fnb_out[k * a.nbnd + b] = fi[k] - fv[k];
       }
     }
   }
@@ -520,6 +820,8 @@ mg_status check_mesh(const mg_mesh* m) {
   if (!m->rx || !m->normals || !m->fscale || !m->vmap_p || !m->weak_diff ||
       !m->lift || !m->face_mask)
     return fail(MG_ERR_ARG, "mesh has a NULL array");
+  if (!m->workspace)
+    return fail(MG_ERR_ARG, "mesh workspace not prepared (mg_mesh_prepare)");
   return MG_OK;
 }
 
@@ -532,103 +834,97 @@ mgdg::KArgs make_args(const mg_mesh* m, const mg_gas* g) {
   a.dw = m->weak_diff;
   a.lift = m->lift;
   a.fmask = m->face_mask;
+  a.ops = m->workspace;
   a.K = m->nelements;
   a.nbnd = m->nbnd_nodes;
   a.gas = to_dev(g);
   return a;
 }
 
-template <int DIM, int P, int NSP, bool MIX, bool VISC>
+template <typename Kern>
+mg_status set_lds(Kern kern, size_t lds) {
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize,
+                                     int(lds));
+  if (e != hipSuccess)
+    return fail(MG_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+  return MG_OK;
+}
+
+template <int DIM, int P, int NSP, bool MIX, int MODE>
 mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
-  using C = mgdg::RhsCfg<DIM, P, NSP, MIX, VISC>;
-  auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, VISC>;
+  using C = mgdg::RhsCfg<DIM, P, NSP, MIX, MODE>;
+  auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, MODE>;
   static bool attr = false;
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       int(C::LDS));
-    if (e != hipSuccess)
-      return fail(MG_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    mg_status s = set_lds(kern, C::LDS);
+    if (s) return s;
     attr = true;
   }
   const int64_t nblk = (a.K + C::EPB - 1) / C::EPB;
-  hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(mgdg::kBlock), C::LDS, st, a);
+  hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(mgdg::kBlock), C::LDS, st, a,
+                     a.ops, a.out);
   return check_launch("rhs_kernel");
 }
 
-template <int DIM, int P, int NSP, bool MIX>
+template <int DIM, int P, int NSP, bool MIX, bool FLUX>
 mg_status launch_grad(const mgdg::KArgs& a, hipStream_t st) {
   using C = mgdg::GradCfg<DIM, P, NSP, MIX>;
-  auto kern = mgdg::grad_kernel<DIM, P, NSP, MIX>;
+  auto kern = mgdg::grad_kernel<DIM, P, NSP, MIX, FLUX>;
   static bool attr = false;
   if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize,
-                                       int(C::LDS));
-    if (e != hipSuccess)
-      return fail(MG_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
+    mg_status s = set_lds(kern, C::LDS);
+    if (s) return s;
     attr = true;
   }
   const int64_t nblk = (a.K + C::EPB - 1) / C::EPB;
-  hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(mgdg::kBlock), C::LDS, st, a);
+  hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(mgdg::kBlock), C::LDS, st, a,
+                     a.ops, a.out);
   return check_launch("grad_kernel");
 }
 
-template <int DIM>
-mg_status dispatch_rhs_dim(int order, const mgdg::KArgs& a, int nsp, bool mix,
-                           bool visc, hipStream_t st);
-
-#define MG_RHS_CASES(DIM)                                                     \
-  template <>                                                                 \
-  mg_status dispatch_rhs_dim<DIM>(int order, const mgdg::KArgs& a, int nsp,   \
-                                  bool mix, bool visc, hipStream_t st) {      \
-    switch (order) {                                                          \
-      case 1: return dispatch_rhs_p2<DIM, 1>(a, nsp, mix, visc, st);          \
-      case 2: return dispatch_rhs_p2<DIM, 2>(a, nsp, mix, visc, st);          \
-      case 3: return dispatch_rhs_p2<DIM, 3>(a, nsp, mix, visc, st);          \
-      case 4: return dispatch_rhs_p2<DIM, 4>(a, nsp, mix, visc, st);          \
-    }                                                                         \
-    return fail(MG_ERR_UNSUPPORTED, "order");                                 \
-  }
-
-template <int DIM, int P>
-mg_status dispatch_rhs_p2(const mgdg::KArgs& a, int nsp, bool mix, bool visc,
-                          hipStream_t st) {
-  if (visc) {
-    if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, false, true>(a, st);
-    if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, false, true>(a, st);
-    if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, true, true>(a, st);
-  } else {
-    if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, false, false>(a, st);
-    if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, false, false>(a, st);
-    if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, true, false>(a, st);
-  }
+// compiled (species count, EOS) combinations: (0, ideal), (7, ideal), (7, mixture)
+template <int DIM, int P, int MODE>
+mg_status rhs_species(const mgdg::KArgs& a, int nsp, bool mix, hipStream_t st) {
+  if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, false, MODE>(a, st);
+  if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, false, MODE>(a, st);
+  if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, true, MODE>(a, st);
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
-MG_RHS_CASES(2)
-MG_RHS_CASES(3)
-
-template <int DIM, int P>
-mg_status dispatch_grad_p(const mgdg::KArgs& a, int nsp, bool mix, hipStream_t st) {
-  if (nsp == 0 && !mix) return launch_grad<DIM, P, 0, false>(a, st);
-  if (nsp == 7 && !mix) return launch_grad<DIM, P, 7, false>(a, st);
-  if (nsp == 7 && mix) return launch_grad<DIM, P, 7, true>(a, st);
+template <int DIM, int P, bool FLUX>
+mg_status grad_species(const mgdg::KArgs& a, int nsp, bool mix, hipStream_t st) {
+  if (nsp == 0 && !mix) return launch_grad<DIM, P, 0, false, FLUX>(a, st);
+  if (nsp == 7 && !mix) return launch_grad<DIM, P, 7, false, FLUX>(a, st);
+  if (nsp == 7 && mix) return launch_grad<DIM, P, 7, true, FLUX>(a, st);
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
-template <int DIM>
-mg_status dispatch_grad_dim(int order, const mgdg::KArgs& a, int nsp, bool mix,
-                            hipStream_t st) {
-  switch (order) {
-    case 1: return dispatch_grad_p<DIM, 1>(a, nsp, mix, st);
-    case 2: return dispatch_grad_p<DIM, 2>(a, nsp, mix, st);
-    case 3: return dispatch_grad_p<DIM, 3>(a, nsp, mix, st);
-    case 4: return dispatch_grad_p<DIM, 4>(a, nsp, mix, st);
+template <int DIM, int P>
+mg_status dispatch_p(const mgdg::KArgs& a, int nsp, bool mix, int what,
+                     hipStream_t st) {
+  switch (what) {
+    case 0: return rhs_species<DIM, P, mgdg::kEuler>(a, nsp, mix, st);
+    case 1: return rhs_species<DIM, P, mgdg::kGrad>(a, nsp, mix, st);
+    case 2: return rhs_species<DIM, P, mgdg::kFlux>(a, nsp, mix, st);
+    case 3: return grad_species<DIM, P, false>(a, nsp, mix, st);
+    case 4: return grad_species<DIM, P, true>(a, nsp, mix, st);
   }
-  return fail(MG_ERR_UNSUPPORTED, "order");
+  return fail(MG_ERR_ARG, "kernel selector");
+}
+
+// what: 0 Euler RHS, 1 NS RHS from gradients, 2 NS RHS from nodal fluxes,
+// 3 gradients, 4 gradients + nodal fluxes
+mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, bool mix,
+                   int what, hipStream_t st) {
+#define MG_CASE(D, P) \
+  if (m->dim == D && m->order == P) return dispatch_p<D, P>(a, nsp, mix, what, st);
+  MG_CASE(2, 1) MG_CASE(2, 2) MG_CASE(2, 3) MG_CASE(2, 4)
+  MG_CASE(3, 1) MG_CASE(3, 2) MG_CASE(3, 3) MG_CASE(3, 4)
+#undef MG_CASE
+  return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }
 
 mg_status check_gas(const mg_gas* g, bool need_transport) {
@@ -649,6 +945,18 @@ mg_status ensure_mech() {
   return fail(MG_ERR_ARG, "mixture mechanism not loaded (mg_load_mechanism)");
 }
 
+template <int D, int P>
+mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
+  const int n = mgdg::Ops<D, P>::TOTAL;
+  auto kern = mgdg::pack_ops_kernel<D, P>;
+  hipLaunchKernelGGL(kern, dim3((n + 255) / 256), dim3(256), 0, st, m->weak_diff,
+                     m->lift, ws);
+  mg_status s = check_launch("pack_ops_kernel");
+  if (s) return s;
+  m->workspace = ws;
+  return MG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -667,6 +975,31 @@ mg_status mg_load_mechanism(const double* packed, int64_t count) {
   if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
   g_mech_loaded = true;
   return MG_OK;
+}
+
+int64_t mg_mesh_workspace_bytes(const mg_mesh* m) {
+  if (!m) return -1;
+  int total = -1;
+#define MG_WS_CASE(D, P) \
+  if (m->dim == D && m->order == P) total = mgdg::Ops<D, P>::TOTAL;
+  MG_WS_CASE(2, 1) MG_WS_CASE(2, 2) MG_WS_CASE(2, 3) MG_WS_CASE(2, 4)
+  MG_WS_CASE(3, 1) MG_WS_CASE(3, 2) MG_WS_CASE(3, 3) MG_WS_CASE(3, 4)
+#undef MG_WS_CASE
+  return total < 0 ? -1 : int64_t(total) * int64_t(sizeof(double));
+}
+
+mg_status mg_mesh_prepare(mg_mesh* m, double* workspace, void* stream) {
+  if (!m || !workspace) return fail(MG_ERR_ARG, "mesh/workspace NULL");
+  if (!m->weak_diff || !m->lift) return fail(MG_ERR_ARG, "mesh matrices NULL");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  mg_status s = MG_ERR_UNSUPPORTED;
+#define MG_PACK_CASE(D, P) \
+  if (m->dim == D && m->order == P) s = pack_ops<D, P>(m, workspace, st);
+  MG_PACK_CASE(2, 1) MG_PACK_CASE(2, 2) MG_PACK_CASE(2, 3) MG_PACK_CASE(2, 4)
+  MG_PACK_CASE(3, 1) MG_PACK_CASE(3, 2) MG_PACK_CASE(3, 3) MG_PACK_CASE(3, 4)
+#undef MG_PACK_CASE
+  if (s == MG_ERR_UNSUPPORTED) return fail(s, "dim/order");
+  return s;
 }
 
 mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
@@ -691,10 +1024,7 @@ mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.out = rhs;
   a.flux_kind = flux_kind;
   a.inviscid_on = 1;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (mesh->dim == 2)
-    return dispatch_rhs_dim<2>(mesh->order, a, gas->nspecies, mix, false, st);
-  return dispatch_rhs_dim<3>(mesh->order, a, gas->nspecies, mix, false, st);
+  return dispatch(mesh, a, gas->nspecies, mix, 0, static_cast<hipStream_t>(stream));
 }
 
 mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas, const double* q,
@@ -714,9 +1044,7 @@ mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas, const double* q
   a.qb = q_bnd;
   a.tb = t_bnd;
   a.out = grad;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (mesh->dim == 2) return dispatch_grad_dim<2>(mesh->order, a, gas->nspecies, mix, st);
-  return dispatch_grad_dim<3>(mesh->order, a, gas->nspecies, mix, st);
+  return dispatch(mesh, a, gas->nspecies, mix, 3, static_cast<hipStream_t>(stream));
 }
 
 mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
@@ -742,10 +1070,60 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.out = rhs;
   a.flux_kind = flux_kind;
   a.inviscid_on = inviscid_on ? 1 : 0;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  if (mesh->dim == 2)
-    return dispatch_rhs_dim<2>(mesh->order, a, gas->nspecies, mix, true, st);
-  return dispatch_rhs_dim<3>(mesh->order, a, gas->nspecies, mix, true, st);
+  return dispatch(mesh, a, gas->nspecies, mix, 1, static_cast<hipStream_t>(stream));
+}
+
+mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                     const double* temperature, const double* q_bnd,
+                     const double* t_bnd, int32_t inviscid_on, double* fnet,
+                     double* fnet_bnd, double* grad, void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, true))) return s;
+  const bool mix = gas->eos_kind == 1;
+  if (mix && (s = ensure_mech())) return s;
+  if (!q || !fnet) return fail(MG_ERR_ARG, "q/fnet NULL");
+  if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
+  if (mesh->nbnd_nodes > 0 && (!q_bnd || !fnet_bnd || (mix && !t_bnd)))
+    return fail(MG_ERR_ARG, "boundary exterior state / flux buffer missing");
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.temp = temperature;
+  a.qb = q_bnd;
+  a.tb = t_bnd;
+  a.out = fnet;
+  a.grad_out = grad;
+  a.fnb_out = fnet_bnd;
+  a.inviscid_on = inviscid_on ? 1 : 0;
+  return dispatch(mesh, a, gas->nspecies, mix, 4, static_cast<hipStream_t>(stream));
+}
+
+mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                    const double* temperature, const double* fnet,
+                    const double* q_bnd, const double* t_bnd,
+                    const double* fnet_bnd, int32_t flux_kind,
+                    int32_t inviscid_on, double* rhs, void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, true))) return s;
+  const bool mix = gas->eos_kind == 1;
+  if (mix && (s = ensure_mech())) return s;
+  if (!q || !fnet || !rhs) return fail(MG_ERR_ARG, "q/fnet/rhs NULL");
+  if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
+  if (mesh->nbnd_nodes > 0 && (!q_bnd || !fnet_bnd || (mix && !t_bnd)))
+    return fail(MG_ERR_ARG, "boundary exterior state / flux missing");
+  if (flux_kind < 0 || flux_kind > 2) return fail(MG_ERR_ARG, "flux_kind");
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.temp = temperature;
+  a.fnet = fnet;
+  a.fnb = fnet_bnd;
+  a.qb = q_bnd;
+  a.tb = t_bnd;
+  a.out = rhs;
+  a.flux_kind = flux_kind;
+  a.inviscid_on = inviscid_on ? 1 : 0;
+  return dispatch(mesh, a, gas->nspecies, mix, 2, static_cast<hipStream_t>(stream));
 }
 
 mg_status mg_mixture_temperature(const mg_gas* gas, int32_t dim, int64_t n,

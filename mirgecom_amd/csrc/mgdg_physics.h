@@ -103,53 +103,16 @@ struct Node {
   double v[DIM];
   double y[NS1];
   double hk[NS1];
-  double p, t, c, gamma, speed;
+  double p, t, c, gamma, speed, inv_rho;
   // transport
   double mu, mu_b, kappa;
   double d[NS1];
 };
 
-// Fill the dependent variables of a node from q (and T for mixtures).
+// Transport coefficients of a node (SimpleTransport / PowerLawTransport).
 template <int DIM, int NSP, bool MIX>
-__device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
-                                           double t_mix) {
-  const double rho = s.q[0];
-  double mm = 0.0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) mm += s.q[2 + d] * s.q[2 + d];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) s.v[d] = s.q[2 + d] / rho;
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) s.y[k] = s.q[2 + DIM + k] / rho;
-  double vv = 0.0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) vv += s.v[d] * s.v[d];
-  s.speed = sqrt(vv);
-  const double e_int = s.q[1] - 0.5 * mm / rho;
-  if constexpr (!MIX) {
-    s.gamma = g.gamma;
-    s.p = e_int * (g.gamma - 1.0);
-    s.t = ((g.gamma - 1.0) / g.gas_const) * e_int / rho;
-    s.c = sqrt(g.gamma / rho * s.p);
-  } else {
-    static_assert(NSP == MIX_NSP, "mixture path is the 7-species mechanism");
-    s.t = t_mix;
-    double rsum = 0.0, cpsum = 0.0;
-#pragma unroll
-    for (int k = 0; k < MIX_NSP; ++k) {
-      double cp_r, h_rt;
-      nasa7_cp_h(k, t_mix, cp_r, h_rt);
-      rsum += s.y[k] * c_mech.inv_wt[k];
-      cpsum += c_mech.inv_wt[k] * cp_r * s.y[k];
-      s.hk[k] = c_mech.gas_constant * c_mech.inv_wt[k] * t_mix * h_rt;
-    }
-    const double rmix = c_mech.gas_constant * rsum;
-    const double cp = c_mech.gas_constant * cpsum;
-    s.p = rho * rmix * t_mix;
-    s.gamma = cp / (cp - rmix);
-    s.c = sqrt(s.gamma * s.p / rho);
-  }
-  // transport
+__device__ __forceinline__ void node_transport(Node<DIM, NSP>& s, const GasDev& g) {
+  const double irho = s.inv_rho;
   if (g.transport_kind == 1) {
     s.mu = g.mu;
     s.mu_b = g.mu_b;
@@ -170,9 +133,55 @@ __device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
     s.kappa = g.pl_sigma * mu * cv;
 #pragma unroll
     for (int k = 0; k < NSP; ++k)
-      s.d[k] = g.has_lewis ? g.pl_sigma * mu / (rho * g.diff[k] * s.gamma)
+      s.d[k] = g.has_lewis ? g.pl_sigma * mu * irho / (g.diff[k] * s.gamma)
                            : g.diff[k];
   }
+}
+
+// Fill the dependent variables of a node from q (and T for mixtures); the
+// transport coefficients only if TRANSPORT.
+template <int DIM, int NSP, bool MIX, bool TRANSPORT = true>
+__device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
+                                           double t_mix) {
+  const double rho = s.q[0];
+  const double irho = 1.0 / rho;
+  s.inv_rho = irho;
+  double mm = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) mm += s.q[2 + d] * s.q[2 + d];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) s.v[d] = s.q[2 + d] * irho;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) s.y[k] = s.q[2 + DIM + k] * irho;
+  double vv = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) vv += s.v[d] * s.v[d];
+  s.speed = sqrt(vv);
+  const double e_int = s.q[1] - 0.5 * mm * irho;
+  if constexpr (!MIX) {
+    s.gamma = g.gamma;
+    s.p = e_int * (g.gamma - 1.0);
+    s.t = ((g.gamma - 1.0) / g.gas_const) * e_int * irho;
+    s.c = sqrt(g.gamma * irho * s.p);
+  } else {
+    static_assert(NSP == MIX_NSP, "mixture path is the 7-species mechanism");
+    s.t = t_mix;
+    double rsum = 0.0, cpsum = 0.0;
+#pragma unroll
+    for (int k = 0; k < MIX_NSP; ++k) {
+      double cp_r, h_rt;
+      nasa7_cp_h(k, t_mix, cp_r, h_rt);
+      rsum += s.y[k] * c_mech.inv_wt[k];
+      cpsum += c_mech.inv_wt[k] * cp_r * s.y[k];
+      s.hk[k] = c_mech.gas_constant * c_mech.inv_wt[k] * t_mix * h_rt;
+    }
+    const double rmix = c_mech.gas_constant * rsum;
+    const double cp = c_mech.gas_constant * cpsum;
+    s.p = rho * rmix * t_mix;
+    s.gamma = cp / (cp - rmix);
+    s.c = sqrt(s.gamma * s.p * irho);
+  }
+  if constexpr (TRANSPORT) node_transport<DIM, NSP, MIX>(s, g);
 }
 
 // Inviscid flux F[f][d] (inviscid.py:65-102).
@@ -256,11 +265,12 @@ __device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
                                              const double (*gq)[DIM],
                                              const double* gt, double (*FV)[DIM]) {
   const double rho = s.q[0];
+  const double irho = s.inv_rho;
   double gv[DIM][DIM];
 #pragma unroll
   for (int i = 0; i < DIM; ++i)
 #pragma unroll
-    for (int j = 0; j < DIM; ++j) gv[i][j] = (gq[2 + i][j] - s.v[i] * gq[0][j]) / rho;
+    for (int j = 0; j < DIM; ++j) gv[i][j] = (gq[2 + i][j] - s.v[i] * gq[0][j]) * irho;
   double divv = 0.0;
 #pragma unroll
   for (int i = 0; i < DIM; ++i) divv += gv[i][i];
@@ -278,7 +288,7 @@ __device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
     for (int k = 0; k < NSP; ++k)
 #pragma unroll
       for (int d = 0; d < DIM; ++d)
-        gy[k][d] = (gq[2 + DIM + k][d] - s.y[k] * gq[0][d]) / rho;
+        gy[k][d] = (gq[2 + DIM + k][d] - s.y[k] * gq[0][d]) * irho;
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       double sum = 0.0;
@@ -307,6 +317,23 @@ __device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
     for (int i = 0; i < DIM; ++i) FV[2 + i][d] = tau[i][d];
 #pragma unroll
     for (int k = 0; k < NSP; ++k) FV[2 + DIM + k][d] = -jd[k][d];
+  }
+}
+
+// Normal viscous flux F_V . n without materialising F_V[f][d].
+template <int DIM, int NSP, bool MIX>
+__device__ __forceinline__ void viscous_flux_n(const Node<DIM, NSP>& s,
+                                               const double (*gq)[DIM],
+                                               const double* gt, const double* n,
+                                               double* out) {
+  double fv[2 + DIM + NSP][DIM];
+  viscous_flux<DIM, NSP, MIX>(s, gq, gt, fv);
+#pragma unroll
+  for (int f = 0; f < 2 + DIM + NSP; ++f) {
+    double c = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) c += fv[f][d] * n[d];
+    out[f] = c;
   }
 }
 

@@ -56,7 +56,15 @@ class DeviceDiscretization:
             rx=self.rx.data_ptr(), normals=self.normals.data_ptr(),
             fscale=self.fscale.data_ptr(), vmap_p=self.vmap_p.data_ptr(),
             weak_diff=self.weak_diff.data_ptr(), lift=self.lift.data_ptr(),
-            face_mask=self.face_mask.data_ptr())
+            face_mask=self.face_mask.data_ptr(), workspace=None)
+        lib = _lib.load()
+        nbytes = lib.mg_mesh_workspace_bytes(self.mesh)
+        if nbytes <= 0:
+            raise NotImplementedError(
+                f"dim={dcoll.dim} order={re.order} has no device kernels")
+        self.workspace = torch.empty(nbytes // 8, **f64)
+        _lib.check(lib.mg_mesh_prepare(self.mesh, _lib.ptr(self.workspace),
+                                       _lib.stream_handle()))
 
     def boundary_index(self, tag):
         """Indices (into the boundary-node arrays) of a boundary tag."""

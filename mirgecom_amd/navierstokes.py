@@ -91,20 +91,45 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
     _check_grad_flux(gradient_numerical_flux_func)
     warn_mixture_limiter(state, limiter_func)
     kind = flux_kind(inviscid_numerical_flux_func)
-    if grad_cv is None or grad_t is None:
-        prep, grad = _gradients(dcoll, gas_model, boundaries, state, time)
-        if grad_cv is not None:
-            grad[:-1] = grad_cv.q
-        if grad_t is not None:
-            grad[-1] = grad_t
-    else:
+    inv = 1 if inviscid_terms_on else 0
+    if grad_cv is None and grad_t is None:
+        # fused path: gradients -> nodal net flux (one kernel), divergence
         prep = Prepared(dcoll, state, gas_model, boundaries, time)
-        grad = torch.cat([grad_cv.q, grad_t[None]], dim=0).contiguous()
-    rhs = torch.empty_like(prep.q)
-    _lib.call("mg_ns_rhs", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
-              _lib.ptr(prep.temp), _lib.ptr(grad), _lib.ptr(prep.qb),
-              _lib.ptr(prep.tb), kind, 1 if inviscid_terms_on else 0,
-              _lib.ptr(rhs), prep.stream)
+        nf, dim = prep.q.shape[0], prep.dim
+        shape = tuple(prep.q.shape[1:])
+        fnet = torch.empty((nf * dim,) + shape, dtype=prep.q.dtype,
+                           device=prep.q.device)
+        fnb = None
+        if prep.qb is not None:
+            fnb = torch.empty((nf, prep.qb.shape[1]), dtype=prep.q.dtype,
+                              device=prep.q.device)
+        grad = None
+        if return_gradients:
+            grad = torch.empty((nf + 1, dim) + shape, dtype=prep.q.dtype,
+                               device=prep.q.device)
+        _lib.call("mg_ns_flux", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+                  _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb),
+                  inv, _lib.ptr(fnet), _lib.ptr(fnb), _lib.ptr(grad),
+                  prep.stream)
+        rhs = torch.empty_like(prep.q)
+        _lib.call("mg_ns_div", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+                  _lib.ptr(prep.temp), _lib.ptr(fnet), _lib.ptr(prep.qb),
+                  _lib.ptr(prep.tb), _lib.ptr(fnb), kind, inv, _lib.ptr(rhs),
+                  prep.stream)
+    else:
+        if grad_cv is None or grad_t is None:
+            prep, grad = _gradients(dcoll, gas_model, boundaries, state, time)
+            if grad_cv is not None:
+                grad[:-1] = grad_cv.q
+            if grad_t is not None:
+                grad[-1] = grad_t
+        else:
+            prep = Prepared(dcoll, state, gas_model, boundaries, time)
+            grad = torch.cat([grad_cv.q, grad_t[None]], dim=0).contiguous()
+        rhs = torch.empty_like(prep.q)
+        _lib.call("mg_ns_rhs", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+                  _lib.ptr(prep.temp), _lib.ptr(grad), _lib.ptr(prep.qb),
+                  _lib.ptr(prep.tb), kind, inv, _lib.ptr(rhs), prep.stream)
     out = ConservedVars(state.dim, rhs)
     if return_gradients:
         return out, ConservedVars(state.dim, grad[:-1]), grad[-1]

@@ -68,3 +68,35 @@ def smooth_perturbed_state(x, seed, nspecies=0, amp=0.05, gamma=1.4):
         ys = [field(0.5 / nspecies, 0.2 / nspecies) for _ in range(nspecies)]
         rows += [rho * y for y in ys]
     return np.stack(rows)
+
+
+def mixture_state(x, seed, velocity_amp=5.0, t_mean=1600.0, t_amp=300.0):
+    """A seeded uiuc_7sp state (conserved ``(F, ...)``) and its temperature:
+    stoichiometric-ish ethylene/air with perturbed mixture fraction and some
+    products, T = t_mean + t_amp * smooth field, p = 1 atm."""
+    from oracle import thermochem as tc
+    rng = np.random.default_rng(seed)
+    dim = x.shape[0]
+
+    def smooth():
+        k = rng.integers(1, 3, size=dim)
+        ph = rng.uniform(0, 2 * np.pi, size=dim)
+        val = np.ones_like(x[0])
+        for i in range(dim):
+            val = val * np.cos(k[i] * x[i] + ph[i])
+        return val
+
+    t = t_mean + t_amp * smooth()
+    # C2H4, O2, CO2, CO, H2O, H2, N2
+    y = np.stack([0.06 * (1 + 0.2 * smooth()), 0.2 * (1 + 0.1 * smooth()),
+                  0.02 * (1 + 0.5 * smooth()), 0.01 * (1 + 0.5 * smooth()),
+                  0.03 * (1 + 0.5 * smooth()), 0.002 * (1 + 0.5 * smooth()),
+                  0 * t])
+    y[-1] = 1.0 - y[:-1].sum(axis=0)
+    p = 101325.0 * (1 + 0.05 * smooth())
+    rho = tc.density(p, t, y)
+    vel = [velocity_amp * smooth() for _ in range(dim)]
+    e = tc.mixture_energy_mass(t, y)
+    energy = rho * (e + 0.5 * sum(v * v for v in vel))
+    q = np.stack([rho, energy] + [rho * v for v in vel] + [rho * yk for yk in y])
+    return q, t

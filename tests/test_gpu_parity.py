@@ -170,6 +170,66 @@ def test_ns_species_dummy_boundary(transport):
     assert err.max() < TOL
 
 
+def test_ns_paths_agree_and_prescribed_viscous_boundary():
+    """Navier-Stokes with a prescribed (non-trivial) boundary state: the
+    fused path (nodal net fluxes), the path with returned gradients and the
+    path from explicitly given gradients all match the oracle."""
+    from mirgecom_amd.boundary import PrescribedFluidBoundary
+    from mirgecom_amd.eos import IdealSingleGas
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from mirgecom_amd.initializers import Vortex2D
+    from mirgecom_amd.inviscid import inviscid_facial_flux_hll
+    from mirgecom_amd.navierstokes import (
+        grad_cv_operator, grad_t_operator, ns_operator)
+    from mirgecom_amd.transport import SimpleTransport
+    from oracle import initializers as oini, operators as oop, physics as ph
+
+    pb = Problem(2, 3, (6, 5), (0.0, -5.0), (10.0, 5.0), (False, False))
+    vortex = Vortex2D(beta=5, center=(5, 0), velocity=(1, 0))
+    tr = dict(viscosity=0.02, thermal_conductivity=0.03, bulk_viscosity=0.01)
+    gm = GasModel(eos=IdealSingleGas(), transport=SimpleTransport(**tr))
+
+    def bstate(dcoll, dd_bdry, gas_model, state_minus, time=0.0, **kw):
+        # exterior = exact vortex at a shifted time, so q_ext != q-
+        return make_fluid_state(vortex(dcoll.nodes(dd_bdry), time=time + 0.3),
+                                gas_model)
+
+    q0 = oini.vortex2d(pb.x, time=0.1, beta=5, center=(5, 0), velocity=(1, 0))
+    st = make_fluid_state(ConservedVars(2, _dev(q0)), gm)
+    bnd = {"all": PrescribedFluidBoundary(boundary_state_func=bstate)}
+    ogm = ph.GasModel(ph.IdealSingleGas(), ph.SimpleTransport(**tr))
+
+    def obstate(xb, sm, time):
+        return ph.make_fluid_state(
+            oini.vortex2d(xb, time + 0.3, beta=5, center=(5, 0), velocity=(1, 0)),
+            ogm, 2)
+
+    for gflux, oflux in [(None, None), (inviscid_facial_flux_hll,
+                                        ph.inviscid_facial_flux_hll)]:
+        kw = {} if gflux is None else {"inviscid_numerical_flux_func": gflux}
+        okw = {} if oflux is None else {"inviscid_numerical_flux_func": oflux}
+        ref, rgcv, rgt = oop.ns_operator(
+            pb.odisc, pb.box, ph.make_fluid_state(q0, ogm, 2), ogm,
+            {"all": oop.PrescribedFluidBoundary(obstate)}, time=0.2,
+            return_gradients=True, **okw)
+        r1 = ns_operator(pb.dcoll, gm, st, bnd, time=0.2, **kw)
+        r2, gcv, gt = ns_operator(pb.dcoll, gm, st, bnd, time=0.2,
+                                  return_gradients=True, **kw)
+        gcv3 = grad_cv_operator(pb.dcoll, gm, bnd, st, time=0.2)
+        gt3 = grad_t_operator(pb.dcoll, gm, bnd, st, time=0.2)
+        r3 = ns_operator(pb.dcoll, gm, st, bnd, time=0.2, grad_cv=gcv3,
+                         grad_t=gt3, **kw)
+        for r in (r1, r2, r3):
+            err = rel_err(_host(r.q), ref)
+            print("ns prescribed", gflux, err)
+            assert err.max() < TOL
+        eg = rel_err(_host(gcv.q).reshape(-1, *rgcv.shape[2:]),
+                     rgcv.reshape(-1, *rgcv.shape[2:]))
+        assert eg.max() < TOL and rel_err(_host(gt), rgt).max() < TOL
+        assert np.array_equal(_host(gcv.q), _host(gcv3.q))
+
+
 def test_ns_own_matrices_noise_floor():
     """Against the oracle built entirely from its own element matrices, p=3:
     the remaining difference is the roundoff of constructing the matrices two
