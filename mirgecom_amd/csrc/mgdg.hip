@@ -186,16 +186,14 @@ struct RhsCfg {
   using E = Elem<DIM, P>;
   static constexpr int NFLD = 2 + DIM + NSP;
   static constexpr int EPB = 64 / NFLD;           // columns fit one wave
-  static constexpr int COLS = 64;
+  static constexpr int COLS = EPB * NFLD;
   static constexpr int NODEF = NFLD * DIM;        // net flux
-  static constexpr int NODEQ = NFLD + 2;          // q, T, wavespeed
   static constexpr int SF = EPB * E::NP * NODEF;
-  static constexpr int SQ = EPB * E::NP * NODEQ;
   static constexpr int SFACE = E::NFN * COLS;
   static constexpr int SGEO = EPB * DIM * DIM;
   static constexpr int STAGE = NFLD * EPB * E::NP;
   static_assert(STAGE <= SF, "staging aliases the flux buffer");
-  static constexpr size_t LDS = sizeof(double) * (SF + SQ + SFACE + SGEO) +
+  static constexpr size_t LDS = sizeof(double) * (SF + SFACE + SGEO) +
                                 sizeof(int) * E::NFN;
   static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
@@ -210,12 +208,11 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   using C = RhsCfg<DIM, P, NSP, MIX, MODE>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
   constexpr int NFLD = C::NFLD, EPB = C::EPB, COLS = C::COLS;
-  constexpr int RPW = O::RPW, NODEF = C::NODEF, NODEQ = C::NODEQ;
+  constexpr int RPW = O::RPW, NODEF = C::NODEF;
   constexpr bool VISC = MODE == kGrad;
   extern __shared__ double smem[];
   double* s_f = smem;                       // [EPB][NP][NFLD][DIM]
-  double* s_q = s_f + C::SF;                // [EPB][NP][NODEQ]
-  double* s_face = s_q + C::SQ;             // [NFN][COLS]
+  double* s_face = s_f + C::SF;             // [NFN][COLS]
   double* s_geo = s_face + C::SFACE;        // [EPB][DIM][DIM]
   int* s_fmask = reinterpret_cast<int*>(s_geo + C::SGEO);
 
@@ -235,15 +232,14 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
     const int64_t e = e0 + le;
     if (e >= a.K) continue;
     const int64_t g = e * NP + i;
-    Node<DIM, NSP> s;
-    load_q<DIM, NSP>(s, a.q, N, g);
-    const double tn = MIX ? a.temp[g] : 0.0;
-    node_state<DIM, NSP, MIX, VISC>(s, a.gas, tn);
     double* fdst = s_f + t * NODEF;
     if constexpr (MODE == kFlux) {
 #pragma unroll
       for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet[k * N + g];
     } else {
+      Node<DIM, NSP> s;
+      load_q<DIM, NSP>(s, a.q, N, g);
+      node_state<DIM, NSP, MIX, VISC>(s, a.gas, MIX ? a.temp[g] : 0.0);
       double F[NFLD][DIM];
       if (inv) {
         inviscid_flux<DIM, NSP>(s, F);
@@ -267,11 +263,6 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
 #pragma unroll
         for (int d = 0; d < DIM; ++d) fdst[f * DIM + d] = F[f][d];
     }
-    double* qdst = s_q + t * NODEQ;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) qdst[f] = s.q[f];
-    qdst[NFLD] = tn;
-    qdst[NFLD + 1] = s.speed + s.c;
   }
   __syncthreads();
 
@@ -292,29 +283,39 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
     const double fs = a.fscale[e * NF + f];
     const int lm = le * NP + im;
     double fnm[NFLD], fnp[NFLD], qm[NFLD], qp[NFLD];
+    const int64_t gm = e * NP + im;
+    double tm, wsm;
+    {
+      Node<DIM, NSP> sm;
+      load_q<DIM, NSP>(sm, a.q, N, gm);
+      tm = MIX ? a.temp[gm] : 0.0;
+      node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
+      wsm = sm.speed + sm.c;
 #pragma unroll
-    for (int k = 0; k < NFLD; ++k) {
-      double c = 0.0;
+      for (int k = 0; k < NFLD; ++k) {
+        double c = 0.0;
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) c += s_f[lm * NODEF + k * DIM + d] * n[d];
-      fnm[k] = c;
-      qm[k] = s_q[lm * NODEQ + k];
+        for (int d = 0; d < DIM; ++d) c += s_f[lm * NODEF + k * DIM + d] * n[d];
+        fnm[k] = c;
+        qm[k] = sm.q[k];
+      }
     }
-    const double tm = s_q[lm * NODEQ + NFLD];
-    const double wsm = s_q[lm * NODEQ + NFLD + 1];
     double tp, wsp;
     if (nb >= g_lo && nb < g_hi) {
       const int lp = int(nb - g_lo);
+      Node<DIM, NSP> sp;
+      load_q<DIM, NSP>(sp, a.q, N, nb);
+      tp = MIX ? a.temp[nb] : 0.0;
+      node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
+      wsp = sp.speed + sp.c;
 #pragma unroll
       for (int k = 0; k < NFLD; ++k) {
         double c = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) c += s_f[lp * NODEF + k * DIM + d] * n[d];
         fnp[k] = c;
-        qp[k] = s_q[lp * NODEQ + k];
+        qp[k] = sp.q[k];
       }
-      tp = s_q[lp * NODEQ + NFLD];
-      wsp = s_q[lp * NODEQ + NFLD + 1];
     } else {
       const bool bnd = nb < 0;
       const int64_t b = bnd ? int64_t(-nb - 1) : 0;
@@ -459,7 +460,8 @@ struct GradCfg {
   static constexpr int KG = E::NP + E::NFN;
   static constexpr int GEO = DIM * DIM + E::NF * DIM;  // rx and normals
   static constexpr int STAGE = NS * DIM * EPB * E::NP;
-  static constexpr size_t LDS = sizeof(double) * (KG * COLS + STAGE + EPB * GEO) +
+  static constexpr int SB = KG * COLS > STAGE ? KG * COLS : STAGE;
+  static constexpr size_t LDS = sizeof(double) * (SB + EPB * GEO) +
                                 sizeof(int) * E::NFN;
   static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
@@ -492,8 +494,8 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   constexpr int RPW = O::RPW, GEO = C::GEO;
   extern __shared__ double smem[];
   double* s_b = smem;                           // [NP + NFN][COLS]
-  double* stage = s_b + C::KG * COLS;           // [NS][DIM][EPB*NP]
-  double* s_geo = stage + C::STAGE;             // [EPB][GEO]
+  double* stage = smem;                         // [NS][DIM][EPB*NP], after phase 3
+  double* s_geo = smem + C::SB;                 // [EPB][GEO]
   int* s_fmask = reinterpret_cast<int*>(s_geo + EPB * GEO);
 
   const int64_t N = a.K * NP;
@@ -599,7 +601,8 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
       for (int rr = 0; rr < RPW; ++rr) accl[rr][f] += GL[(f * NFP + m) * RPW + rr] * bv;
     }
   }
-  // phase 4: stage the gradients [NS][DIM][EPB*NP]
+  __syncthreads();
+  // phase 4: stage the gradients [NS][DIM][EPB*NP] (over the trace buffer)
   if (c < EPB * NS) {
     const int le = c / NS;
     const int sc = c - le * NS;
@@ -631,8 +634,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
       if (e >= a.K) continue;
       const int64_t g = e * NP + i;
       Node<DIM, NSP> s;
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) s.q[f] = s_b[i * COLS + le * NS + f];
+      load_q<DIM, NSP>(s, a.q, N, g);
       node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
       double gq[NFLD][DIM], gt[DIM], F[NFLD][DIM];
 #pragma unroll

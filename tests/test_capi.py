@@ -1,0 +1,93 @@
+"""The C ABI library: it loads without a GPU, exports every entry point that
+include/mgdg.h declares, and rejects bad arguments with an error code and a
+message before touching the device (no compute calls here)."""
+
+import ctypes
+import os
+import re
+
+import pytest
+
+from mirgecom_amd import _lib
+
+HEADER = os.path.join(os.path.dirname(os.path.dirname(__file__)), "include",
+                      "mgdg.h")
+
+
+def _declared():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:mg_status|int64_t|int32_t|const char\*)"
+                                 r"\s+(mg_\w+)\s*\(", text, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = _declared()
+    for n in ("mg_euler_rhs", "mg_ns_gradient", "mg_ns_rhs", "mg_ns_flux",
+              "mg_ns_div", "mg_mixture_temperature", "mg_species_source",
+              "mg_rk_axpy", "mg_mesh_prepare", "mg_mesh_workspace_bytes",
+              "mg_load_mechanism", "mg_last_error", "mg_abi_version"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    for name in _declared():
+        assert hasattr(lib, name), name
+        assert name in _lib.SIGNATURES, name
+    assert lib.mg_abi_version() == 1
+
+
+def test_null_arguments_rejected():
+    lib = _lib.load()
+    rc = lib.mg_euler_rhs(None, None, None, None, None, None, 0, None, None)
+    assert rc == 1
+    assert b"mesh" in lib.mg_last_error()
+    rc = lib.mg_rk_axpy(10, None, None, 1.0, None, None, 0.0, None)
+    assert rc == 1
+
+
+def test_mesh_validation():
+    lib = _lib.load()
+    m = _lib.MgMesh(dim=3, order=3, np=20, nfp=10, nfaces=4, nelements=10,
+                    nbnd_nodes=0)
+    # matrices missing
+    rc = lib.mg_ns_flux(m, _lib.MgGas(transport_kind=1), None, None, None,
+                        None, 1, None, None, None, None)
+    assert rc == 1
+    m.np = 21
+    assert lib.mg_euler_rhs(m, _lib.MgGas(), None, None, None, None, 0, None,
+                            None) == 1
+    assert b"np/nfp" in lib.mg_last_error()
+    m5 = _lib.MgMesh(dim=3, order=5, np=56, nfp=21, nfaces=4, nelements=1)
+    assert lib.mg_mesh_workspace_bytes(m5) == -1
+    m3 = _lib.MgMesh(dim=3, order=3, np=20, nfp=10, nfaces=4, nelements=1)
+    assert lib.mg_mesh_workspace_bytes(m3) > 0
+
+
+def test_mechanism_size_checked():
+    lib = _lib.load()
+    arr = (ctypes.c_double * 3)()
+    assert lib.mg_load_mechanism(arr, 3) == 1
+    assert b"size" in lib.mg_last_error()
+    from mirgecom_amd.mechanisms import packed_mechanism
+    assert packed_mechanism().size == 130
+
+
+def test_gas_validation():
+    lib = _lib.load()
+    g = _lib.MgGas(eos_kind=1, nspecies=5)
+    rc = lib.mg_mixture_temperature(g, 3, 10, None, None, 5, None, None)
+    assert rc == 1
+    assert b"7" in lib.mg_last_error()
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 4), (3, 1), (3, 4)])
+def test_workspace_sizes(dim, order):
+    import math
+    lib = _lib.load()
+    np_ = math.comb(order + dim, dim)
+    nfp = math.comb(order + dim - 1, dim - 1)
+    m = _lib.MgMesh(dim=dim, order=order, np=np_, nfp=nfp, nfaces=dim + 1,
+                    nelements=1)
+    rpw = (np_ + 3) // 4
+    assert lib.mg_mesh_workspace_bytes(m) == 8 * 4 * rpw * (np_ * dim + (dim + 1) * nfp)
