@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=12,
                     help="cells per axis of the CPU-baseline sample")
+    ap.add_argument("--state-steps", type=int, default=200,
+                    help="RK4 steps of the mixed-vs-FP64 state check")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3x), the "
                          "reference kernel for HBM-traffic counter scaling")
@@ -102,6 +104,20 @@ def _tgv_transport():
         reynolds=1600.0, prandtl=0.71, gamma=1.4, gas_const=287.1)
 
 
+def _rel_err_cv(a, b, dim):
+    """Per-field max|a-b|/max|b|; momentum components normalised by the
+    largest momentum component."""
+    import torch
+    out = []
+    nf = b.shape[0]
+    mscale = b[2:2 + dim].abs().max()
+    for f in range(nf):
+        scale = mscale if 2 <= f < 2 + dim else b[f].abs().max()
+        out.append(float((a[f] - b[f]).abs().max() / scale))
+    del torch
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -109,6 +125,8 @@ def main():
               "(the multi-GPU path is a design note, not built)",
               file=sys.stderr)
         sys.exit(2)
+    import math
+
     import torch
 
     from mirgecom_amd import _lib
@@ -117,11 +135,11 @@ def main():
     from mirgecom_amd.eos import IdealSingleGas
     from mirgecom_amd.gas_model import GasModel, make_fluid_state
     from mirgecom_amd.initializers import TaylorGreenVortex
+    from mirgecom_amd.integrators import rk4_step
     from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
     from mirgecom_amd.navierstokes import ns_operator
     from mirgecom_amd.transport import SimpleTransport
 
-    import math
     n, order = args.n, args.order
     t0 = time.perf_counter()
     mesh = generate_box_mesh(3, 0.0, 2 * math.pi, (n,) * 3, periodic=(True,) * 3)
@@ -135,43 +153,73 @@ def main():
     gm = GasModel(eos=IdealSingleGas(gamma=1.4, gas_const=287.1),
                   transport=SimpleTransport(viscosity=mu,
                                             thermal_conductivity=kappa))
-    x = dcoll.nodes()
-    cv = tgv(x, eos=gm.eos)
+    cv = tgv(dcoll.nodes(), eos=gm.eos)
     K, npe = mesh.nelements, dcoll.refel.nnodes
     nfld = 5
     dofs = K * npe * nfld
 
-    def step():
-        st = make_fluid_state(cv, gm)
-        return ns_operator(dcoll, gm, st, {},
-                           inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
+    def rhs(state_cv, precision):
+        return ns_operator(dcoll, gm, make_fluid_state(state_cv, gm), {},
+                           inviscid_numerical_flux_func=inviscid_facial_flux_rusanov,
+                           precision=precision)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    _lib.timing = {}
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    timing = _lib.timing
-    _lib.timing = None
-    kern_ms = {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
-               for k, v in timing.items()}
-    ms = wall / args.steps * 1e3
+    def measure(precision):
+        for _ in range(args.warmup):
+            rhs(cv, precision)
+        torch.cuda.synchronize()
+        _lib.timing = {}
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            rhs(cv, precision)
+        t_issue = time.perf_counter() - t0
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        print(f"# {precision}: host issue {t_issue / args.steps * 1e3:.3f} ms/step, "
+              f"wall {wall / args.steps * 1e3:.3f} ms/step", file=sys.stderr,
+              flush=True)
+        timing = _lib.timing
+        _lib.timing = None
+        kern = {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
+                for k, v in timing.items()}
+        return wall / args.steps * 1e3, kern
+
+    ms64, k64 = measure("float64")
+    ms32, k32 = measure("mixed")
+
+    # accuracy of the mixed path against FP64 on this workload: the RHS, and
+    # the state after RK4 steps at CFL 0.3 (FP64 state and accumulation)
+    err_rhs = _rel_err_cv(rhs(cv, "mixed").q, rhs(cv, "float64").q, 3)
+    h = 2 * math.pi / n
+    c0 = math.sqrt(1.4 * tgv.p0(1.4) / tgv.rho0)
+    dt = 0.3 * h / (order**2 * (tgv.v0 + c0))
+    nsteps = args.state_steps
+    a = b = cv
+    for k in range(nsteps):
+        a = rk4_step(a, k * dt, dt, lambda t, s: rhs(s, "float64"))
+        b = rk4_step(b, k * dt, dt, lambda t, s: rhs(s, "mixed"))
+    err_state = _rel_err_cv(b.q, a.q, 3)
+    mixed_ok = max(err_state) <= 1e-6
+    print(f"# mixed vs fp64: rhs rel err {err_rhs}, state after {nsteps} RK4 "
+          f"steps (dt={dt:.3e}) {err_state}", file=sys.stderr, flush=True)
+
     abytes = alg_bytes(K, npe, dcoll.refel.nface_nodes, 3, nfld)
     aflops = alg_flops(K, npe, dcoll.refel.nface_nodes, 3, nfld)
-    dom = max(kern_ms, key=kern_ms.get)
-    achieved = abytes[dom] / (kern_ms[dom] * 1e-3) / 1e9
-    per_kernel = {k: {"ms": round(kern_ms[k], 4),
-                      "alg_bytes": abytes[k],
-                      "GB_s": round(abytes[k] / (kern_ms[k] * 1e-3) / 1e9, 1),
-                      "TFLOP_s_contractions": round(
-                          aflops[k] / (kern_ms[k] * 1e-3) / 1e12, 2)}
-                  for k in kern_ms}
-    print("# kernels: " + json.dumps(per_kernel), file=sys.stderr, flush=True)
+    # mixed path: nodal fluxes are FP32
+    abytes32 = {"mg_ns_flux_mixed": abytes["mg_ns_flux"] - 4 * K * npe * nfld * 3,
+                "mg_ns_div_mixed": abytes["mg_ns_div"] - 4 * K * npe * nfld * 3}
+    aflops32 = {"mg_ns_flux_mixed": aflops["mg_ns_flux"],
+                "mg_ns_div_mixed": aflops["mg_ns_div"]}
+
+    def table(kern, ab, af):
+        return {k: {"ms": round(kern[k], 4), "alg_bytes": ab[k],
+                    "GB_s": round(ab[k] / (kern[k] * 1e-3) / 1e9, 1),
+                    "TFLOP_s_contractions": round(af[k] / (kern[k] * 1e-3) / 1e12, 2)}
+                for k in kern}
+
+    t64 = table(k64, abytes, aflops)
+    t32 = table(k32, abytes32, aflops32)
+    print("# kernels fp64: " + json.dumps(t64), file=sys.stderr, flush=True)
+    print("# kernels mixed: " + json.dumps(t32), file=sys.stderr, flush=True)
     if args.copy_calibration:
         src = torch.ones(32 * 1024 * 1024, dtype=torch.float64, device=dev)
         dst = torch.empty_like(src)
@@ -183,8 +231,14 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n, order)
+
+    ms = ms32 if mixed_ok else ms64
+    kern, ab = (k32, abytes32) if mixed_ok else (k64, abytes)
+    dom = max(kern, key=kern.get)
+    achieved = ab[dom] / (kern[dom] * 1e-3) / 1e9
     out = {
-        "metric": "DOF-RHS/s (3D NS p=3) mixed vs FP64 at <=1e-6 rel err; chem-surrogate speedup",
+        "metric": "DOF-RHS/s (3D NS p=3) mixed vs FP64 at <=1e-6 rel err; "
+                  "chem-surrogate speedup",
         "value": dofs / (ms * 1e-3),
         "unit": "DOF-RHS/s",
         "n_gpus": 1,
@@ -194,13 +248,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32-mixed (f64 state, physics, RK)" if mixed_ok else "f64",
         "data": "synthetic (Taylor-Green vortex initial state)",
         "config": {"workload": f"3D NS Taylor-Green, periodic [0,2pi]^3, "
                                f"{n}^3x6={K} tets, p={order}, Re=1600, "
                                f"Ma=0.1, Pr=0.71, Rusanov + central viscous",
                    "nodes": K * npe, "fields": nfld, "dofs": dofs,
-                   "kernels": per_kernel},
+                   "fp64": {"value": dofs / (ms64 * 1e-3), "ms_per_step": ms64,
+                            "kernels": t64},
+                   "mixed": {"value": dofs / (ms32 * 1e-3), "ms_per_step": ms32,
+                             "kernels": t32,
+                             "speedup_vs_fp64": ms64 / ms32,
+                             "rel_err_rhs": err_rhs,
+                             "rel_err_state": err_state,
+                             "state_steps": nsteps, "dt": dt}},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},

@@ -132,6 +132,26 @@ mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
                     const double* fnet_bnd, int32_t flux_kind,
                     int32_t inviscid_on, double* rhs, void* stream);
 
+/* Mixed-precision variants of mg_ns_flux / mg_ns_div (the FP32 path of
+ * the same right-hand side): gradients contracted in FP32 from traces held
+ * relative to each element's first node, physics in FP64, nodal net flux
+ * stored in FP32 with the pressure of node 0 (p_ref) removed from the
+ * momentum-flux diagonal, divergence contracted in FP32 from fluxes held
+ * relative to each element's first node, face fluxes formed in FP64.  Both
+ * shifts leave the DG divergence unchanged in exact arithmetic.  The state q
+ * and the RHS stay FP64.  fnet_bnd carries the p_ref shift as well. */
+mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas,
+                           const double* q, const double* temperature,
+                           const double* q_bnd, const double* t_bnd,
+                           int32_t inviscid_on, float* fnet, double* fnet_bnd,
+                           void* stream);
+mg_status mg_ns_div_mixed(const mg_mesh* mesh, const mg_gas* gas,
+                          const double* q, const double* temperature,
+                          const float* fnet, const double* q_bnd,
+                          const double* t_bnd, const double* fnet_bnd,
+                          int32_t flux_kind, int32_t inviscid_on, double* rhs,
+                          void* stream);
+
 /* Mixture temperature by a fixed number of Newton iterations from a seed
  * (replaces PyrometheusMixture.temperature, mirgecom/eos.py:777-803, and
  * get_temperature, mirgecom/thermochemistry.py:126-167). q is [F][n]. */

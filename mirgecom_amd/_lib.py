@@ -63,6 +63,10 @@ SIGNATURES = {
                           _P, _P, _P, _P, _I32, _P, _P, _P, _P]),
     "mg_ns_div": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                          _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
+    "mg_ns_flux_mixed": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                                _P, _P, _P, _P, _I32, _P, _P, _P]),
+    "mg_ns_div_mixed": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                               _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "mg_mixture_temperature": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P,
                                       _P, _I32, _P, _P]),
     "mg_species_source": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,

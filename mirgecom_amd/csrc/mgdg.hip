@@ -55,10 +55,13 @@ struct KArgs {
   const double* ops;   // packed operators (mg_mesh_prepare)
   const double* fnet;  // nodal net flux [F*DIM][N] (kFlux)
   const double* fnb;   // exterior net normal flux [F][nbnd] (kFlux)
+  const float* fnet32; // nodal net flux, FP32 (mixed precision)
+  float* fnet32_out;   // FP32 nodal net flux output (mixed precision)
   double* grad_out;    // optional gradient output of the fused flux kernel
   double* fnb_out;     // exterior net normal flux output of the flux kernel
   int64_t K;
   int64_t nbnd;
+  int64_t nnodes;      // K * NP
   GasDev gas;
   int flux_kind;
   int inviscid_on;
@@ -113,7 +116,8 @@ __device__ __forceinline__ void load_grad(double (*gq)[DIM], double* gt,
 // SGPRs) from a packed copy in which each wave's rows are contiguous:
 //   dvol [w][j][r][rr]     Dw_r[i][j]       (divergence and gradient volume)
 //   lift [w][m][rr]        LIFT[i][m]
-// with i = w*RPW + rr (rows >= NP are zero).
+// with i = w*RPW + rr (rows >= NP are zero).  The workspace holds the FP64
+// copy followed by an FP32 copy for the mixed-precision kernels.
 
 constexpr int kWaves = kBlock / 64;
 
@@ -124,6 +128,8 @@ struct Ops {
   static constexpr int DVOL = kWaves * E::NP * DIM * RPW;
   static constexpr int LIFT = kWaves * E::NFN * RPW;
   static constexpr int TOTAL = DVOL + LIFT;
+  // bytes: TOTAL doubles + TOTAL floats
+  static constexpr int BYTES = TOTAL * 12;
 };
 
 template <int DIM, int P>
@@ -134,37 +140,64 @@ __global__ __launch_bounds__(kBlock) void pack_ops_kernel(const double* dw,
   using O = Ops<DIM, P>;
   constexpr int NP = E::NP, NFN = E::NFN, RPW = O::RPW;
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double v = 0.0;
   if (t < O::DVOL) {
     const int rr = t % RPW;
     const int r = (t / RPW) % DIM;
     const int j = (t / (RPW * DIM)) % NP;
     const int w = t / (RPW * DIM * NP);
     const int i = w * RPW + rr;
-    ws[t] = i < NP ? dw[(r * NP + i) * NP + j] : 0.0;
+    v = i < NP ? dw[(r * NP + i) * NP + j] : 0.0;
   } else if (t < O::TOTAL) {
     const int u = t - O::DVOL;
     const int rr = u % RPW;
     const int m = (u / RPW) % NFN;
     const int w = u / (RPW * NFN);
     const int i = w * RPW + rr;
-    ws[t] = i < NP ? lift[i * NFN + m] : 0.0;
+    v = i < NP ? lift[i * NFN + m] : 0.0;
+  }
+  if (t < O::TOTAL) {
+    ws[t] = v;
+    reinterpret_cast<float*>(ws + O::TOTAL)[t] = float(v);
+  }
+}
+
+template <typename R, int DIM, int P>
+__device__ __forceinline__ const R* packed_ops(const double* ws) {
+  if constexpr (sizeof(R) == 8) {
+    return reinterpret_cast<const R*>(ws);
+  } else {
+    return reinterpret_cast<const R*>(ws + Ops<DIM, P>::TOTAL);
   }
 }
 
 // Coalesced write-out of a block's staged results: stage[plane][le*NP + i]
 // -> out[plane*N + (e0 + le)*NP + i].
-template <int NP>
-__device__ __forceinline__ void write_staged(const double* stage, int nplanes,
+template <int NP, typename S, typename T>
+__device__ __forceinline__ void write_staged(const S* stage, int nplanes,
                                              int epb, int64_t e0, int64_t K,
-                                             double* __restrict__ out) {
+                                             T* __restrict__ out) {
   const int64_t N = K * NP;
   const int64_t nvalid = (K - e0 < epb ? K - e0 : epb) * NP;
   const int per = epb * NP;
   for (int t = threadIdx.x; t < nplanes * per; t += blockDim.x) {
     const int plane = t / per;
     const int idx = t - plane * per;
-    if (idx < nvalid) out[plane * N + e0 * NP + idx] = stage[t];
+    if (idx < nvalid) out[plane * N + e0 * NP + idx] = T(stage[t]);
   }
+}
+
+// Reference pressure of the mixed-precision path: the pressure at the first
+// node of the mesh.  Subtracting a constant p_ref from the momentum flux
+// diagonal does not change the DG divergence (the weak divergence of a
+// constant flux cancels its lift exactly), but it removes the large
+// low-Mach background from the values that are rounded to FP32.
+template <int DIM, int NSP, bool MIX>
+__device__ __forceinline__ double reference_pressure(const KArgs& a) {
+  Node<DIM, NSP> s;
+  load_q<DIM, NSP>(s, a.q, a.nnodes, 0);
+  node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[0] : 0.0);
+  return s.p;
 }
 
 // }}}
@@ -174,14 +207,19 @@ __device__ __forceinline__ void write_staged(const double* stage, int nplanes,
 // Net flux F = F_I - F_V per node, by mode:
 //   kEuler  F_I(q) evaluated here
 //   kGrad   F_I(q) - F_V(q, grad q, grad T) from given gradients
-//   kFlux   read from the nodal-flux array written by flux_kernel
+//   kFlux   read from the nodal-flux array written by grad_kernel<FLUX>
 // Face flux from the net normal fluxes Fn = F.n of both sides:
 //   Rusanov / central  (Fn- + Fn+ + lam (q- - q+)) / 2
 //   HLL                HLL(F_I- . n, F_I+ . n) - (F_V- . n + F_V+ . n) / 2
+// R is the type of the LDS buffers and of the contraction (double, or float
+// for the mixed-precision path, which only exists for kFlux).  With float,
+// each element's fluxes are stored relative to the element's first node
+// (the weak divergence of that constant cancels its lift), so that FP32
+// holds small variations; face fluxes are formed in FP64 and shifted alike.
 
 constexpr int kEuler = 0, kGrad = 1, kFlux = 2;
 
-template <int DIM, int P, int NSP, bool MIX, int MODE>
+template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
 struct RhsCfg {
   using E = Elem<DIM, P>;
   static constexpr int NFLD = 2 + DIM + NSP;
@@ -191,55 +229,82 @@ struct RhsCfg {
   static constexpr int SF = EPB * E::NP * NODEF;
   static constexpr int SFACE = E::NFN * COLS;
   static constexpr int SGEO = EPB * DIM * DIM;
+  static constexpr int SREF = sizeof(R) == 4 ? EPB * NODEF : 0;
+  static constexpr int SWS = EPB * E::NP;          // wavespeed |v| + c per node
   static constexpr int STAGE = NFLD * EPB * E::NP;
   static_assert(STAGE <= SF, "staging aliases the flux buffer");
-  static constexpr size_t LDS = sizeof(double) * (SF + SFACE + SGEO) +
+  static constexpr size_t LDS = sizeof(R) * (SF + SFACE) +
+                                sizeof(double) * (SGEO + SREF + SWS) +
                                 sizeof(int) * E::NFN;
   static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
 
-template <int DIM, int P, int NSP, bool MIX, int MODE>
+template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
 __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
-                                                     const double* __restrict__ ops,
+                                                     const double* __restrict__ ws,
                                                      double* __restrict__ out) {
   using E = Elem<DIM, P>;
   using O = Ops<DIM, P>;
-  using C = RhsCfg<DIM, P, NSP, MIX, MODE>;
+  using C = RhsCfg<DIM, P, NSP, MIX, MODE, R>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
   constexpr int NFLD = C::NFLD, EPB = C::EPB, COLS = C::COLS;
   constexpr int RPW = O::RPW, NODEF = C::NODEF;
   constexpr bool VISC = MODE == kGrad;
+  constexpr bool MIXED = sizeof(R) == 4;
+  static_assert(!MIXED || MODE == kFlux, "mixed precision is the fused NS path");
   extern __shared__ double smem[];
-  double* s_f = smem;                       // [EPB][NP][NFLD][DIM]
-  double* s_face = s_f + C::SF;             // [NFN][COLS]
-  double* s_geo = s_face + C::SFACE;        // [EPB][DIM][DIM]
-  int* s_fmask = reinterpret_cast<int*>(s_geo + C::SGEO);
+  double* s_geo = smem;                                   // [EPB][DIM][DIM]
+  double* s_ref = s_geo + C::SGEO;                        // [EPB][NODEF] (mixed)
+  double* s_ws = s_ref + C::SREF;                         // [EPB][NP]
+  R* s_f = reinterpret_cast<R*>(s_ws + C::SWS);           // [EPB][NP][NFLD][DIM]
+  R* s_face = s_f + C::SF;                                // [NFN][COLS]
+  int* s_fmask = reinterpret_cast<int*>(s_face + C::SFACE);
+  const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
   const int64_t e0 = int64_t(blockIdx.x) * EPB;
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   const bool inv = MODE == kEuler || a.inviscid_on;
+  const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
   for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
   for (int t = threadIdx.x; t < EPB * DIM * DIM; t += blockDim.x)
     s_geo[t] = t < nel * DIM * DIM ? a.rx[e0 * DIM * DIM + t] : 0.0;
+  if constexpr (MIXED) {
+    for (int t = threadIdx.x; t < EPB * NODEF; t += blockDim.x) {
+      const int le = t / NODEF;
+      const int k = t - le * NODEF;
+      s_ref[t] = le < nel ? double(a.fnet32[k * N + (e0 + le) * NP]) : 0.0;
+    }
+  }
 
-  // phase 1: state, wavespeed and net flux at the volume nodes (coalesced)
+  // phase 1: net flux at the volume nodes (coalesced)
   for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
     const int le = t / NP;
     const int i = t - le * NP;
     const int64_t e = e0 + le;
     if (e >= a.K) continue;
     const int64_t g = e * NP + i;
-    double* fdst = s_f + t * NODEF;
+    R* fdst = s_f + t * NODEF;
     if constexpr (MODE == kFlux) {
+      Node<DIM, NSP> s;
+      load_q<DIM, NSP>(s, a.q, N, g);
+      node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[g] : 0.0);
+      s_ws[t] = s.speed + s.c;
+      if constexpr (MIXED) {
+        const int64_t g0 = e * NP;
 #pragma unroll
-      for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet[k * N + g];
+        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet32[k * N + g] - a.fnet32[k * N + g0];
+      } else {
+#pragma unroll
+        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet[k * N + g];
+      }
     } else {
       Node<DIM, NSP> s;
       load_q<DIM, NSP>(s, a.q, N, g);
       node_state<DIM, NSP, MIX, VISC>(s, a.gas, MIX ? a.temp[g] : 0.0);
+      s_ws[t] = s.speed + s.c;
       double F[NFLD][DIM];
       if (inv) {
         inviscid_flux<DIM, NSP>(s, F);
@@ -266,8 +331,8 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   }
   __syncthreads();
 
-  // phase 2: numerical face fluxes.  The own side and neighbours inside this
-  // workgroup come from LDS; other neighbours are gathered here.
+  // phase 2: numerical face fluxes (FP64).  The own side and neighbours
+  // inside this workgroup come from LDS; other neighbours are gathered here.
   const bool hll = inv && a.flux_kind == 1;
   for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
     const int le = t / NFN;
@@ -282,39 +347,40 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
     for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + f) * DIM + d];
     const double fs = a.fscale[e * NF + f];
     const int lm = le * NP + im;
-    double fnm[NFLD], fnp[NFLD], qm[NFLD], qp[NFLD];
-    const int64_t gm = e * NP + im;
-    double tm, wsm;
-    {
-      Node<DIM, NSP> sm;
-      load_q<DIM, NSP>(sm, a.q, N, gm);
-      tm = MIX ? a.temp[gm] : 0.0;
-      node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
-      wsm = sm.speed + sm.c;
+    double fnm[NFLD], fnp[NFLD], qm[NFLD], qp[NFLD], shift[NFLD];
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) {
-        double c = 0.0;
+    for (int k = 0; k < NFLD; ++k) {
+      double c = 0.0, sh = 0.0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) c += s_f[lm * NODEF + k * DIM + d] * n[d];
-        fnm[k] = c;
-        qm[k] = sm.q[k];
+      for (int d = 0; d < DIM; ++d) {
+        const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
+        c += (double(s_f[lm * NODEF + k * DIM + d]) + r0) * n[d];
+        sh += r0 * n[d];
       }
+      fnm[k] = c;
+      shift[k] = sh;
     }
+    const int64_t gm = e * NP + im;
+    const double tm = MIX ? a.temp[gm] : 0.0;
+    const double wsm = s_ws[lm];
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) qm[k] = a.q[k * N + gm];
     double tp, wsp;
     if (nb >= g_lo && nb < g_hi) {
       const int lp = int(nb - g_lo);
-      Node<DIM, NSP> sp;
-      load_q<DIM, NSP>(sp, a.q, N, nb);
+      const int ple = lp / NP;
       tp = MIX ? a.temp[nb] : 0.0;
-      node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
-      wsp = sp.speed + sp.c;
+      wsp = s_ws[lp];
+#pragma unroll
+      for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
 #pragma unroll
       for (int k = 0; k < NFLD; ++k) {
         double c = 0.0;
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) c += s_f[lp * NODEF + k * DIM + d] * n[d];
+        for (int d = 0; d < DIM; ++d)
+          c += (double(s_f[lp * NODEF + k * DIM + d]) +
+                (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
         fnp[k] = c;
-        qp[k] = sp.q[k];
       }
     } else {
       const bool bnd = nb < 0;
@@ -337,7 +403,9 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
           for (int k = 0; k < NFLD; ++k) {
             double c = 0.0;
 #pragma unroll
-            for (int d = 0; d < DIM; ++d) c += a.fnet[(k * DIM + d) * N + nb] * n[d];
+            for (int d = 0; d < DIM; ++d)
+              c += (MIXED ? double(a.fnet32[(k * DIM + d) * N + nb])
+                          : a.fnet[(k * DIM + d) * N + nb]) * n[d];
             fnp[k] = c;
           }
         } else {
@@ -374,16 +442,20 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
       inviscid_flux_n<DIM, NSP>(sm, n, fim);
       inviscid_flux_n<DIM, NSP>(sp, n, fip);
       inviscid_numflux<DIM, NSP>(sm, sp, n, 1, fl);
+      // stored net fluxes exclude p_ref I: F_V.n = F_I.n - (Fn + p_ref n)
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k)
-        fl[k] -= ((fim[k] - fnm[k]) + (fip[k] - fnp[k])) / 2;
+      for (int k = 0; k < NFLD; ++k) {
+        const double pn = (k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0;
+        fl[k] -= ((fim[k] - fnm[k] - pn) + (fip[k] - fnp[k] - pn)) / 2 + pn;
+      }
     } else {
       const double lam = (inv && a.flux_kind == 0) ? fmax(wsm, wsp) : 0.0;
 #pragma unroll
       for (int k = 0; k < NFLD; ++k) fl[k] = (fnm[k] + fnp[k] + lam * (qm[k] - qp[k])) / 2;
     }
 #pragma unroll
-    for (int k = 0; k < NFLD; ++k) s_face[fn * COLS + le * NFLD + k] = -(fs * fl[k]);
+    for (int k = 0; k < NFLD; ++k)
+      s_face[fn * COLS + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
   }
   __syncthreads();
 
@@ -392,27 +464,28 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   // the waves, operator entries wave-uniform
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = threadIdx.x & 63;
-  const int cle = c < EPB * NFLD ? c / NFLD : 0;
-  const int cf = c < EPB * NFLD ? c - cle * NFLD : 0;
-  double rxl[DIM][DIM];
+  const int cc = c < COLS ? c : 0;
+  const int cle = cc / NFLD;
+  const int cf = cc - cle * NFLD;
+  R rxl[DIM][DIM];
 #pragma unroll
   for (int r = 0; r < DIM; ++r)
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) rxl[r][d] = s_geo[cle * DIM * DIM + r * DIM + d];
-  const double* DV = ops + w * (NP * DIM * RPW);
-  const double* LF = ops + O::DVOL + w * (NFN * RPW);
-  double acc[RPW];
+    for (int d = 0; d < DIM; ++d) rxl[r][d] = R(s_geo[cle * DIM * DIM + r * DIM + d]);
+  const R* DV = ops + w * (NP * DIM * RPW);
+  const R* LF = ops + O::DVOL + w * (NFN * RPW);
+  R acc[RPW];
 #pragma unroll
-  for (int rr = 0; rr < RPW; ++rr) acc[rr] = 0.0;
+  for (int rr = 0; rr < RPW; ++rr) acc[rr] = R(0);
 #pragma unroll 2
   for (int j = 0; j < NP; ++j) {
-    const double* fj = s_f + (cle * NP + j) * NODEF + cf * DIM;
-    double fd[DIM];
+    const R* fj = s_f + (cle * NP + j) * NODEF + cf * DIM;
+    R fd[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) fd[d] = fj[d];
 #pragma unroll
     for (int r = 0; r < DIM; ++r) {
-      double gr = 0.0;
+      R gr = R(0);
 #pragma unroll
       for (int d = 0; d < DIM; ++d) gr += rxl[r][d] * fd[d];
 #pragma unroll
@@ -421,14 +494,14 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   }
 #pragma unroll 4
   for (int m = 0; m < NFN; ++m) {
-    const double bv = s_face[m * COLS + c];
+    const R bv = s_face[m * COLS + cc];
 #pragma unroll
     for (int rr = 0; rr < RPW; ++rr) acc[rr] += LF[m * RPW + rr] * bv;
   }
   __syncthreads();
   // phase 4: stage in LDS (over s_f), write coalesced
-  double* stage = s_f;                       // [NFLD][EPB*NP]
-  if (c < EPB * NFLD) {
+  R* stage = s_f;                            // [NFLD][EPB*NP]
+  if (c < COLS) {
 #pragma unroll
     for (int rr = 0; rr < RPW; ++rr) {
       const int i = w * RPW + rr;
@@ -450,8 +523,13 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
 // pointwise) and writes that instead of (or besides) the gradients; for
 // boundary face nodes it also writes (F_I - F_V)(q_ext, grad q-) . n, the
 // exterior side of the central viscous boundary flux.
+// With R = float (mixed precision) the traces are held relative to each
+// element's first node (the DG gradient of a constant is zero, and the
+// central face values are shifted by the same constant), the contraction
+// runs in FP32, the physics in FP64 and the nodal fluxes are stored in FP32
+// with p_ref removed from the momentum-flux diagonal.
 
-template <int DIM, int P, int NSP, bool MIX>
+template <int DIM, int P, int NSP, bool MIX, typename R>
 struct GradCfg {
   using E = Elem<DIM, P>;
   static constexpr int NS = 3 + DIM + NSP;  // conserved fields + T
@@ -461,7 +539,8 @@ struct GradCfg {
   static constexpr int GEO = DIM * DIM + E::NF * DIM;  // rx and normals
   static constexpr int STAGE = NS * DIM * EPB * E::NP;
   static constexpr int SB = KG * COLS > STAGE ? KG * COLS : STAGE;
-  static constexpr size_t LDS = sizeof(double) * (SB + EPB * GEO) +
+  static constexpr int SREF = sizeof(R) == 4 ? EPB * NS : 0;
+  static constexpr size_t LDS = sizeof(R) * SB + sizeof(double) * (EPB * GEO + SREF) +
                                 sizeof(int) * E::NFN;
   static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
@@ -482,21 +561,25 @@ __device__ __forceinline__ double node_temperature(const double* qv,
   }
 }
 
-template <int DIM, int P, int NSP, bool MIX, bool FLUX>
+template <int DIM, int P, int NSP, bool MIX, bool FLUX, typename R>
 __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
-                                                      const double* __restrict__ ops,
+                                                      const double* __restrict__ ws,
                                                       double* __restrict__ out) {
   using E = Elem<DIM, P>;
   using O = Ops<DIM, P>;
-  using C = GradCfg<DIM, P, NSP, MIX>;
+  using C = GradCfg<DIM, P, NSP, MIX, R>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
   constexpr int NFLD = 2 + DIM + NSP, NS = C::NS, EPB = C::EPB, COLS = C::COLS;
   constexpr int RPW = O::RPW, GEO = C::GEO;
+  constexpr bool MIXED = sizeof(R) == 4;
+  static_assert(!MIXED || FLUX, "mixed precision is the fused NS path");
   extern __shared__ double smem[];
-  double* s_b = smem;                           // [NP + NFN][COLS]
-  double* stage = smem;                         // [NS][DIM][EPB*NP], after phase 3
-  double* s_geo = smem + C::SB;                 // [EPB][GEO]
-  int* s_fmask = reinterpret_cast<int*>(s_geo + EPB * GEO);
+  double* s_geo = smem;                          // [EPB][GEO]
+  double* s_ref = s_geo + EPB * GEO;             // [EPB][NS] (mixed)
+  R* s_b = reinterpret_cast<R*>(s_ref + C::SREF);  // [NP + NFN][COLS]
+  R* stage = s_b;                                // [NS][DIM][EPB*NP], after phase 3
+  int* s_fmask = reinterpret_cast<int*>(s_b + C::SB);
+  const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
   const int64_t e0 = int64_t(blockIdx.x) * EPB;
@@ -512,6 +595,26 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
       v = u < DIM * DIM ? a.rx[e * DIM * DIM + u] : a.normals[e * NF * DIM + u - DIM * DIM];
     s_geo[t] = v;
   }
+  if constexpr (MIXED) {
+    // per-element reference values: q and T at the element's first node
+    for (int t = threadIdx.x; t < EPB; t += blockDim.x) {
+      double qv[NFLD];
+      if (t < nel) {
+        const int64_t g0 = (e0 + t) * NP;
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) qv[f] = a.q[f * N + g0];
+        s_ref[t * NS + NFLD] =
+            node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g0] : 0.0);
+      } else {
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) qv[f] = 0.0;
+        s_ref[t * NS + NFLD] = 0.0;
+      }
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s_ref[t * NS + f] = qv[f];
+    }
+    __syncthreads();
+  }
 
   // phase 1: conserved fields and T at the volume nodes
   for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
@@ -523,10 +626,11 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
     double qv[NFLD];
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) qv[f] = a.q[f * N + g];
+    const double tn = node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g] : 0.0);
+    const double* ref = s_ref + le * NS;
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_b[i * COLS + le * NS + f] = qv[f];
-    s_b[i * COLS + le * NS + NFLD] =
-        node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g] : 0.0);
+    for (int f = 0; f < NFLD; ++f) s_b[i * COLS + le * NS + f] = R(qv[f] - (MIXED ? ref[f] : 0.0));
+    s_b[i * COLS + le * NS + NFLD] = R(tn - (MIXED ? ref[NFLD] : 0.0));
   }
   __syncthreads();
 
@@ -541,31 +645,38 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
     const int im = s_fmask[fn];
     const int32_t nb = a.vmap_p[e * NFN + fn];
     const double fs = a.fscale[e * NF + f];
-    double* dst = s_b + (NP + fn) * COLS + le * NS;
-    const double* own = s_b + im * COLS + le * NS;
+    R* dst = s_b + (NP + fn) * COLS + le * NS;
+    const R* own = s_b + im * COLS + le * NS;
+    const double* ref = s_ref + le * NS;
     if (nb >= g_lo && nb < g_hi) {
       const int lp = int(nb - g_lo);
       const int ple = lp / NP;
-      const double* nbv = s_b + (lp - ple * NP) * COLS + ple * NS;
+      const R* nbv = s_b + (lp - ple * NP) * COLS + ple * NS;
+      const double* pref = s_ref + ple * NS;
 #pragma unroll
-      for (int k = 0; k <= NFLD; ++k) dst[k] = fs * ((nbv[k] + own[k]) / 2);
-    } else if (nb >= 0) {
-      double qp[NFLD];
-#pragma unroll
-      for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
-      const double tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.temp[nb] : 0.0);
-#pragma unroll
-      for (int k = 0; k < NFLD; ++k) dst[k] = fs * ((qp[k] + own[k]) / 2);
-      dst[NFLD] = fs * ((tp + own[NFLD]) / 2);
+      for (int k = 0; k <= NFLD; ++k) {
+        const double dref = MIXED ? pref[k] - ref[k] : 0.0;
+        dst[k] = R(fs * ((double(nbv[k]) + dref + double(own[k])) / 2));
+      }
     } else {
-      const int64_t b = -int64_t(nb) - 1;
       double qp[NFLD];
+      double tp;
+      const bool bnd = nb < 0;
+      if (!bnd) {
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) qp[k] = a.qb[k * a.nbnd + b];
-      const double tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.tb[b] : 0.0);
+        for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
+        tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.temp[nb] : 0.0);
+      } else {
+        const int64_t b = -int64_t(nb) - 1;
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) dst[k] = fs * qp[k];
-      dst[NFLD] = fs * tp;
+        for (int k = 0; k < NFLD; ++k) qp[k] = a.qb[k * a.nbnd + b];
+        tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.tb[b] : 0.0);
+      }
+#pragma unroll
+      for (int k = 0; k <= NFLD; ++k) {
+        const double up = (k < NFLD ? qp[k] : tp) - (MIXED ? ref[k] : 0.0);
+        dst[k] = R(bnd ? fs * up : fs * ((up + double(own[k])) / 2));
+      }
     }
   }
   __syncthreads();
@@ -574,19 +685,19 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   // grad_d = sum_f n_f[d] L_f - sum_r rx[r][d] W_r
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = threadIdx.x & 63;
-  const double* GV = ops + w * (NP * DIM * RPW);
-  const double* GL = ops + O::DVOL + w * (NFN * RPW);
-  double accw[RPW][DIM], accl[RPW][NF];
+  const R* GV = ops + w * (NP * DIM * RPW);
+  const R* GL = ops + O::DVOL + w * (NFN * RPW);
+  R accw[RPW][DIM], accl[RPW][NF];
 #pragma unroll
   for (int rr = 0; rr < RPW; ++rr) {
 #pragma unroll
-    for (int r = 0; r < DIM; ++r) accw[rr][r] = 0.0;
+    for (int r = 0; r < DIM; ++r) accw[rr][r] = R(0);
 #pragma unroll
-    for (int f = 0; f < NF; ++f) accl[rr][f] = 0.0;
+    for (int f = 0; f < NF; ++f) accl[rr][f] = R(0);
   }
 #pragma unroll 2
   for (int j = 0; j < NP; ++j) {
-    const double bv = s_b[j * COLS + c];
+    const R bv = s_b[j * COLS + c];
 #pragma unroll
     for (int r = 0; r < DIM; ++r)
 #pragma unroll
@@ -596,7 +707,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   for (int f = 0; f < NF; ++f) {
 #pragma unroll 2
     for (int m = 0; m < NFP; ++m) {
-      const double bv = s_b[(NP + f * NFP + m) * COLS + c];
+      const R bv = s_b[(NP + f * NFP + m) * COLS + c];
 #pragma unroll
       for (int rr = 0; rr < RPW; ++rr) accl[rr][f] += GL[(f * NFP + m) * RPW + rr] * bv;
     }
@@ -613,19 +724,21 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
       for (int rr = 0; rr < RPW; ++rr) {
         const int i = w * RPW + rr;
         if (i >= NP) continue;
-        double vsum = 0.0, fsum = 0.0;
+        R vsum = R(0), fsum = R(0);
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) vsum += geo[r * DIM + d] * accw[rr][r];
+        for (int r = 0; r < DIM; ++r) vsum += R(geo[r * DIM + d]) * accw[rr][r];
 #pragma unroll
-        for (int f = 0; f < NF; ++f) fsum += geo[DIM * DIM + f * DIM + d] * accl[rr][f];
+        for (int f = 0; f < NF; ++f) fsum += R(geo[DIM * DIM + f * DIM + d]) * accl[rr][f];
         stage[(sc * DIM + d) * (EPB * NP) + le * NP + i] = fsum - vsum;
       }
     }
   }
   __syncthreads();
-  if (!FLUX || a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, e0, a.K,
-                                             FLUX ? a.grad_out : out);
-  if constexpr (FLUX) {
+  if constexpr (!FLUX) {
+    write_staged<NP>(stage, NS * DIM, EPB, e0, a.K, out);
+  } else {
+    if (a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, e0, a.K, a.grad_out);
+    const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
     // phase 5: net flux F_I - F_V at every node, written coalesced
     for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
       const int le = t / NP;
@@ -640,9 +753,9 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 #pragma unroll
       for (int f = 0; f < NFLD; ++f)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) gq[f][d] = stage[(f * DIM + d) * (EPB * NP) + t];
+        for (int d = 0; d < DIM; ++d) gq[f][d] = double(stage[(f * DIM + d) * (EPB * NP) + t]);
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) gt[d] = stage[(NFLD * DIM + d) * (EPB * NP) + t];
+      for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * (EPB * NP) + t]);
       viscous_flux<DIM, NSP, MIX>(s, gq, gt, F);
       double FI[NFLD][DIM];
       if (a.inviscid_on) {
@@ -656,7 +769,14 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 #pragma unroll
       for (int f = 0; f < NFLD; ++f)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) out[(f * DIM + d) * N + g] = FI[f][d] - F[f][d];
+        for (int d = 0; d < DIM; ++d) {
+          const double v = FI[f][d] - F[f][d] - ((f == 2 + d) ? p_ref : 0.0);
+          if constexpr (MIXED) {
+            a.fnet32_out[(f * DIM + d) * N + g] = float(v);
+          } else {
+            out[(f * DIM + d) * N + g] = v;
+          }
+        }
     }
     // phase 6: exterior net normal flux at boundary face nodes, with the
     // interior gradients (boundary.py:944-970)
@@ -682,9 +802,10 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 #pragma unroll
         for (int k = 0; k < NFLD; ++k)
 #pragma unroll
-          for (int d = 0; d < DIM; ++d) gq[k][d] = stage[(k * DIM + d) * (EPB * NP) + ti];
+          for (int d = 0; d < DIM; ++d)
+            gq[k][d] = double(stage[(k * DIM + d) * (EPB * NP) + ti]);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) gt[d] = stage[(NFLD * DIM + d) * (EPB * NP) + ti];
+        for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * (EPB * NP) + ti]);
         viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
         if (a.inviscid_on) {
           inviscid_flux_n<DIM, NSP>(s, n, fi);
@@ -693,7 +814,9 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
           for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < NFLD; ++k) a.fnb_out[k * a.nbnd + b] = fi[k] - fv[k];
+        for (int k = 0; k < NFLD; ++k)
+          a.fnb_out[k * a.nbnd + b] =
+              fi[k] - fv[k] - ((k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0);
       }
     }
   }
@@ -839,6 +962,7 @@ mgdg::KArgs make_args(const mg_mesh* m, const mg_gas* g) {
   a.ops = m->workspace;
   a.K = m->nelements;
   a.nbnd = m->nbnd_nodes;
+  a.nnodes = m->nelements * m->np;
   a.gas = to_dev(g);
   return a;
 }
@@ -853,10 +977,10 @@ mg_status set_lds(Kern kern, size_t lds) {
   return MG_OK;
 }
 
-template <int DIM, int P, int NSP, bool MIX, int MODE>
+template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
 mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
-  using C = mgdg::RhsCfg<DIM, P, NSP, MIX, MODE>;
-  auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, MODE>;
+  using C = mgdg::RhsCfg<DIM, P, NSP, MIX, MODE, R>;
+  auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, MODE, R>;
   static bool attr = false;
   if (!attr) {
     mg_status s = set_lds(kern, C::LDS);
@@ -869,10 +993,10 @@ mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
   return check_launch("rhs_kernel");
 }
 
-template <int DIM, int P, int NSP, bool MIX, bool FLUX>
+template <int DIM, int P, int NSP, bool MIX, bool FLUX, typename R>
 mg_status launch_grad(const mgdg::KArgs& a, hipStream_t st) {
-  using C = mgdg::GradCfg<DIM, P, NSP, MIX>;
-  auto kern = mgdg::grad_kernel<DIM, P, NSP, MIX, FLUX>;
+  using C = mgdg::GradCfg<DIM, P, NSP, MIX, R>;
+  auto kern = mgdg::grad_kernel<DIM, P, NSP, MIX, FLUX, R>;
   static bool attr = false;
   if (!attr) {
     mg_status s = set_lds(kern, C::LDS);
@@ -886,20 +1010,20 @@ mg_status launch_grad(const mgdg::KArgs& a, hipStream_t st) {
 }
 
 // compiled (species count, EOS) combinations: (0, ideal), (7, ideal), (7, mixture)
-template <int DIM, int P, int MODE>
+template <int DIM, int P, int MODE, typename R>
 mg_status rhs_species(const mgdg::KArgs& a, int nsp, bool mix, hipStream_t st) {
-  if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, false, MODE>(a, st);
-  if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, false, MODE>(a, st);
-  if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, true, MODE>(a, st);
+  if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, false, MODE, R>(a, st);
+  if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, false, MODE, R>(a, st);
+  if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, true, MODE, R>(a, st);
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
-template <int DIM, int P, bool FLUX>
+template <int DIM, int P, bool FLUX, typename R>
 mg_status grad_species(const mgdg::KArgs& a, int nsp, bool mix, hipStream_t st) {
-  if (nsp == 0 && !mix) return launch_grad<DIM, P, 0, false, FLUX>(a, st);
-  if (nsp == 7 && !mix) return launch_grad<DIM, P, 7, false, FLUX>(a, st);
-  if (nsp == 7 && mix) return launch_grad<DIM, P, 7, true, FLUX>(a, st);
+  if (nsp == 0 && !mix) return launch_grad<DIM, P, 0, false, FLUX, R>(a, st);
+  if (nsp == 7 && !mix) return launch_grad<DIM, P, 7, false, FLUX, R>(a, st);
+  if (nsp == 7 && mix) return launch_grad<DIM, P, 7, true, FLUX, R>(a, st);
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
@@ -908,17 +1032,19 @@ template <int DIM, int P>
 mg_status dispatch_p(const mgdg::KArgs& a, int nsp, bool mix, int what,
                      hipStream_t st) {
   switch (what) {
-    case 0: return rhs_species<DIM, P, mgdg::kEuler>(a, nsp, mix, st);
-    case 1: return rhs_species<DIM, P, mgdg::kGrad>(a, nsp, mix, st);
-    case 2: return rhs_species<DIM, P, mgdg::kFlux>(a, nsp, mix, st);
-    case 3: return grad_species<DIM, P, false>(a, nsp, mix, st);
-    case 4: return grad_species<DIM, P, true>(a, nsp, mix, st);
+    case 0: return rhs_species<DIM, P, mgdg::kEuler, double>(a, nsp, mix, st);
+    case 1: return rhs_species<DIM, P, mgdg::kGrad, double>(a, nsp, mix, st);
+    case 2: return rhs_species<DIM, P, mgdg::kFlux, double>(a, nsp, mix, st);
+    case 3: return grad_species<DIM, P, false, double>(a, nsp, mix, st);
+    case 4: return grad_species<DIM, P, true, double>(a, nsp, mix, st);
+    case 5: return grad_species<DIM, P, true, float>(a, nsp, mix, st);
+    case 6: return rhs_species<DIM, P, mgdg::kFlux, float>(a, nsp, mix, st);
   }
   return fail(MG_ERR_ARG, "kernel selector");
 }
 
 // what: 0 Euler RHS, 1 NS RHS from gradients, 2 NS RHS from nodal fluxes,
-// 3 gradients, 4 gradients + nodal fluxes
+// 3 gradients, 4 gradients + nodal fluxes, 5/6 the mixed-precision 4/2
 mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, bool mix,
                    int what, hipStream_t st) {
 #define MG_CASE(D, P) \
@@ -983,11 +1109,11 @@ int64_t mg_mesh_workspace_bytes(const mg_mesh* m) {
   if (!m) return -1;
   int total = -1;
 #define MG_WS_CASE(D, P) \
-  if (m->dim == D && m->order == P) total = mgdg::Ops<D, P>::TOTAL;
+  if (m->dim == D && m->order == P) total = mgdg::Ops<D, P>::BYTES;
   MG_WS_CASE(2, 1) MG_WS_CASE(2, 2) MG_WS_CASE(2, 3) MG_WS_CASE(2, 4)
   MG_WS_CASE(3, 1) MG_WS_CASE(3, 2) MG_WS_CASE(3, 3) MG_WS_CASE(3, 4)
 #undef MG_WS_CASE
-  return total < 0 ? -1 : int64_t(total) * int64_t(sizeof(double));
+  return total < 0 ? -1 : int64_t(total);
 }
 
 mg_status mg_mesh_prepare(mg_mesh* m, double* workspace, void* stream) {
@@ -1126,6 +1252,58 @@ mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.flux_kind = flux_kind;
   a.inviscid_on = inviscid_on ? 1 : 0;
   return dispatch(mesh, a, gas->nspecies, mix, 2, static_cast<hipStream_t>(stream));
+}
+
+mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                           const double* temperature, const double* q_bnd,
+                           const double* t_bnd, int32_t inviscid_on, float* fnet,
+                           double* fnet_bnd, void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, true))) return s;
+  const bool mix = gas->eos_kind == 1;
+  if (mix && (s = ensure_mech())) return s;
+  if (!q || !fnet) return fail(MG_ERR_ARG, "q/fnet NULL");
+  if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
+  if (mesh->nbnd_nodes > 0 && (!q_bnd || !fnet_bnd || (mix && !t_bnd)))
+    return fail(MG_ERR_ARG, "boundary exterior state / flux buffer missing");
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.temp = temperature;
+  a.qb = q_bnd;
+  a.tb = t_bnd;
+  a.fnet32_out = fnet;
+  a.fnb_out = fnet_bnd;
+  a.inviscid_on = inviscid_on ? 1 : 0;
+  return dispatch(mesh, a, gas->nspecies, mix, 5, static_cast<hipStream_t>(stream));
+}
+
+mg_status mg_ns_div_mixed(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                          const double* temperature, const float* fnet,
+                          const double* q_bnd, const double* t_bnd,
+                          const double* fnet_bnd, int32_t flux_kind,
+                          int32_t inviscid_on, double* rhs, void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, true))) return s;
+  const bool mix = gas->eos_kind == 1;
+  if (mix && (s = ensure_mech())) return s;
+  if (!q || !fnet || !rhs) return fail(MG_ERR_ARG, "q/fnet/rhs NULL");
+  if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
+  if (mesh->nbnd_nodes > 0 && (!q_bnd || !fnet_bnd || (mix && !t_bnd)))
+    return fail(MG_ERR_ARG, "boundary exterior state / flux missing");
+  if (flux_kind < 0 || flux_kind > 2) return fail(MG_ERR_ARG, "flux_kind");
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.temp = temperature;
+  a.fnet32 = fnet;
+  a.fnb = fnet_bnd;
+  a.qb = q_bnd;
+  a.tb = t_bnd;
+  a.out = rhs;
+  a.flux_kind = flux_kind;
+  a.inviscid_on = inviscid_on ? 1 : 0;
+  return dispatch(mesh, a, gas->nspecies, mix, 6, static_cast<hipStream_t>(stream));
 }
 
 mg_status mg_mixture_temperature(const mg_gas* gas, int32_t dim, int64_t n,

@@ -62,7 +62,7 @@ class DeviceDiscretization:
         if nbytes <= 0:
             raise NotImplementedError(
                 f"dim={dcoll.dim} order={re.order} has no device kernels")
-        self.workspace = torch.empty(nbytes // 8, **f64)
+        self.workspace = torch.empty((nbytes + 7) // 8, **f64)
         _lib.check(lib.mg_mesh_prepare(self.mesh, _lib.ptr(self.workspace),
                                        _lib.stream_handle()))
 

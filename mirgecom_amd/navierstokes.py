@@ -73,10 +73,14 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
                 dd=DD_VOLUME_ALL, comm_tag=None, limiter_func=None,
                 entropy_min=None, operator_states_quad=None, use_esdg=False,
                 grad_cv=None, grad_t=None, inviscid_terms_on=True,
-                entropy_conserving_flux_func=None):
+                entropy_conserving_flux_func=None, precision="float64"):
     r"""RHS of the Navier-Stokes equations,
     :math:`\partial_t Q = \nabla\cdot(F_V - F_I)` (mirgecom/navierstokes.py:
-    350-575, fused inviscid path)."""
+    350-575, fused inviscid path).
+
+    ``precision`` (not in the reference signature): ``"float64"`` (default)
+    or ``"mixed"`` -- FP32 contractions and FP32 nodal fluxes with FP64
+    physics, FP64 state and FP64 RHS (mg_ns_flux_mixed / mg_ns_div_mixed)."""
     if not state.is_viscous:
         raise ValueError("Navier-Stokes operator expects viscous gas model.")
     check_dd(dd)
@@ -92,7 +96,29 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
     warn_mixture_limiter(state, limiter_func)
     kind = flux_kind(inviscid_numerical_flux_func)
     inv = 1 if inviscid_terms_on else 0
-    if grad_cv is None and grad_t is None:
+    if precision not in ("float64", "mixed"):
+        raise ValueError("precision must be 'float64' or 'mixed'")
+    if precision == "mixed":
+        if return_gradients or grad_cv is not None or grad_t is not None:
+            raise NotImplementedError("the mixed-precision path computes the "
+                                      "gradients internally")
+        prep = Prepared(dcoll, state, gas_model, boundaries, time)
+        nf, dim = prep.q.shape[0], prep.dim
+        fnet = torch.empty((nf * dim,) + tuple(prep.q.shape[1:]),
+                           dtype=torch.float32, device=prep.q.device)
+        fnb = None
+        if prep.qb is not None:
+            fnb = torch.empty((nf, prep.qb.shape[1]), dtype=prep.q.dtype,
+                              device=prep.q.device)
+        _lib.call("mg_ns_flux_mixed", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+                  _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb),
+                  inv, _lib.ptr(fnet), _lib.ptr(fnb), prep.stream)
+        rhs = torch.empty_like(prep.q)
+        _lib.call("mg_ns_div_mixed", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+                  _lib.ptr(prep.temp), _lib.ptr(fnet), _lib.ptr(prep.qb),
+                  _lib.ptr(prep.tb), _lib.ptr(fnb), kind, inv, _lib.ptr(rhs),
+                  prep.stream)
+    elif grad_cv is None and grad_t is None:
         # fused path: gradients -> nodal net flux (one kernel), divergence
         prep = Prepared(dcoll, state, gas_model, boundaries, time)
         nf, dim = prep.q.shape[0], prep.dim

@@ -46,6 +46,20 @@ def rel_err(gpu, ref):
     return np.array(out)
 
 
+def rel_err_cv(gpu, ref, dim):
+    """Per-field relative error with the momentum components normalised by
+    the largest momentum component (a vector field; a component that starts
+    at zero, like w of the Taylor-Green vortex, would otherwise divide by a
+    vanishing norm)."""
+    gpu = np.asarray(gpu)
+    ref = np.asarray(ref)
+    out = rel_err(gpu, ref)
+    mom = slice(2, 2 + dim)
+    scale = np.abs(ref[mom]).max()
+    out[mom] = np.abs(gpu[mom] - ref[mom]).reshape(dim, -1).max(axis=1) / scale
+    return out
+
+
 def smooth_perturbed_state(x, seed, nspecies=0, amp=0.05, gamma=1.4):
     """A smooth, seeded, non-trivial ideal-gas state (conserved ``(F, ...)``)."""
     rng = np.random.default_rng(seed)

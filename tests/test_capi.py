@@ -90,4 +90,4 @@ def test_workspace_sizes(dim, order):
     m = _lib.MgMesh(dim=dim, order=order, np=np_, nfp=nfp, nfaces=dim + 1,
                     nelements=1)
     rpw = (np_ + 3) // 4
-    assert lib.mg_mesh_workspace_bytes(m) == 8 * 4 * rpw * (np_ * dim + (dim + 1) * nfp)
+    assert lib.mg_mesh_workspace_bytes(m) == 12 * 4 * rpw * (np_ * dim + (dim + 1) * nfp)
