@@ -39,6 +39,8 @@ def parse():
                     help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--state-steps", type=int, default=200,
                     help="RK4 steps of the mixed-vs-FP64 state check")
+    ap.add_argument("--reacting-n", type=int, default=32,
+                    help="cells per axis of the reacting-box line (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run a plain 256 MiB device copy (3x), the "
                          "reference kernel for HBM-traffic counter scaling")
@@ -96,6 +98,63 @@ def cpu_baseline(n, order):
             "sample": f"FP64 NumPy oracle, TGV {n}^3x6={K} tets p={order}, "
                       f"{reps} RHS evaluations in {el:.1f} s "
                       f"(NumPy; BLAS threads <= {cores})"}
+
+
+def measure_reacting(args, n):
+    """Plain-chemistry reacting box (uiuc_7sp, p = 2, n^3 x 6 tets, Euler +
+    species sources, FP64): one RHS = Newton temperature from the previous
+    stage's temperature + mixture Euler RHS + species source + their sum."""
+    import torch
+
+    from mirgecom_amd import _lib
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.gas_model import make_fluid_state
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+    from mirgecom_amd.workloads import reacting_box
+
+    wl = reacting_box(n=n, order=2)
+    gm, dcoll = wl.gas_model, wl.dcoll
+    cv, tseed = wl.cv, wl.temperature
+    K, npe = dcoll.mesh.nelements, dcoll.refel.nnodes
+    nfld = cv.q.shape[0]
+    import warnings
+
+    def rhs():
+        st = make_fluid_state(cv, gm, temperature_seed=tseed)
+        with warnings.catch_warnings():
+            warnings.simplefilter("ignore")
+            r = euler_operator(dcoll, st, gm, {},
+                               inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
+        return r + gm.eos.get_species_source_terms(cv, st.temperature)
+
+    for _ in range(args.warmup):
+        rhs()
+    torch.cuda.synchronize()
+    _lib.timing = {}
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        rhs()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    timing, _lib.timing = _lib.timing, None
+    kern = {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
+            for k, v in timing.items()}
+    nn = K * npe
+    nfp = dcoll.refel.nface_nodes
+    geo = K * (8 * (9 + 4 * 3 + 4) + 4 * 4 * nfp)
+    ab = {"mg_mixture_temperature": 8 * nn * (nfld + 2),
+          "mg_euler_rhs": 8 * nn * (2 * nfld + 1) + geo,
+          "mg_species_source": 8 * nn * (1 + 7 + 1 + 7)}
+    tab = {k: {"ms": round(kern[k], 4), "alg_bytes": ab.get(k),
+               "GB_s": round(ab[k] / (kern[k] * 1e-3) / 1e9, 1) if k in ab else None}
+           for k in kern}
+    dofs = nn * nfld
+    print(f"# reacting box {n}^3x6 p=2: {ms:.3f} ms/RHS, kernels {json.dumps(tab)}",
+          file=sys.stderr, flush=True)
+    return {"workload": f"3D reacting box uiuc_7sp, periodic, {n}^3x6={K} tets, "
+                        "p=2, Euler + species sources (plain chemistry, FP64)",
+            "dofs": dofs, "ms_per_rhs": ms, "value": dofs / (ms * 1e-3),
+            "unit": "DOF-RHS/s", "kernels": tab}
 
 
 def _tgv_transport():
@@ -228,6 +287,7 @@ def main():
         torch.cuda.synchronize()
         print("# copy calibration: 3 x copy of 268435456 bytes", file=sys.stderr,
               flush=True)
+    reacting = measure_reacting(args, args.reacting_n) if args.reacting_n else None
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n, order)
@@ -261,7 +321,8 @@ def main():
                              "speedup_vs_fp64": ms64 / ms32,
                              "rel_err_rhs": err_rhs,
                              "rel_err_state": err_state,
-                             "state_steps": nsteps, "dt": dt}},
+                             "state_steps": nsteps, "dt": dt},
+                   "reacting": reacting},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None},

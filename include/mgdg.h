@@ -165,6 +165,15 @@ mg_status mg_species_source(const mg_gas* gas, int32_t dim, int64_t n,
                             const double* q, const double* temperature,
                             double* src, void* stream);
 
+/* Node-local stable timestep into dt [K*np] from the element characteristic
+ * lengths lengthscale [K]: h/(|v|+c) (viscous == 0; replaces
+ * get_inviscid_timestep, mirgecom/inviscid.py:379-422) or
+ * h/(|v|+c+(mu/rho+max_k d_k)/h) (viscous != 0; replaces
+ * get_viscous_timestep, mirgecom/viscous.py:547-607). */
+mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                            const double* temperature, const double* lengthscale,
+                            int32_t viscous, double* dt, void* stream);
+
 /* Classical RK4 stage helpers over n values (mirgecom/integrators/
  * explicit_rk.py:31-38): out = y + a*k ; acc = acc + w*k (acc may be NULL). */
 mg_status mg_rk_axpy(int64_t n, const double* y, const double* k, double a,

@@ -71,6 +71,8 @@ SIGNATURES = {
                                       _P, _I32, _P, _P]),
     "mg_species_source": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,
                                  _P, _P]),
+    "mg_local_timestep": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                                 _P, _P, _P, _I32, _P, _P]),
     "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
     "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),
     "mg_mesh_workspace_bytes": (_I64, [ctypes.POINTER(MgMesh)]),

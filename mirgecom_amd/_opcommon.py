@@ -110,3 +110,27 @@ def warn_mixture_limiter(state, limiter_func):
              "one should pass the operator_states_quad argument with "
              "limited states or pass a limiter_func to this operator.",
              stacklevel=3)
+
+
+def local_timestep(dcoll, state, viscous):
+    """Node-local stable timestep ``(K, Np)`` from ``mg_local_timestep``."""
+    q = state.cv.q
+    if not q.is_cuda or q.dtype != torch.float64:
+        raise RuntimeError("the GPU timestep needs a float64 state on a HIP "
+                           "device")
+    dev = get_device_discretization(dcoll, q.device)
+    K, npe = dcoll.nelements, dcoll.refel.nnodes
+    if tuple(q.shape[1:]) != (K, npe):
+        raise ValueError(f"state shape {tuple(q.shape)} does not match the "
+                         f"discretization ({K}, {npe})")
+    q = q.contiguous()
+    gas = state.gas_model.device_params(state.cv.nspecies,
+                                        with_transport=viscous)
+    temp = None
+    if gas.eos_kind == 1:
+        temp = state.temperature.contiguous()
+    out = torch.empty((K, npe), dtype=q.dtype, device=q.device)
+    _lib.call("mg_local_timestep", dev.mesh, gas, _lib.ptr(q), _lib.ptr(temp),
+              _lib.ptr(dev.lengthscale), 1 if viscous else 0, _lib.ptr(out),
+              _lib.stream_handle())
+    return out

@@ -881,6 +881,32 @@ __global__ __launch_bounds__(kBlock) void species_source_kernel(
   for (int k = 0; k < MIX_NSP; ++k) src[k * n + g] = c_mech.wt[k] * w[k];
 }
 
+// Node-local stable timestep (inviscid.py:379-422, viscous.py:547-607):
+//   dt = h_e / (|v| + c)                                   inviscid
+//   dt = h_e / (|v| + c + (mu / rho + max_k d_k) / h_e)     viscous
+// with h_e the characteristic length of the node's element.
+template <int DIM, int NSP, bool MIX>
+__global__ __launch_bounds__(kBlock) void local_timestep_kernel(
+    int64_t n, int np, const double* __restrict__ q, const double* __restrict__ temp,
+    const double* __restrict__ hlen, GasDev gas, int viscous,
+    double* __restrict__ dt) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Node<DIM, NSP> s;
+  load_q<DIM, NSP>(s, q, n, g);
+  node_state<DIM, NSP, MIX, false>(s, gas, MIX ? temp[g] : 0.0);
+  const double h = hlen[g / np];
+  double rate = s.speed + s.c;
+  if (viscous) {
+    node_transport<DIM, NSP, MIX>(s, gas);
+    double dmax = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) dmax = k == 0 ? s.d[0] : fmax(dmax, s.d[k]);
+    rate += (s.mu * s.inv_rho + dmax) / h;
+  }
+  dt[g] = h / rate;
+}
+
 __global__ __launch_bounds__(kBlock) void rk_axpy_kernel(
     int64_t n, const double* y, const double* k, double a, double* out,
     double* acc, double w) {
@@ -1086,6 +1112,33 @@ mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
 }
 
 }  // namespace
+
+template <int DIM, int NSP, bool MIX>
+mg_status launch_timestep(const mg_mesh* m, const mgdg::GasDev& g, const double* q,
+                          const double* t, const double* h, int viscous, double* dt,
+                          hipStream_t st) {
+  const int64_t n = m->nelements * m->np;
+  const unsigned nblk = unsigned((n + mgdg::kBlock - 1) / mgdg::kBlock);
+  hipLaunchKernelGGL((mgdg::local_timestep_kernel<DIM, NSP, MIX>), dim3(nblk),
+                     dim3(mgdg::kBlock), 0, st, n, int(m->np), q, t, h, g, viscous, dt);
+  return check_launch("local_timestep_kernel");
+}
+
+template <int DIM>
+mg_status timestep_species(const mg_mesh* m, const mg_gas* gas, const double* q,
+                           const double* t, const double* h, int viscous, double* dt,
+                           hipStream_t st) {
+  const mgdg::GasDev g = to_dev(gas);
+  const bool mix = gas->eos_kind == 1;
+  if (gas->nspecies == 0 && !mix)
+    return launch_timestep<DIM, 0, false>(m, g, q, t, h, viscous, dt, st);
+  if (gas->nspecies == 7 && !mix)
+    return launch_timestep<DIM, 7, false>(m, g, q, t, h, viscous, dt, st);
+  if (gas->nspecies == 7 && mix)
+    return launch_timestep<DIM, 7, true>(m, g, q, t, h, viscous, dt, st);
+  return fail(MG_ERR_UNSUPPORTED,
+              "unsupported species count / EOS combination (nspecies 0 or 7)");
+}
 
 extern "C" {
 
@@ -1349,6 +1402,24 @@ mg_status mg_species_source(const mg_gas* gas, int32_t dim, int64_t n,
   else
     return fail(MG_ERR_ARG, "dim");
   return check_launch("species_source_kernel");
+}
+
+mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                            const double* temperature, const double* lengthscale,
+                            int32_t viscous, double* dt, void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, viscous != 0))) return s;
+  const bool mix = gas->eos_kind == 1;
+  if (mix && (s = ensure_mech())) return s;
+  if (!q || !lengthscale || !dt) return fail(MG_ERR_ARG, "q/lengthscale/dt NULL");
+  if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (mesh->dim == 2)
+    return timestep_species<2>(mesh, gas, q, temperature, lengthscale, viscous ? 1 : 0,
+                               dt, st);
+  return timestep_species<3>(mesh, gas, q, temperature, lengthscale, viscous ? 1 : 0,
+                             dt, st);
 }
 
 mg_status mg_rk_axpy(int64_t n, const double* y, const double* k, double a,

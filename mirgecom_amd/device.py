@@ -37,6 +37,7 @@ class DeviceDiscretization:
         self.lift = t64(re.lift)
         self.face_mask = t32(re.face_mask)
         self.nodes = t64(dcoll.host_nodes)
+        self.lengthscale = t64(dcoll.lengthscales)
         nfp = re.nface_nodes
         bf = dcoll.bnd_faces
         # flat volume node index of every boundary node

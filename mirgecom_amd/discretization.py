@@ -157,6 +157,7 @@ class Discretization:
     bnd_nodes: np.ndarray      # (dim, Nbf*Nfp)
     bnd_normals: np.ndarray    # (dim, Nbf*Nfp)
     _device_cache: dict = field(default_factory=dict, repr=False)
+    _host_cache: dict = field(default_factory=dict, repr=False)
 
     @property
     def dim(self):
@@ -173,6 +174,14 @@ class Discretization:
     @property
     def nnodes(self):
         return self.nelements * self.refel.nnodes
+
+    @property
+    def lengthscales(self):
+        """Element characteristic lengths ``(K,)`` (see
+        ``characteristic_lengthscales``)."""
+        if "h" not in self._host_cache:
+            self._host_cache["h"] = characteristic_lengthscales(self)
+        return self._host_cache["h"]
 
     @property
     def nbnd_nodes(self):
@@ -363,21 +372,20 @@ def create_discretization_collection(mesh: BoxMesh, order: int) -> Discretizatio
 
 
 def characteristic_lengthscales(dcoll: Discretization):
-    """Per-element length scale: the in-radius of the simplex
-    (d * volume / total face measure) times half the minimum node spacing of
-    the reference element.
-
-    The reference obtains this from grudge's ``characteristic_lengthscales``
-    (mirgecom/inviscid.py:415-421), whose exact formula is not available here;
-    this uses in-radius times the minimum reference node spacing of the
-    warp-and-blend set.  Shape ``(K,)``."""
+    """Per-element characteristic length ``h_e = min_{i != j} |r_i - r_j| * r_D``
+    (shape ``(K,)``): the smallest distance between two distinct nodes of the
+    biunit reference simplex times the in-radius ``r_D = d V / S`` of the
+    element (volume ``V``, total face measure ``S``), the estimate of
+    Hesthaven & Warburton, Nodal DG Methods, sec. 6.4, that the reference takes
+    from grudge's ``characteristic_lengthscales`` (mirgecom/inviscid.py:415-421,
+    mirgecom/viscous.py:594-607)."""
     refel = dcoll.refel
     dim = dcoll.dim
     vol = dcoll.jac * refel.ref_volume
+    # every face of the biunit simplex has reference measure 2
     face_meas = dcoll.face_jac.sum(axis=1) * 2.0
     inradius = dim * vol / face_meas
     x = refel.nodes
     d = np.linalg.norm(x[:, None, :] - x[None, :, :], axis=2)
-    d[d == 0] = np.inf
-    hmin_ref = d.min()
-    return inradius * hmin_ref / 2.0
+    d[np.eye(len(d), dtype=bool)] = np.inf
+    return inradius * d.min()

@@ -26,8 +26,10 @@ class GasModel:
         self.eos = eos
         self.transport = transport
 
-    def device_params(self, nspecies: int) -> "_lib.MgGas":
-        """The ``mg_gas`` struct of this model for a state with ``nspecies``."""
+    def device_params(self, nspecies: int,
+                      with_transport: bool = True) -> "_lib.MgGas":
+        """The ``mg_gas`` struct of this model for a state with ``nspecies``
+        (without the transport model if ``with_transport`` is false)."""
         eos = self.eos
         g = _lib.MgGas()
         g.nspecies = nspecies
@@ -41,7 +43,7 @@ class GasModel:
             g.gas_const = eos.gas_const()
         if nspecies > 8:
             raise NotImplementedError("at most 8 species")
-        if self.transport is not None:
+        if self.transport is not None and with_transport:
             prm = self.transport.device_params(nspecies)
             diff = prm.pop("diff")
             for k, v in prm.items():

@@ -10,6 +10,7 @@ from __future__ import annotations
 import torch
 
 from mirgecom_amd import _lib
+from mirgecom_amd.dof_desc import DD_VOLUME_ALL
 from mirgecom_amd.flux import num_flux_hll, num_flux_lfr
 from mirgecom_amd.fluid import ConservedVars
 
@@ -86,3 +87,18 @@ def flux_kind(numerical_flux_func):
         raise NotImplementedError(
             f"numerical flux {numerical_flux_func!r} has no device kernel; "
             "use inviscid_facial_flux_rusanov, _hll or _central") from None
+
+
+def get_inviscid_timestep(dcoll, state, dd=DD_VOLUME_ALL):
+    """mirgecom/inviscid.py:379-422: node-local ``h_e / (|v| + c)``, shape
+    ``(K, Np)``, with ``h_e`` from
+    ``mirgecom_amd.discretization.characteristic_lengthscales`` (one pass of
+    ``mg_local_timestep``)."""
+    from mirgecom_amd._opcommon import check_dd, local_timestep
+    check_dd(dd)
+    return local_timestep(dcoll, state, viscous=False)
+
+
+def get_inviscid_cfl(dcoll, state, dt):
+    """mirgecom/inviscid.py:425-448: ``dt / get_inviscid_timestep``."""
+    return dt / get_inviscid_timestep(dcoll, state=state)

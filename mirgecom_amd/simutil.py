@@ -1,0 +1,80 @@
+"""Driver utilities on the timestep path (call surface of
+mirgecom/simutil.py:203-306, 480-565): the simulation timestep, the step
+check, and the health / comparison checks drivers run on device states."""
+
+from __future__ import annotations
+
+import torch
+
+from mirgecom_amd.dof_desc import DD_VOLUME_ALL
+
+
+def check_step(step, interval):
+    """mirgecom/simutil.py:203-221: True when ``step`` is a multiple of a
+    positive ``interval``; ``interval == 0`` always, ``interval < 0`` never."""
+    if interval == 0:
+        return True
+    if interval < 0:
+        return False
+    return step % interval == 0
+
+
+def get_sim_timestep(dcoll, state, t, dt, cfl, t_final=0.0, constant_cfl=False,
+                     local_dt=False, fluid_dd=DD_VOLUME_ALL):
+    """mirgecom/simutil.py:224-306.
+
+    * constant dt (default): ``min(dt, t_final - t)`` (not below 0);
+    * ``constant_cfl``: ``min(cfl * min_nodes(dt_l), t_final - t)``, with
+      ``dt_l`` from ``get_viscous_timestep``;
+    * ``local_dt``: ``cfl`` times the per-element minimum of ``dt_l`` on every
+      node, shape ``(K, Np)`` (``t_final`` ignored)."""
+    from mirgecom_amd.viscous import get_viscous_timestep
+    if local_dt:
+        vdt = get_viscous_timestep(dcoll, state, dd=fluid_dd)
+        emin = vdt.min(dim=1, keepdim=True).values
+        return cfl * emin.expand_as(vdt).contiguous()
+    my_dt = dt
+    t_remaining = max(0, t_final - t)
+    if constant_cfl:
+        my_dt = float(cfl * get_viscous_timestep(dcoll, state, dd=fluid_dd).min())
+    return min(t_remaining, my_dt)
+
+
+def check_range_local(dcoll, dd, field, min_value, max_value):
+    """mirgecom/simutil.py:480-496: the field's minimum and/or maximum where
+    they fall outside ``[min_value, max_value]`` (empty list if inside)."""
+    lo, hi = float(field.min()), float(field.max())
+    failing = []
+    if lo < min_value:
+        failing.append(lo)
+    if hi > max_value:
+        failing.append(hi)
+    return failing
+
+
+def check_naninf_local(dcoll, dd, field):
+    """mirgecom/simutil.py:499-505: True if the field holds a NaN or Inf."""
+    return not bool(torch.isfinite(field.sum()))
+
+
+def componentwise_norms(dcoll, fields, order=float("inf"), *, dd=DD_VOLUME_ALL):
+    """mirgecom/simutil.py:534-547 for ``order = inf`` (max norm per
+    component of a ``ConservedVars`` or a tensor with the components first)."""
+    q = getattr(fields, "q", fields)
+    if order != float("inf"):
+        raise NotImplementedError("only the max norm is supported")
+    if q.dim() == 2:
+        return q.abs().max()
+    return q.reshape(q.shape[0], -1).abs().max(dim=1).values
+
+
+def max_component_norm(dcoll, fields, order=float("inf"), *, dd=DD_VOLUME_ALL):
+    """mirgecom/simutil.py:550-564: the largest component norm."""
+    return float(componentwise_norms(dcoll, fields, order, dd=dd).max())
+
+
+def compare_fluid_solutions(dcoll, red_state, blue_state, *, dd=DD_VOLUME_ALL):
+    """mirgecom/simutil.py:508-531: max norm of ``red - blue`` per conserved
+    component, as a list."""
+    resid = red_state.q - blue_state.q
+    return componentwise_norms(dcoll, resid, dd=dd).tolist()

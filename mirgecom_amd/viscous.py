@@ -7,6 +7,7 @@ from __future__ import annotations
 
 import torch
 
+from mirgecom_amd.dof_desc import DD_VOLUME_ALL
 from mirgecom_amd.fluid import (
     ConservedVars, species_mass_fraction_gradient, velocity_gradient)
 from mirgecom_amd.flux import num_flux_central
@@ -58,3 +59,31 @@ def viscous_facial_flux_central(dcoll, state_pair, grad_cv_pair, grad_t_pair,
     f_ext = viscous_flux(state_pair.ext, grad_cv_pair.ext, grad_t_pair.ext)
     return ConservedVars(state_pair.int.dim, torch.einsum(
         "fi...,i...->f...", num_flux_central(f_int.q, f_ext.q), normal))
+
+
+def get_local_max_species_diffusivity(actx, d_alpha):
+    """mirgecom/viscous.py:660-680: node-wise maximum over species of the
+    diffusivities (0 without species; ``actx`` is unused)."""
+    if len(d_alpha) == 0:
+        return 0
+    if not isinstance(d_alpha[0], torch.Tensor):
+        return max(d_alpha)
+    out = d_alpha[0]
+    for d in d_alpha[1:]:
+        out = torch.maximum(out, d)
+    return out
+
+
+def get_viscous_timestep(dcoll, state, *, dd=DD_VOLUME_ALL):
+    """mirgecom/viscous.py:547-607: node-local
+    ``h_e / (|v| + c + (mu / rho + max_k d_k) / h_e)`` for a viscous state,
+    ``h_e / (|v| + c)`` otherwise; shape ``(K, Np)``, one pass of
+    ``mg_local_timestep``."""
+    from mirgecom_amd._opcommon import check_dd, local_timestep
+    check_dd(dd)
+    return local_timestep(dcoll, state, viscous=state.is_viscous)
+
+
+def get_viscous_cfl(dcoll, dt, state, *, dd=DD_VOLUME_ALL):
+    """mirgecom/viscous.py:610-636: ``dt / get_viscous_timestep``."""
+    return dt / get_viscous_timestep(dcoll, state=state, dd=dd)

@@ -277,3 +277,50 @@ def rk4_step(state, t, dt, rhs):
     k3 = rhs(t + dt / 2, state + dt / 2 * k2)
     k4 = rhs(t + dt, state + dt * k3)
     return state + dt / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+
+
+# {{{ timestep estimates
+
+def characteristic_lengthscales(disc):
+    """Element length ``h_e = min_{i != j} |r_i - r_j| * r_D`` from the
+    vertices: the smallest distance between distinct reference nodes (biunit
+    simplex) times the in-radius ``r_D = d V / S`` (grudge
+    ``characteristic_lengthscales`` as used by mirgecom/inviscid.py:415-421;
+    grudge is absent here, so the formula follows Hesthaven & Warburton,
+    Nodal DG Methods, sec. 6.4 -- parity unpinned beyond the closed forms)."""
+    from itertools import combinations
+    from math import factorial
+    v = disc.v                                   # (K, d+1, d)
+    d = disc.dim
+    vol = np.abs(np.linalg.det(v[:, 1:] - v[:, :1])) / factorial(d)
+    surf = np.zeros(disc.K)
+    for face in combinations(range(d + 1), d):
+        e = v[:, list(face[1:])] - v[:, [face[0]]]        # (K, d-1, d)
+        gram = np.einsum("kai,kbi->kab", e, e)
+        surf += np.sqrt(np.linalg.det(gram)) / factorial(d - 1)
+    r = disc.el.nodes
+    dmin = min(np.linalg.norm(r[i] - r[j])
+               for i in range(len(r)) for j in range(len(r)) if i != j)
+    return dmin * d * vol / surf
+
+
+def get_inviscid_timestep(disc, state):
+    """mirgecom/inviscid.py:379-422: ``h / (|v| + c)`` per node ``(K, Np)``."""
+    h = characteristic_lengthscales(disc)[:, None]
+    return h / state.wavespeed
+
+
+def get_viscous_timestep(disc, state):
+    """mirgecom/viscous.py:547-607: ``h / (|v| + c + (nu + max_k d_k) / h)``
+    (``nu = mu / rho``; no viscous term without transport)."""
+    h = characteristic_lengthscales(disc)[:, None]
+    if state.tv is None:
+        return h / state.wavespeed
+    nu = state.tv.viscosity / state.mass
+    d = state.tv.species_diffusivity
+    dmax = 0.0
+    if d is not None and len(d) > 0:
+        dmax = np.max(np.asarray(d), axis=0)
+    return h / (state.wavespeed + (nu + dmax) / h)
+
+# }}}
