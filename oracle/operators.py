@@ -16,7 +16,8 @@ Restated from the reference in NumPy FP64:
   ``div_operator(F_V - F_I, f_V* - f_I*)``)
 * ``PrescribedFluidBoundary`` / ``DummyBoundary`` -- mirgecom/boundary.py:
   796-1049 (gradient flux takes the exterior value, viscous flux is central
-  with identical gradients)
+  with identical gradients); the Mengaldo family lives in
+  ``oracle.boundaries`` (``kind == "mengaldo"``)
 
 Parity: pinned by the reference's invariant and convergence tests restated in
 tests/ (uniform state => RHS = 0, test/test_euler.py:80-232 and
@@ -118,6 +119,7 @@ class OperatorStates:
 
     def __init__(self, disc, box, state, gas_model, boundaries, time):
         self.disc = disc
+        self.gas_model = gas_model
         self.vol = state
         dim = state.dim
         cv_m = _flat(disc.trace(state.cv))
@@ -140,6 +142,8 @@ class OperatorStates:
             ext = None
             if bdry.kind == "prescribed":
                 ext = bdry.state_func(self.x[:, idx], sm, time)
+            elif bdry.kind == "mengaldo":
+                ext = bdry.state_plus(sm, self.normal[:, idx], gas_model)
             self.groups.append((bdry, idx, sm, ext))
         self.nfn = cv_m.shape[-1]
 
@@ -209,8 +213,12 @@ def grad_cv_operator(ops: OperatorStates):
     bvals = []
     for bdry, idx, sm, ext in ops.groups:
         # Prescribed: exterior value (boundary.py:70-72, 895-908);
-        # Dummy: interior value (boundary.py:1031-1035)
-        bvals.append(sm.cv if bdry.kind == "dummy" else ext.cv)
+        # Dummy: interior value (boundary.py:1031-1035);
+        # Mengaldo: CV_bc (boundary.py:713-750)
+        if bdry.kind == "mengaldo":
+            bvals.append(bdry.state_bc(sm, ops.normal[:, idx], ops.gas_model).cv)
+        else:
+            bvals.append(sm.cv if bdry.kind == "dummy" else ext.cv)
     flux = _gradient_face_flux(ops, ops.minus.cv, ops.plus_int.cv, bvals)
     return grad_operator(ops.disc, ops.vol.cv, ops.unflat(flux))
 
@@ -219,8 +227,11 @@ def grad_t_operator(ops: OperatorStates):
     """mirgecom/navierstokes.py:238-347 -> ``(dim, K, Np)``."""
     bvals = []
     for bdry, idx, sm, ext in ops.groups:
-        bvals.append(sm.temperature if bdry.kind == "dummy"
-                     else ext.temperature)
+        if bdry.kind == "mengaldo":         # T_bc (boundary.py:753-793)
+            bvals.append(bdry.temperature_bc(sm))
+        else:
+            bvals.append(sm.temperature if bdry.kind == "dummy"
+                         else ext.temperature)
     flux = _gradient_face_flux(ops, ops.minus.temperature,
                                ops.plus_int.temperature, bvals)
     return grad_operator(ops.disc, ops.vol.temperature, ops.unflat(flux))
@@ -244,6 +255,9 @@ def viscous_flux_on_element_boundary(ops: OperatorStates, grad_cv, grad_t):
         if bdry.kind == "dummy":
             out[:, idx] = ph._dot_normal(
                 ph.viscous_flux(sm, gcv_m[..., idx], gt_m[..., idx]), n)
+        elif bdry.kind == "mengaldo":
+            out[:, idx] = bdry.viscous_flux_n(sm, gcv_m[..., idx],
+                                              gt_m[..., idx], n, ops.gas_model)
         else:
             out[:, idx] = ph.viscous_facial_flux_central(
                 sm, ext, gcv_m[..., idx], gcv_m[..., idx], gt_m[..., idx],
