@@ -74,6 +74,27 @@ typedef struct mg_gas {
   double species_diffusivity[8];
 } mg_gas;
 
+/* Boundary data of Mengaldo-type boundaries (AdiabaticSlipBoundary,
+ * IsothermalWallBoundary, AdiabaticNoslipWallBoundary, FarfieldBoundary,
+ * PressureOutflowBoundary, ...; mirgecom/boundary.py:391-1857).  On boundary
+ * nodes whose flags hold MG_BC_MENGALDO the gradients take CV_bc and T_bc as
+ * boundary values (boundary.py:713-793) and the viscous boundary flux is
+ * F_V(state_bc, grad_cv_bc, grad_t_bc) . n (boundary.py:630-710), with the
+ * interior gradients conditioned by the other bits (slip: boundary.py:
+ * 159-173; impermeable: 193-213; adiabatic: 186-190).  The inviscid flux
+ * keeps using q_bnd/t_bnd (state_plus).  A NULL mg_wall (or NULL flags)
+ * means no such nodes. */
+#define MG_BC_MENGALDO 1
+#define MG_BC_SLIP 2
+#define MG_BC_IMPERMEABLE 4
+#define MG_BC_ADIABATIC 8
+typedef struct mg_wall {
+  const double* q_bc;        /* [F][nbnd_nodes] CV_bc                       */
+  const double* t_bc;        /* [nbnd_nodes] T_bc, the gradient value of T   */
+  const double* t_state_bc;  /* [nbnd_nodes] temperature of state_bc (mixtures) */
+  const int32_t* flags;      /* [nbnd_nodes] MG_BC_* bits                    */
+} mg_wall;
+
 /* Numerical inviscid flux kinds: inviscid_facial_flux_rusanov,
  * inviscid_facial_flux_hll, inviscid_facial_flux_central
  * (mirgecom/inviscid.py:105-287). */
@@ -91,20 +112,20 @@ mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas,
                        int32_t flux_kind, double* rhs, void* stream);
 
 /* DG gradients of the conserved fields and of T with the central flux on
- * interior faces and the exterior value on boundary faces
- * (replaces grad_cv_operator + grad_t_operator,
+ * interior faces and the exterior value (CV_bc / T_bc on Mengaldo nodes) on
+ * boundary faces (replaces grad_cv_operator + grad_t_operator,
  * mirgecom/navierstokes.py:129-347). */
 mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas,
                          const double* q, const double* temperature,
                          const double* q_bnd, const double* t_bnd,
-                         double* grad, void* stream);
+                         const mg_wall* wall, double* grad, void* stream);
 
 /* Navier-Stokes RHS div(F_V - F_I) from q and its gradients (replaces the
  * divergence part of ns_operator, mirgecom/navierstokes.py:533-575). */
 mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas,
                     const double* q, const double* temperature,
                     const double* grad, const double* q_bnd,
-                    const double* t_bnd, int32_t flux_kind,
+                    const double* t_bnd, const mg_wall* wall, int32_t flux_kind,
                     int32_t inviscid_on, double* rhs, void* stream);
 
 /* Fused first half of the Navier-Stokes RHS: the DG gradients of the
@@ -113,13 +134,15 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas,
  * [F*dim][N] (F_I omitted if inviscid_on == 0).  The gradients themselves
  * are written only if grad != NULL.  For boundary face nodes fnet_bnd
  * [F][nbnd_nodes] receives (F_I - F_V)(q_ext, grad q-) . n, the exterior side
- * of the boundary fluxes (mirgecom/boundary.py:944-970).  Replaces
+ * of the boundary fluxes (mirgecom/boundary.py:944-970); on Mengaldo nodes
+ * its viscous part is 2 F_V(bc).n - F_V(q-, grad q-).n, so that the central
+ * average is F_V(bc).n.  Replaces
  * grad_cv_operator, grad_t_operator and the volume viscous_flux of
  * ns_operator (mirgecom/navierstokes.py:494-541). */
 mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
                      const double* temperature, const double* q_bnd,
-                     const double* t_bnd, int32_t inviscid_on, double* fnet,
-                     double* fnet_bnd, double* grad, void* stream);
+                     const double* t_bnd, const mg_wall* wall, int32_t inviscid_on,
+                     double* fnet, double* fnet_bnd, double* grad, void* stream);
 
 /* Second half: divergence of the nodal net fluxes with the numerical face
  * flux (Rusanov/central applied to the net normal fluxes, HLL to the
@@ -143,8 +166,8 @@ mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
 mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas,
                            const double* q, const double* temperature,
                            const double* q_bnd, const double* t_bnd,
-                           int32_t inviscid_on, float* fnet, double* fnet_bnd,
-                           void* stream);
+                           const mg_wall* wall, int32_t inviscid_on, float* fnet,
+                           double* fnet_bnd, void* stream);
 mg_status mg_ns_div_mixed(const mg_mesh* mesh, const mg_gas* gas,
                           const double* q, const double* temperature,
                           const float* fnet, const double* q_bnd,

@@ -52,19 +52,28 @@ _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
 _I64 = ctypes.c_int64
 _D = ctypes.c_double
+
+
+class MgWall(ctypes.Structure):
+    """``mg_wall`` (include/mgdg.h): Mengaldo boundary data."""
+    _fields_ = [("q_bc", ctypes.c_void_p), ("t_bc", ctypes.c_void_p),
+                ("t_state_bc", ctypes.c_void_p), ("flags", ctypes.c_void_p)]
+
+
+_W = ctypes.POINTER(MgWall)
 SIGNATURES = {
     "mg_euler_rhs": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                             _P, _P, _P, _P, _I32, _P, _P]),
     "mg_ns_gradient": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
-                              _P, _P, _P, _P, _P, _P]),
+                              _P, _P, _P, _P, _W, _P, _P]),
     "mg_ns_rhs": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
-                         _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
+                         _P, _P, _P, _P, _P, _W, _I32, _I32, _P, _P]),
     "mg_ns_flux": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
-                          _P, _P, _P, _P, _I32, _P, _P, _P, _P]),
+                          _P, _P, _P, _P, _W, _I32, _P, _P, _P, _P]),
     "mg_ns_div": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                          _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "mg_ns_flux_mixed": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
-                                _P, _P, _P, _P, _I32, _P, _P, _P]),
+                                _P, _P, _P, _P, _W, _I32, _P, _P, _P]),
     "mg_ns_div_mixed": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "mg_mixture_temperature": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P,

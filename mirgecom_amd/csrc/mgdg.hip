@@ -59,6 +59,11 @@ struct KArgs {
   float* fnet32_out;   // FP32 nodal net flux output (mixed precision)
   double* grad_out;    // optional gradient output of the fused flux kernel
   double* fnb_out;     // exterior net normal flux output of the flux kernel
+  // Mengaldo boundary data per boundary node (NULL wflag: none)
+  const double* wq;     // CV_bc [F][nbnd]
+  const double* wt;     // T_bc [nbnd]
+  const double* wts;    // temperature of state_bc [nbnd] (mixtures)
+  const int32_t* wflag; // MG_BC_* bits [nbnd]
   int64_t K;
   int64_t nbnd;
   int64_t nnodes;      // K * NP
@@ -198,6 +203,81 @@ __device__ __forceinline__ double reference_pressure(const KArgs& a) {
   load_q<DIM, NSP>(s, a.q, a.nnodes, 0);
   node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[0] : 0.0);
   return s.p;
+}
+
+// Mengaldo boundary nodes (boundary.py:391-793): gradient values CV_bc and
+// T_bc, and the viscous flux F_V(state_bc, grad_cv_bc, grad_t_bc) . n with
+// the boundary conditions on the gradients applied to the interior ones.
+constexpr int kBcMengaldo = 1, kBcSlip = 2, kBcImpermeable = 4, kBcAdiabatic = 8;
+
+__device__ __forceinline__ int wall_flags(const KArgs& a, int64_t b) {
+  return a.wflag ? a.wflag[b] : 0;
+}
+
+// Effective exterior viscous normal flux 2 F_V(bc).n - F_V(q-, grad-).n: the
+// face flux averages both sides, which then gives F_V(bc).n.  gq/gt hold the
+// interior gradients on entry and the boundary-condition gradients on exit.
+template <int DIM, int NSP, bool MIX>
+__device__ void wall_viscous_flux_n(const KArgs& a, int64_t b, int64_t gm, int fl,
+                                    const double* n, double (*gq)[DIM], double* gt,
+                                    double* fv) {
+  constexpr int NFLD = 2 + DIM + NSP;
+  Node<DIM, NSP> s;
+  load_q<DIM, NSP>(s, a.q, a.nnodes, gm);
+  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[gm] : 0.0);
+  double fm[NFLD];
+  viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fm);
+  if (fl & kBcSlip) {
+    // grad v_bc = G - (G n) n^T + n (n^T G n) n^T (boundary.py:159-173 in
+    // frame-free form); grad(rho v)_bc = rho_bc grad v_bc + v_bc grad rho-
+    const double rho_bc = a.wq[b];
+    double gv[DIM][DIM], gn[DIM], ngn = 0.0;
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+      gn[i] = 0.0;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        gv[i][j] = (gq[2 + i][j] - s.v[i] * gq[0][j]) * s.inv_rho;
+        gn[i] += gv[i][j] * n[j];
+      }
+      ngn += n[i] * gn[i];
+    }
+#pragma unroll
+    for (int i = 0; i < DIM; ++i) {
+      const double vbc = a.wq[(2 + i) * a.nbnd + b] / rho_bc;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j)
+        gq[2 + i][j] = rho_bc * (gv[i][j] - gn[i] * n[j] + n[i] * ngn * n[j]) +
+                       vbc * gq[0][j];
+    }
+  }
+  if (fl & kBcImpermeable) {
+    // grad Y_bc = grad Y - (grad Y . n) n (boundary.py:196-213)
+#pragma unroll
+    for (int k = 0; k < NSP; ++k) {
+      double gy[DIM], gyn = 0.0;
+#pragma unroll
+      for (int j = 0; j < DIM; ++j) {
+        gy[j] = (gq[2 + DIM + k][j] - s.y[k] * gq[0][j]) * s.inv_rho;
+        gyn += gy[j] * n[j];
+      }
+#pragma unroll
+      for (int j = 0; j < DIM; ++j)
+        gq[2 + DIM + k][j] = s.q[0] * (gy[j] - gyn * n[j]) + s.y[k] * gq[0][j];
+    }
+  }
+  if (fl & kBcAdiabatic) {
+    double tn = 0.0;
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) tn += gt[j] * n[j];
+#pragma unroll
+    for (int j = 0; j < DIM; ++j) gt[j] -= tn * n[j];
+  }
+  load_q<DIM, NSP>(s, a.wq, a.nbnd, b);
+  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.wts[b] : 0.0);
+  viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
+#pragma unroll
+  for (int k = 0; k < NFLD; ++k) fv[k] = 2.0 * fv[k] - fm[k];
 }
 
 // }}}
@@ -422,7 +502,11 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
         if constexpr (VISC) {
           double gq[NFLD][DIM], gt[DIM], fv[NFLD];
           load_grad<DIM, NSP>(gq, gt, a.grad, N, bnd ? e * NP + im : int64_t(nb));
-          viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
+          const int wfl = bnd ? wall_flags(a, b) : 0;
+          if (wfl & kBcMengaldo)
+            wall_viscous_flux_n<DIM, NSP, MIX>(a, b, e * NP + im, wfl, n, gq, gt, fv);
+          else
+            viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
 #pragma unroll
           for (int k = 0; k < NFLD; ++k) fnp[k] -= fv[k];
         }
@@ -668,9 +752,16 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
         tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.temp[nb] : 0.0);
       } else {
         const int64_t b = -int64_t(nb) - 1;
+        if (wall_flags(a, b) & kBcMengaldo) {
+          // CV_bc n and T_bc n (boundary.py:713-793)
 #pragma unroll
-        for (int k = 0; k < NFLD; ++k) qp[k] = a.qb[k * a.nbnd + b];
-        tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.tb[b] : 0.0);
+          for (int k = 0; k < NFLD; ++k) qp[k] = a.wq[k * a.nbnd + b];
+          tp = a.wt[b];
+        } else {
+#pragma unroll
+          for (int k = 0; k < NFLD; ++k) qp[k] = a.qb[k * a.nbnd + b];
+          tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.tb[b] : 0.0);
+        }
       }
 #pragma unroll
       for (int k = 0; k <= NFLD; ++k) {
@@ -778,8 +869,9 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
           }
         }
     }
-    // phase 6: exterior net normal flux at boundary face nodes, with the
-    // interior gradients (boundary.py:944-970)
+    // phase 6: exterior net normal flux at boundary face nodes: central
+    // viscous flux with the interior gradients (boundary.py:944-970), or the
+    // Mengaldo viscous flux F_V(bc) . n (boundary.py:630-710)
     if (a.nbnd > 0) {
       for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
         const int le = t / NFN;
@@ -806,13 +898,17 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
             gq[k][d] = double(stage[(k * DIM + d) * (EPB * NP) + ti]);
 #pragma unroll
         for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * (EPB * NP) + ti]);
-        viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
         if (a.inviscid_on) {
           inviscid_flux_n<DIM, NSP>(s, n, fi);
         } else {
 #pragma unroll
           for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
         }
+        const int wfl = wall_flags(a, b);
+        if (wfl & kBcMengaldo)
+          wall_viscous_flux_n<DIM, NSP, MIX>(a, b, e * NP + im, wfl, n, gq, gt, fv);
+        else
+          viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
 #pragma unroll
         for (int k = 0; k < NFLD; ++k)
           a.fnb_out[k * a.nbnd + b] =
@@ -1113,6 +1209,17 @@ mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
 
 }  // namespace
 
+mg_status set_wall(mgdg::KArgs& a, const mg_mesh* m, const mg_wall* w, bool mix) {
+  if (!w || !w->flags || m->nbnd_nodes == 0) return MG_OK;
+  if (!w->q_bc || !w->t_bc || (mix && !w->t_state_bc))
+    return fail(MG_ERR_ARG, "mg_wall: q_bc / t_bc (and t_state_bc for mixtures) needed");
+  a.wq = w->q_bc;
+  a.wt = w->t_bc;
+  a.wts = w->t_state_bc;
+  a.wflag = w->flags;
+  return MG_OK;
+}
+
 template <int DIM, int NSP, bool MIX>
 mg_status launch_timestep(const mg_mesh* m, const mgdg::GasDev& g, const double* q,
                           const double* t, const double* h, int viscous, double* dt,
@@ -1144,7 +1251,7 @@ extern "C" {
 
 const char* mg_last_error(void) { return g_err.c_str(); }
 
-int32_t mg_abi_version(void) { return 1; }
+int32_t mg_abi_version(void) { return 2; }
 
 /* Upload the uiuc_7sp mechanism constants (host array of 7*7*2 + 7 + 7 + 7
  * + 2 + 9 doubles, packed as in mgdg::MechDev). */
@@ -1210,7 +1317,8 @@ mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
 
 mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas, const double* q,
                          const double* temperature, const double* q_bnd,
-                         const double* t_bnd, double* grad, void* stream) {
+                         const double* t_bnd, const mg_wall* wall, double* grad,
+                         void* stream) {
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, false))) return s;
@@ -1224,14 +1332,15 @@ mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas, const double* q
   a.temp = temperature;
   a.qb = q_bnd;
   a.tb = t_bnd;
+  if ((s = set_wall(a, mesh, wall, mix))) return s;
   a.out = grad;
   return dispatch(mesh, a, gas->nspecies, mix, 3, static_cast<hipStream_t>(stream));
 }
 
 mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
                     const double* temperature, const double* grad,
-                    const double* q_bnd, const double* t_bnd, int32_t flux_kind,
-                    int32_t inviscid_on, double* rhs, void* stream) {
+                    const double* q_bnd, const double* t_bnd, const mg_wall* wall,
+                    int32_t flux_kind, int32_t inviscid_on, double* rhs, void* stream) {
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
@@ -1248,6 +1357,7 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.grad = grad;
   a.qb = q_bnd;
   a.tb = t_bnd;
+  if ((s = set_wall(a, mesh, wall, mix))) return s;
   a.out = rhs;
   a.flux_kind = flux_kind;
   a.inviscid_on = inviscid_on ? 1 : 0;
@@ -1256,8 +1366,8 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
 
 mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
                      const double* temperature, const double* q_bnd,
-                     const double* t_bnd, int32_t inviscid_on, double* fnet,
-                     double* fnet_bnd, double* grad, void* stream) {
+                     const double* t_bnd, const mg_wall* wall, int32_t inviscid_on,
+                     double* fnet, double* fnet_bnd, double* grad, void* stream) {
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
@@ -1272,6 +1382,7 @@ mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.temp = temperature;
   a.qb = q_bnd;
   a.tb = t_bnd;
+  if ((s = set_wall(a, mesh, wall, mix))) return s;
   a.out = fnet;
   a.grad_out = grad;
   a.fnb_out = fnet_bnd;
@@ -1309,8 +1420,9 @@ mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
 
 mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas, const double* q,
                            const double* temperature, const double* q_bnd,
-                           const double* t_bnd, int32_t inviscid_on, float* fnet,
-                           double* fnet_bnd, void* stream) {
+                           const double* t_bnd, const mg_wall* wall,
+                           int32_t inviscid_on, float* fnet, double* fnet_bnd,
+                           void* stream) {
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
@@ -1325,6 +1437,7 @@ mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas, const double*
   a.temp = temperature;
   a.qb = q_bnd;
   a.tb = t_bnd;
+  if ((s = set_wall(a, mesh, wall, mix))) return s;
   a.fnet32_out = fnet;
   a.fnb_out = fnet_bnd;
   a.inviscid_on = inviscid_on ? 1 : 0;

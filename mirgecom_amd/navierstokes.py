@@ -32,7 +32,7 @@ def _gradients(dcoll, gas_model, boundaries, state, time):
                        dtype=prep.q.dtype, device=prep.q.device)
     _lib.call("mg_ns_gradient", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
               _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb),
-              _lib.ptr(grad), prep.stream)
+              prep.wall, _lib.ptr(grad), prep.stream)
     return prep, grad
 
 
@@ -112,7 +112,7 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
                               device=prep.q.device)
         _lib.call("mg_ns_flux_mixed", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
                   _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb),
-                  inv, _lib.ptr(fnet), _lib.ptr(fnb), prep.stream)
+                  prep.wall, inv, _lib.ptr(fnet), _lib.ptr(fnb), prep.stream)
         rhs = torch.empty_like(prep.q)
         _lib.call("mg_ns_div_mixed", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
                   _lib.ptr(prep.temp), _lib.ptr(fnet), _lib.ptr(prep.qb),
@@ -135,7 +135,7 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
                                device=prep.q.device)
         _lib.call("mg_ns_flux", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
                   _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb),
-                  inv, _lib.ptr(fnet), _lib.ptr(fnb), _lib.ptr(grad),
+                  prep.wall, inv, _lib.ptr(fnet), _lib.ptr(fnb), _lib.ptr(grad),
                   prep.stream)
         rhs = torch.empty_like(prep.q)
         _lib.call("mg_ns_div", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
@@ -155,7 +155,8 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
         rhs = torch.empty_like(prep.q)
         _lib.call("mg_ns_rhs", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
                   _lib.ptr(prep.temp), _lib.ptr(grad), _lib.ptr(prep.qb),
-                  _lib.ptr(prep.tb), kind, inv, _lib.ptr(rhs), prep.stream)
+                  _lib.ptr(prep.tb), prep.wall, kind, inv, _lib.ptr(rhs),
+                  prep.stream)
     out = ConservedVars(state.dim, rhs)
     if return_gradients:
         return out, ConservedVars(state.dim, grad[:-1]), grad[-1]

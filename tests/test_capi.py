@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
     for name in _declared():
         assert hasattr(lib, name), name
         assert name in _lib.SIGNATURES, name
-    assert lib.mg_abi_version() == 1
+    assert lib.mg_abi_version() == 2
 
 
 def test_null_arguments_rejected():
@@ -52,7 +52,7 @@ def test_mesh_validation():
                     nbnd_nodes=0)
     # matrices missing
     rc = lib.mg_ns_flux(m, _lib.MgGas(transport_kind=1), None, None, None,
-                        None, 1, None, None, None, None)
+                        None, None, 1, None, None, None, None)
     assert rc == 1
     m.np = 21
     assert lib.mg_euler_rhs(m, _lib.MgGas(), None, None, None, None, 0, None,
@@ -71,6 +71,21 @@ def test_mechanism_size_checked():
     assert b"size" in lib.mg_last_error()
     from mirgecom_amd.mechanisms import packed_mechanism
     assert packed_mechanism().size == 130
+
+
+def test_wall_validation():
+    """An mg_wall with flags but without its CV_bc / T_bc arrays is rejected
+    before any launch (checked after the mesh, so give a valid-looking one
+    with boundary nodes and fake non-NULL device arrays)."""
+    lib = _lib.load()
+    m = _lib.MgMesh(dim=2, order=1, np=3, nfp=2, nfaces=3, nelements=2,
+                    nbnd_nodes=4, rx=1, normals=1, fscale=1, vmap_p=1,
+                    weak_diff=1, lift=1, face_mask=1, workspace=1)
+    w = _lib.MgWall(q_bc=None, t_bc=None, t_state_bc=None, flags=1)
+    rc = lib.mg_ns_gradient(m, _lib.MgGas(gamma=1.4, gas_const=1.0), 1, None,
+                            1, None, w, 1, None)
+    assert rc == 1
+    assert b"mg_wall" in lib.mg_last_error()
 
 
 def test_gas_validation():
