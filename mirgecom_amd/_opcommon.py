@@ -118,9 +118,14 @@ class Prepared:
                         self._wt[idx] = ext.temperature
                         if self.mix:
                             self._wts[idx] = ext.temperature
-            if int(covered.min()) != 1 or int(covered.max()) != 1:
-                raise ValueError("every boundary face needs exactly one "
-                                 "boundary condition")
+            # the coverage depends only on the tags: check it once per tag
+            # set (the check synchronises, which a captured call must not)
+            key = tuple(sorted(b.tag for b in boundaries))
+            if key not in dev.checked_tag_sets:
+                if int(covered.min()) != 1 or int(covered.max()) != 1:
+                    raise ValueError("every boundary face needs exactly one "
+                                     "boundary condition")
+                dev.checked_tag_sets.add(key)
             if walls:
                 self.wall = _lib.MgWall(
                     q_bc=self._wq.data_ptr(), t_bc=self._wt.data_ptr(),

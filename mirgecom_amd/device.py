@@ -50,6 +50,7 @@ class DeviceDiscretization:
         self.bnd_nodes = t64(dcoll.bnd_nodes)
         self.bnd_normals = t64(dcoll.bnd_normals)
         self._tag_index = {}
+        self.checked_tag_sets = set()
         self.mesh = _lib.MgMesh(
             dim=dcoll.dim, order=re.order, np=re.nnodes, nfp=nfp,
             nfaces=re.nfaces, pad0=0, nelements=dcoll.nelements,
