@@ -39,6 +39,8 @@ def parse():
                     help="cells per axis of the CPU-baseline sample")
     ap.add_argument("--state-steps", type=int, default=200,
                     help="RK4 steps of the mixed-vs-FP64 state check")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the HIP-graph replay timing")
     ap.add_argument("--reacting-n", type=int, default=32,
                     help="cells per axis of the reacting-box line (0: skip)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -264,8 +266,10 @@ def main():
         del g
         return ms
 
-    ms64_graph = measure_graph("float64")
-    ms32_graph = measure_graph("mixed")
+    ms64_graph = ms32_graph = float("inf")
+    if not args.no_graph:
+        ms64_graph = measure_graph("float64")
+        ms32_graph = measure_graph("mixed")
     ms64 = min(ms64_eager, ms64_graph)
     ms32 = min(ms32_eager, ms32_graph)
 
@@ -341,10 +345,12 @@ def main():
                    "timing": "best of eager calls and HIP-graph replay of "
                              "the same RHS (both listed per precision)",
                    "fp64": {"value": dofs / (ms64 * 1e-3), "ms_per_step": ms64,
-                            "eager_ms": ms64_eager, "graph_ms": ms64_graph,
+                            "eager_ms": ms64_eager,
+                            "graph_ms": None if args.no_graph else ms64_graph,
                             "kernels": t64},
                    "mixed": {"value": dofs / (ms32 * 1e-3), "ms_per_step": ms32,
-                             "eager_ms": ms32_eager, "graph_ms": ms32_graph,
+                             "eager_ms": ms32_eager,
+                             "graph_ms": None if args.no_graph else ms32_graph,
                              "kernels": t32,
                              "speedup_vs_fp64": ms64 / ms32,
                              "rel_err_rhs": err_rhs,
