@@ -93,6 +93,9 @@ typedef struct mg_wall {
   const double* t_bc;        /* [nbnd_nodes] T_bc, the gradient value of T   */
   const double* t_state_bc;  /* [nbnd_nodes] temperature of state_bc (mixtures) */
   const int32_t* flags;      /* [nbnd_nodes] MG_BC_* bits                    */
+  const int32_t* vol_index;  /* [nbnd_nodes] volume node behind each boundary node */
+  const double* normals;     /* [dim][nbnd_nodes] outward unit normals        */
+  double* work;              /* [(F+1)*dim][nbnd_nodes] scratch (caller-owned) */
 } mg_wall;
 
 /* Numerical inviscid flux kinds: inviscid_facial_flux_rusanov,

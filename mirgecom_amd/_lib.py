@@ -57,7 +57,9 @@ _D = ctypes.c_double
 class MgWall(ctypes.Structure):
     """``mg_wall`` (include/mgdg.h): Mengaldo boundary data."""
     _fields_ = [("q_bc", ctypes.c_void_p), ("t_bc", ctypes.c_void_p),
-                ("t_state_bc", ctypes.c_void_p), ("flags", ctypes.c_void_p)]
+                ("t_state_bc", ctypes.c_void_p), ("flags", ctypes.c_void_p),
+                ("vol_index", ctypes.c_void_p), ("normals", ctypes.c_void_p),
+                ("work", ctypes.c_void_p)]
 
 
 _W = ctypes.POINTER(MgWall)

@@ -127,10 +127,15 @@ class Prepared:
                                      "boundary condition")
                 dev.checked_tag_sets.add(key)
             if walls:
+                self._wwork = torch.empty(((q.shape[0] + 1) * self.dim, nb),
+                                          dtype=q.dtype, device=q.device)
                 self.wall = _lib.MgWall(
                     q_bc=self._wq.data_ptr(), t_bc=self._wt.data_ptr(),
                     t_state_bc=(self._wts.data_ptr() if self.mix else None),
-                    flags=self._wflag.data_ptr())
+                    flags=self._wflag.data_ptr(),
+                    vol_index=dev.bnd_vol_index32.data_ptr(),
+                    normals=dev.bnd_normals.data_ptr(),
+                    work=self._wwork.data_ptr())
         elif boundaries:
             unknown = [b.tag for b in boundaries
                        if b.tag != BTAG_ALL

@@ -64,6 +64,9 @@ struct KArgs {
   const double* wt;     // T_bc [nbnd]
   const double* wts;    // temperature of state_bc [nbnd] (mixtures)
   const int32_t* wflag; // MG_BC_* bits [nbnd]
+  const int32_t* wvol;  // interior volume node of each boundary node [nbnd]
+  const double* wn;     // unit normal [DIM][nbnd]
+  double* wwork;        // boundary gradients / viscous fluxes [(F+1)*DIM][nbnd]
   int64_t K;
   int64_t nbnd;
   int64_t nnodes;      // K * NP
@@ -502,11 +505,13 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
         if constexpr (VISC) {
           double gq[NFLD][DIM], gt[DIM], fv[NFLD];
           load_grad<DIM, NSP>(gq, gt, a.grad, N, bnd ? e * NP + im : int64_t(nb));
-          const int wfl = bnd ? wall_flags(a, b) : 0;
-          if (wfl & kBcMengaldo)
-            wall_viscous_flux_n<DIM, NSP, MIX>(a, b, e * NP + im, wfl, n, gq, gt, fv);
-          else
+          if (bnd && (wall_flags(a, b) & kBcMengaldo)) {
+            // precomputed by wall_flux_kernel
+#pragma unroll
+            for (int k = 0; k < NFLD; ++k) fv[k] = a.wwork[k * a.nbnd + b];
+          } else {
             viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
+          }
 #pragma unroll
           for (int k = 0; k < NFLD; ++k) fnp[k] -= fv[k];
         }
@@ -883,6 +888,15 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
         const int64_t b = -int64_t(nb) - 1;
         const int f = fn / NFP;
         const int im = s_fmask[fn];
+        if (wall_flags(a, b) & kBcMengaldo) {
+          // the interior gradients go to wall_flux_kernel, which forms this
+          // node's exterior flux
+          const int ti = le * NP + im;
+#pragma unroll
+          for (int k = 0; k < NS * DIM; ++k)
+            a.wwork[k * a.nbnd + b] = double(stage[k * (EPB * NP) + ti]);
+          continue;
+        }
         double n[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) n[d] = s_geo[le * GEO + DIM * DIM + f * DIM + d];
@@ -904,17 +918,68 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 #pragma unroll
           for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
         }
-        const int wfl = wall_flags(a, b);
-        if (wfl & kBcMengaldo)
-          wall_viscous_flux_n<DIM, NSP, MIX>(a, b, e * NP + im, wfl, n, gq, gt, fv);
-        else
-          viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
+        viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
 #pragma unroll
         for (int k = 0; k < NFLD; ++k)
           a.fnb_out[k * a.nbnd + b] =
               fi[k] - fv[k] - ((k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0);
       }
     }
+  }
+}
+
+// }}}
+
+// {{{ Mengaldo boundary nodes
+//
+// One thread per boundary node; nodes without MG_BC_MENGALDO return at once.
+// FUSED: the gradients come from wwork (stored by grad_kernel<FLUX>) and the
+// node's exterior net normal flux F_I(q+).n - [2 F_V(bc) - F_V(q-)].n (- p_ref n
+// for the mixed path) is written to fnb_out.  Otherwise the gradients come
+// from the full gradient array and the effective exterior viscous flux
+// 2 F_V(bc).n - F_V(q-).n is written to wwork for rhs_kernel<kGrad>.
+template <int DIM, int NSP, bool MIX, bool FUSED>
+__global__ __launch_bounds__(kBlock) void wall_flux_kernel(KArgs a, int shift_pref) {
+  constexpr int NFLD = 2 + DIM + NSP;
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= a.nbnd) return;
+  const int fl = a.wflag[b];
+  if (!(fl & kBcMengaldo)) return;
+  const int64_t gm = a.wvol[b];
+  double n[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) n[d] = a.wn[d * a.nbnd + b];
+  double gq[NFLD][DIM], gt[DIM], fv[NFLD];
+  if constexpr (FUSED) {
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) gq[k][d] = a.wwork[(k * DIM + d) * a.nbnd + b];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) gt[d] = a.wwork[(NFLD * DIM + d) * a.nbnd + b];
+  } else {
+    load_grad<DIM, NSP>(gq, gt, a.grad, a.nnodes, gm);
+  }
+  wall_viscous_flux_n<DIM, NSP, MIX>(a, b, gm, fl, n, gq, gt, fv);
+  if constexpr (FUSED) {
+    double fi[NFLD];
+    if (a.inviscid_on) {
+      Node<DIM, NSP> s;
+      load_q<DIM, NSP>(s, a.qb, a.nbnd, b);
+      node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.tb[b] : 0.0);
+      inviscid_flux_n<DIM, NSP>(s, n, fi);
+    } else {
+#pragma unroll
+      for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
+    }
+    const double p_ref = shift_pref ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k)
+      a.fnb_out[k * a.nbnd + b] =
+          fi[k] - fv[k] - ((k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) a.wwork[k * a.nbnd + b] = fv[k];
   }
 }
 
@@ -1211,13 +1276,46 @@ mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
 
 mg_status set_wall(mgdg::KArgs& a, const mg_mesh* m, const mg_wall* w, bool mix) {
   if (!w || !w->flags || m->nbnd_nodes == 0) return MG_OK;
-  if (!w->q_bc || !w->t_bc || (mix && !w->t_state_bc))
-    return fail(MG_ERR_ARG, "mg_wall: q_bc / t_bc (and t_state_bc for mixtures) needed");
+  if (!w->q_bc || !w->t_bc || (mix && !w->t_state_bc) || !w->vol_index ||
+      !w->normals || !w->work)
+    return fail(MG_ERR_ARG,
+                "mg_wall: q_bc, t_bc, vol_index, normals, work (and t_state_bc for "
+                "mixtures) needed");
   a.wq = w->q_bc;
   a.wt = w->t_bc;
   a.wts = w->t_state_bc;
   a.wflag = w->flags;
+  a.wvol = w->vol_index;
+  a.wn = w->normals;
+  a.wwork = w->work;
   return MG_OK;
+}
+
+template <int DIM, int NSP, bool MIX, bool FUSED>
+mg_status launch_wall(const mgdg::KArgs& a, int shift, hipStream_t st) {
+  const unsigned nblk = unsigned((a.nbnd + mgdg::kBlock - 1) / mgdg::kBlock);
+  hipLaunchKernelGGL((mgdg::wall_flux_kernel<DIM, NSP, MIX, FUSED>), dim3(nblk),
+                     dim3(mgdg::kBlock), 0, st, a, shift);
+  return check_launch("wall_flux_kernel");
+}
+
+// Mengaldo boundary nodes: after grad_kernel<FLUX> (fused) or before
+// rhs_kernel<kGrad>; nothing to do without wall data.
+template <bool FUSED>
+mg_status wall_pass(const mgdg::KArgs& a, int dim, int nsp, bool mix, int shift,
+                    hipStream_t st) {
+  if (!a.wflag) return MG_OK;
+  if (dim == 2) {
+    if (nsp == 0 && !mix) return launch_wall<2, 0, false, FUSED>(a, shift, st);
+    if (nsp == 7 && !mix) return launch_wall<2, 7, false, FUSED>(a, shift, st);
+    if (nsp == 7 && mix) return launch_wall<2, 7, true, FUSED>(a, shift, st);
+  } else {
+    if (nsp == 0 && !mix) return launch_wall<3, 0, false, FUSED>(a, shift, st);
+    if (nsp == 7 && !mix) return launch_wall<3, 7, false, FUSED>(a, shift, st);
+    if (nsp == 7 && mix) return launch_wall<3, 7, true, FUSED>(a, shift, st);
+  }
+  return fail(MG_ERR_UNSUPPORTED,
+              "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
 template <int DIM, int NSP, bool MIX>
@@ -1361,7 +1459,9 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.out = rhs;
   a.flux_kind = flux_kind;
   a.inviscid_on = inviscid_on ? 1 : 0;
-  return dispatch(mesh, a, gas->nspecies, mix, 1, static_cast<hipStream_t>(stream));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if ((s = wall_pass<false>(a, mesh->dim, gas->nspecies, mix, 0, st))) return s;
+  return dispatch(mesh, a, gas->nspecies, mix, 1, st);
 }
 
 mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
@@ -1387,7 +1487,9 @@ mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.grad_out = grad;
   a.fnb_out = fnet_bnd;
   a.inviscid_on = inviscid_on ? 1 : 0;
-  return dispatch(mesh, a, gas->nspecies, mix, 4, static_cast<hipStream_t>(stream));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if ((s = dispatch(mesh, a, gas->nspecies, mix, 4, st))) return s;
+  return wall_pass<true>(a, mesh->dim, gas->nspecies, mix, 0, st);
 }
 
 mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
@@ -1441,7 +1543,9 @@ mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas, const double*
   a.fnet32_out = fnet;
   a.fnb_out = fnet_bnd;
   a.inviscid_on = inviscid_on ? 1 : 0;
-  return dispatch(mesh, a, gas->nspecies, mix, 5, static_cast<hipStream_t>(stream));
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if ((s = dispatch(mesh, a, gas->nspecies, mix, 5, st))) return s;
+  return wall_pass<true>(a, mesh->dim, gas->nspecies, mix, 1, st);
 }
 
 mg_status mg_ns_div_mixed(const mg_mesh* mesh, const mg_gas* gas, const double* q,

@@ -47,6 +47,8 @@ class DeviceDiscretization:
             vol_idx = np.zeros(0, dtype=np.int64)
         self.bnd_vol_index = torch.as_tensor(vol_idx, dtype=torch.int64,
                                              device=self.device)
+        self.bnd_vol_index32 = torch.as_tensor(vol_idx, dtype=torch.int32,
+                                               device=self.device)
         self.bnd_nodes = t64(dcoll.bnd_nodes)
         self.bnd_normals = t64(dcoll.bnd_normals)
         self._tag_index = {}

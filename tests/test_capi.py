@@ -81,7 +81,8 @@ def test_wall_validation():
     m = _lib.MgMesh(dim=2, order=1, np=3, nfp=2, nfaces=3, nelements=2,
                     nbnd_nodes=4, rx=1, normals=1, fscale=1, vmap_p=1,
                     weak_diff=1, lift=1, face_mask=1, workspace=1)
-    w = _lib.MgWall(q_bc=None, t_bc=None, t_state_bc=None, flags=1)
+    w = _lib.MgWall(q_bc=1, t_bc=1, t_state_bc=None, flags=1, vol_index=None,
+                    normals=1, work=1)
     rc = lib.mg_ns_gradient(m, _lib.MgGas(gamma=1.4, gas_const=1.0), 1, None,
                             1, None, w, 1, None)
     assert rc == 1

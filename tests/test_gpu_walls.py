@@ -68,7 +68,7 @@ def _ideal(dim, order, nsp, kind):
     n = (4, 3) if dim == 2 else (3, 2, 2)
     pb = Problem(dim, order, n, 0.0, (1.0, 0.8, 0.6)[:dim], (False,) * dim)
     q0 = smooth_perturbed_state(pb.x, seed=17, nspecies=nsp)
-    d = np.linspace(0.01, 0.03, nsp) if nsp else None
+    d = np.linspace(0.01, 0.03, nsp) if nsp else None  # passive species
     gm = GasModel(eos=IdealSingleGas(gas_const=1.0),
                   transport=SimpleTransport(viscosity=0.02, bulk_viscosity=0.01,
                                             thermal_conductivity=0.03,
@@ -84,7 +84,7 @@ def _ideal(dim, order, nsp, kind):
 
 
 @pytest.mark.parametrize("kind", KINDS)
-@pytest.mark.parametrize("dim,order,nsp", [(2, 3, 0), (3, 2, 2)])
+@pytest.mark.parametrize("dim,order,nsp", [(2, 3, 0), (3, 2, 7)])
 def test_ns_walls(kind, dim, order, nsp):
     from mirgecom_amd.inviscid import inviscid_facial_flux_hll
     from mirgecom_amd.navierstokes import ns_operator
