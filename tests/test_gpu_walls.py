@@ -99,8 +99,10 @@ def test_ns_walls(kind, dim, order, nsp):
     # gradients: CV_bc / T_bc boundary values
     ops = oop.OperatorStates(pb.odisc, pb.box, ost, ogm, obnd, 0.0)
     gref = oop.grad_cv_operator(ops)
+    # rho = 1 +- 0.05: grad rho is ~20x smaller than the values whose
+    # differences form it, so its round-off floor is ~20x the RHS's
     assert rel_err(_host(gcv.q).reshape(gref.shape[0], -1),
-                   gref.reshape(gref.shape[0], -1)).max() < TOL
+                   gref.reshape(gref.shape[0], -1)).max() < 20 * TOL
     # from explicit gradients (mg_ns_rhs), with HLL
     rhs2 = ns_operator(pb.dcoll, gm, st, gbnd, grad_cv=gcv, grad_t=gt,
                        inviscid_numerical_flux_func=inviscid_facial_flux_hll)
