@@ -183,17 +183,22 @@ __device__ __forceinline__ const R* packed_ops(const double* ws) {
 // -> out[plane*N + (e0 + le)*NP + i].
 template <int NP, typename S, typename T>
 __device__ __forceinline__ void write_staged(const S* stage, int nplanes,
-                                             int epb, int64_t e0, int64_t K,
-                                             T* __restrict__ out) {
+                                             int epb, int pstride, int64_t e0,
+                                             int64_t K, T* __restrict__ out) {
   const int64_t N = K * NP;
   const int64_t nvalid = (K - e0 < epb ? K - e0 : epb) * NP;
   const int per = epb * NP;
   for (int t = threadIdx.x; t < nplanes * per; t += blockDim.x) {
     const int plane = t / per;
     const int idx = t - plane * per;
-    if (idx < nvalid) out[plane * N + e0 * NP + idx] = T(stage[t]);
+    if (idx < nvalid) out[plane * N + e0 * NP + idx] = T(stage[plane * pstride + idx]);
   }
 }
+
+// LDS row strides are padded to an odd number of elements: rows written by
+// consecutive lanes then fall into different banks (an even stride of 60-64
+// doubles maps whole waves onto a few banks).
+constexpr int odd_stride(int n) { return n | 1; }
 
 // Reference pressure of the mixed-precision path: the pressure at the first
 // node of the mesh.  Subtracting a constant p_ref from the momentum flux
@@ -309,12 +314,15 @@ struct RhsCfg {
   static constexpr int EPB = 64 / NFLD;           // columns fit one wave
   static constexpr int COLS = EPB * NFLD;
   static constexpr int NODEF = NFLD * DIM;        // net flux
-  static constexpr int SF = EPB * E::NP * NODEF;
-  static constexpr int SFACE = E::NFN * COLS;
+  static constexpr int NODEP = odd_stride(NODEF);  // LDS node stride
+  static constexpr int FROW = odd_stride(COLS);    // LDS face-row stride
+  static constexpr int SP = odd_stride(EPB * E::NP);  // staging plane stride
+  static constexpr int SF = EPB * E::NP * NODEP;
+  static constexpr int SFACE = E::NFN * FROW;
   static constexpr int SGEO = EPB * DIM * DIM;
   static constexpr int SREF = sizeof(R) == 4 ? EPB * NODEF : 0;
   static constexpr int SWS = EPB * E::NP;          // wavespeed |v| + c per node
-  static constexpr int STAGE = NFLD * EPB * E::NP;
+  static constexpr int STAGE = NFLD * SP;
   static_assert(STAGE <= SF, "staging aliases the flux buffer");
   static constexpr size_t LDS = sizeof(R) * (SF + SFACE) +
                                 sizeof(double) * (SGEO + SREF + SWS) +
@@ -332,7 +340,8 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   using C = RhsCfg<DIM, P, NSP, MIX, MODE, R>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
   constexpr int NFLD = C::NFLD, EPB = C::EPB, COLS = C::COLS;
-  constexpr int RPW = O::RPW, NODEF = C::NODEF;
+  constexpr int RPW = O::RPW, NODEF = C::NODEF, NODEP = C::NODEP;
+  constexpr int FROW = C::FROW, SP = C::SP;
   constexpr bool VISC = MODE == kGrad;
   constexpr bool MIXED = sizeof(R) == 4;
   static_assert(!MIXED || MODE == kFlux, "mixed precision is the fused NS path");
@@ -341,7 +350,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   double* s_ref = s_geo + C::SGEO;                        // [EPB][NODEF] (mixed)
   double* s_ws = s_ref + C::SREF;                         // [EPB][NP]
   R* s_f = reinterpret_cast<R*>(s_ws + C::SWS);           // [EPB][NP][NFLD][DIM]
-  R* s_face = s_f + C::SF;                                // [NFN][COLS]
+  R* s_face = s_f + C::SF;                                // [NFN][FROW]
   int* s_fmask = reinterpret_cast<int*>(s_face + C::SFACE);
   const R* ops = packed_ops<R, DIM, P>(ws);
 
@@ -369,7 +378,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
     const int64_t e = e0 + le;
     if (e >= a.K) continue;
     const int64_t g = e * NP + i;
-    R* fdst = s_f + t * NODEF;
+    R* fdst = s_f + t * NODEP;
     if constexpr (MODE == kFlux) {
       Node<DIM, NSP> s;
       load_q<DIM, NSP>(s, a.q, N, g);
@@ -437,7 +446,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
 #pragma unroll
       for (int d = 0; d < DIM; ++d) {
         const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
-        c += (double(s_f[lm * NODEF + k * DIM + d]) + r0) * n[d];
+        c += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
         sh += r0 * n[d];
       }
       fnm[k] = c;
@@ -461,7 +470,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
         double c = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; ++d)
-          c += (double(s_f[lp * NODEF + k * DIM + d]) +
+          c += (double(s_f[lp * NODEP + k * DIM + d]) +
                 (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
         fnp[k] = c;
       }
@@ -544,7 +553,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
     }
 #pragma unroll
     for (int k = 0; k < NFLD; ++k)
-      s_face[fn * COLS + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
+      s_face[fn * FROW + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
   }
   __syncthreads();
 
@@ -568,7 +577,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   for (int rr = 0; rr < RPW; ++rr) acc[rr] = R(0);
 #pragma unroll 2
   for (int j = 0; j < NP; ++j) {
-    const R* fj = s_f + (cle * NP + j) * NODEF + cf * DIM;
+    const R* fj = s_f + (cle * NP + j) * NODEP + cf * DIM;
     R fd[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) fd[d] = fj[d];
@@ -583,22 +592,22 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   }
 #pragma unroll 4
   for (int m = 0; m < NFN; ++m) {
-    const R bv = s_face[m * COLS + cc];
+    const R bv = s_face[m * FROW + cc];
 #pragma unroll
     for (int rr = 0; rr < RPW; ++rr) acc[rr] += LF[m * RPW + rr] * bv;
   }
   __syncthreads();
   // phase 4: stage in LDS (over s_f), write coalesced
-  R* stage = s_f;                            // [NFLD][EPB*NP]
+  R* stage = s_f;                            // [NFLD][SP]
   if (c < COLS) {
 #pragma unroll
     for (int rr = 0; rr < RPW; ++rr) {
       const int i = w * RPW + rr;
-      if (i < NP) stage[cf * (EPB * NP) + cle * NP + i] = acc[rr];
+      if (i < NP) stage[cf * SP + cle * NP + i] = acc[rr];
     }
   }
   __syncthreads();
-  write_staged<NP>(stage, NFLD, EPB, e0, a.K, out);
+  write_staged<NP>(stage, NFLD, EPB, SP, e0, a.K, out);
 }
 
 // }}}
@@ -624,10 +633,12 @@ struct GradCfg {
   static constexpr int NS = 3 + DIM + NSP;  // conserved fields + T
   static constexpr int EPB = 64 / NS;
   static constexpr int COLS = 64;
+  static constexpr int ROW = odd_stride(COLS);         // LDS trace-row stride
+  static constexpr int SP = odd_stride(EPB * E::NP);   // gradient plane stride
   static constexpr int KG = E::NP + E::NFN;
   static constexpr int GEO = DIM * DIM + E::NF * DIM;  // rx and normals
-  static constexpr int STAGE = NS * DIM * EPB * E::NP;
-  static constexpr int SB = KG * COLS > STAGE ? KG * COLS : STAGE;
+  static constexpr int STAGE = NS * DIM * SP;
+  static constexpr int SB = KG * ROW > STAGE ? KG * ROW : STAGE;
   static constexpr int SREF = sizeof(R) == 4 ? EPB * NS : 0;
   static constexpr size_t LDS = sizeof(R) * SB + sizeof(double) * (EPB * GEO + SREF) +
                                 sizeof(int) * E::NFN;
@@ -659,13 +670,14 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   using C = GradCfg<DIM, P, NSP, MIX, R>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
   constexpr int NFLD = 2 + DIM + NSP, NS = C::NS, EPB = C::EPB, COLS = C::COLS;
+  constexpr int ROW = C::ROW, SP = C::SP;
   constexpr int RPW = O::RPW, GEO = C::GEO;
   constexpr bool MIXED = sizeof(R) == 4;
   static_assert(!MIXED || FLUX, "mixed precision is the fused NS path");
   extern __shared__ double smem[];
   double* s_geo = smem;                          // [EPB][GEO]
   double* s_ref = s_geo + EPB * GEO;             // [EPB][NS] (mixed)
-  R* s_b = reinterpret_cast<R*>(s_ref + C::SREF);  // [NP + NFN][COLS]
+  R* s_b = reinterpret_cast<R*>(s_ref + C::SREF);  // [NP + NFN][ROW]
   R* stage = s_b;                                // [NS][DIM][EPB*NP], after phase 3
   int* s_fmask = reinterpret_cast<int*>(s_b + C::SB);
   const R* ops = packed_ops<R, DIM, P>(ws);
@@ -718,8 +730,8 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
     const double tn = node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g] : 0.0);
     const double* ref = s_ref + le * NS;
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_b[i * COLS + le * NS + f] = R(qv[f] - (MIXED ? ref[f] : 0.0));
-    s_b[i * COLS + le * NS + NFLD] = R(tn - (MIXED ? ref[NFLD] : 0.0));
+    for (int f = 0; f < NFLD; ++f) s_b[i * ROW + le * NS + f] = R(qv[f] - (MIXED ? ref[f] : 0.0));
+    s_b[i * ROW + le * NS + NFLD] = R(tn - (MIXED ? ref[NFLD] : 0.0));
   }
   __syncthreads();
 
@@ -734,13 +746,13 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
     const int im = s_fmask[fn];
     const int32_t nb = a.vmap_p[e * NFN + fn];
     const double fs = a.fscale[e * NF + f];
-    R* dst = s_b + (NP + fn) * COLS + le * NS;
-    const R* own = s_b + im * COLS + le * NS;
+    R* dst = s_b + (NP + fn) * ROW + le * NS;
+    const R* own = s_b + im * ROW + le * NS;
     const double* ref = s_ref + le * NS;
     if (nb >= g_lo && nb < g_hi) {
       const int lp = int(nb - g_lo);
       const int ple = lp / NP;
-      const R* nbv = s_b + (lp - ple * NP) * COLS + ple * NS;
+      const R* nbv = s_b + (lp - ple * NP) * ROW + ple * NS;
       const double* pref = s_ref + ple * NS;
 #pragma unroll
       for (int k = 0; k <= NFLD; ++k) {
@@ -793,7 +805,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   }
 #pragma unroll 2
   for (int j = 0; j < NP; ++j) {
-    const R bv = s_b[j * COLS + c];
+    const R bv = s_b[j * ROW + c];
 #pragma unroll
     for (int r = 0; r < DIM; ++r)
 #pragma unroll
@@ -803,7 +815,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   for (int f = 0; f < NF; ++f) {
 #pragma unroll 2
     for (int m = 0; m < NFP; ++m) {
-      const R bv = s_b[(NP + f * NFP + m) * COLS + c];
+      const R bv = s_b[(NP + f * NFP + m) * ROW + c];
 #pragma unroll
       for (int rr = 0; rr < RPW; ++rr) accl[rr][f] += GL[(f * NFP + m) * RPW + rr] * bv;
     }
@@ -825,15 +837,15 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
         for (int r = 0; r < DIM; ++r) vsum += R(geo[r * DIM + d]) * accw[rr][r];
 #pragma unroll
         for (int f = 0; f < NF; ++f) fsum += R(geo[DIM * DIM + f * DIM + d]) * accl[rr][f];
-        stage[(sc * DIM + d) * (EPB * NP) + le * NP + i] = fsum - vsum;
+        stage[(sc * DIM + d) * SP + le * NP + i] = fsum - vsum;
       }
     }
   }
   __syncthreads();
   if constexpr (!FLUX) {
-    write_staged<NP>(stage, NS * DIM, EPB, e0, a.K, out);
+    write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, out);
   } else {
-    if (a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, e0, a.K, a.grad_out);
+    if (a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, a.grad_out);
     const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
     // phase 5: net flux F_I - F_V at every node, written coalesced
     for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
@@ -849,9 +861,9 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 #pragma unroll
       for (int f = 0; f < NFLD; ++f)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) gq[f][d] = double(stage[(f * DIM + d) * (EPB * NP) + t]);
+        for (int d = 0; d < DIM; ++d) gq[f][d] = double(stage[(f * DIM + d) * SP + t]);
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * (EPB * NP) + t]);
+      for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * SP + t]);
       viscous_flux<DIM, NSP, MIX>(s, gq, gt, F);
       double FI[NFLD][DIM];
       if (a.inviscid_on) {
@@ -894,7 +906,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
           const int ti = le * NP + im;
 #pragma unroll
           for (int k = 0; k < NS * DIM; ++k)
-            a.wwork[k * a.nbnd + b] = double(stage[k * (EPB * NP) + ti]);
+            a.wwork[k * a.nbnd + b] = double(stage[k * SP + ti]);
           continue;
         }
         double n[DIM];
@@ -909,9 +921,9 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
         for (int k = 0; k < NFLD; ++k)
 #pragma unroll
           for (int d = 0; d < DIM; ++d)
-            gq[k][d] = double(stage[(k * DIM + d) * (EPB * NP) + ti]);
+            gq[k][d] = double(stage[(k * DIM + d) * SP + ti]);
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * (EPB * NP) + ti]);
+        for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * SP + ti]);
         if (a.inviscid_on) {
           inviscid_flux_n<DIM, NSP>(s, n, fi);
         } else {
