@@ -195,6 +195,21 @@ __device__ __forceinline__ void write_staged(const S* stage, int nplanes,
   }
 }
 
+// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
+// XCDs (blockIdx.x % 8 picks the XCD), each with its own L2.  Consecutive
+// element blocks are mesh neighbours (elements are ordered cube by cube), so
+// the logical block index gives every XCD one contiguous range of the mesh:
+// the neighbour gathers of a block then mostly hit the L2 that loaded the
+// neighbouring blocks.  A bijection on [0, gridDim.x).
+constexpr int kXcds = 8;
+
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t nb = gridDim.x, b = blockIdx.x;
+  const int64_t q = nb / kXcds, r = nb % kXcds;
+  const int64_t x = b % kXcds, i = b / kXcds;
+  return x * q + (x < r ? x : r) + i;
+}
+
 // LDS row strides are padded to an odd number of elements: rows written by
 // consecutive lanes then fall into different banks (an even stride of 60-64
 // doubles maps whole waves onto a few banks).
@@ -355,7 +370,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
-  const int64_t e0 = int64_t(blockIdx.x) * EPB;
+  const int64_t e0 = xcd_block() * EPB;
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   const bool inv = MODE == kEuler || a.inviscid_on;
@@ -683,7 +698,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
-  const int64_t e0 = int64_t(blockIdx.x) * EPB;
+  const int64_t e0 = xcd_block() * EPB;
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
