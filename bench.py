@@ -159,6 +159,18 @@ def measure_reacting(args, n):
             "unit": "DOF-RHS/s", "kernels": tab}
 
 
+def measured_traffic(entry, n, order):
+    """HBM bytes per launch of ``entry`` on the 24^3x6 p=3 workload from the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated),
+    or None for other workloads."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "ns_rhs_tgv24_p3_traffic.json")
+    if (n, order) != (24, 3) or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)["kernels"].get(entry, {}).get("traffic_bytes")
+
+
 def _tgv_transport():
     from mirgecom_amd.initializers import TaylorGreenVortex
     return TaylorGreenVortex(mach=0.1).transport_for(
@@ -362,7 +374,9 @@ def main():
                    "reacting": reacting},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(dom, n, order),
+                     "alg_bytes": ab[dom]},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
