@@ -346,7 +346,9 @@ struct RhsCfg {
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
 
-template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
+// HLL selects the HLL face flux (compiled separately: its two full face
+// states cost registers that the Rusanov / central path does not need).
+template <int DIM, int P, int NSP, bool MIX, int MODE, typename R, bool HLL>
 __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
                                                      const double* __restrict__ ws,
                                                      double* __restrict__ out) {
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
 
   // phase 2: numerical face fluxes (FP64).  The own side and neighbours
   // inside this workgroup come from LDS; other neighbours are gathered here.
-  const bool hll = inv && a.flux_kind == 1;
+  constexpr bool hll = HLL;
   for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
     const int le = t / NFN;
     const int fn = t - le * NFN;
@@ -542,7 +544,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
       }
     }
     double fl[NFLD];
-    if (hll) {
+    if constexpr (hll) {
       Node<DIM, NSP> sm, sp;
 #pragma unroll
       for (int k = 0; k < NFLD; ++k) {
@@ -1191,10 +1193,10 @@ mg_status set_lds(Kern kern, size_t lds) {
   return MG_OK;
 }
 
-template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
-mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
+template <int DIM, int P, int NSP, bool MIX, int MODE, typename R, bool HLL>
+mg_status launch_rhs_kernel(const mgdg::KArgs& a, hipStream_t st) {
   using C = mgdg::RhsCfg<DIM, P, NSP, MIX, MODE, R>;
-  auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, MODE, R>;
+  auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, MODE, R, HLL>;
   static bool attr = false;
   if (!attr) {
     mg_status s = set_lds(kern, C::LDS);
@@ -1205,6 +1207,14 @@ mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
   hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(mgdg::kBlock), C::LDS, st, a,
                      a.ops, a.out);
   return check_launch("rhs_kernel");
+}
+
+template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
+mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
+  const bool inv = MODE == mgdg::kEuler || a.inviscid_on;
+  if (inv && a.flux_kind == MG_FLUX_HLL)
+    return launch_rhs_kernel<DIM, P, NSP, MIX, MODE, R, true>(a, st);
+  return launch_rhs_kernel<DIM, P, NSP, MIX, MODE, R, false>(a, st);
 }
 
 template <int DIM, int P, int NSP, bool MIX, bool FLUX, typename R>
