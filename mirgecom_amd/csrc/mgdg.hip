@@ -25,6 +25,7 @@
 namespace mgdg {
 
 __constant__ MechDev c_mech;
+__constant__ PolyDev c_poly;
 
 constexpr int kBlock = 256;
 constexpr int kLdsBudget = 64 * 1024;
@@ -443,6 +444,11 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   // phase 2: numerical face fluxes (FP64).  The own side and neighbours
   // inside this workgroup come from LDS; other neighbours are gathered here.
   constexpr bool hll = HLL;
+  // With many fields (the 12-field mixture) the Rusanov / central face flux
+  // is formed field by field, without per-field arrays that would hold ~70
+  // doubles per thread and cut occupancy to one wave per SIMD; with five
+  // fields the arrays are cheap and let all loads issue up front.
+  constexpr bool STREAM = !HLL && NFLD > 6;
   for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
     const int le = t / NFN;
     const int fn = t - le * NFN;
@@ -456,80 +462,146 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
     for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + f) * DIM + d];
     const double fs = a.fscale[e * NF + f];
     const int lm = le * NP + im;
-    double fnm[NFLD], fnp[NFLD], qm[NFLD], qp[NFLD], shift[NFLD];
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k) {
-      double c = 0.0, sh = 0.0;
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-        const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
-        c += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
-        sh += r0 * n[d];
-      }
-      fnm[k] = c;
-      shift[k] = sh;
-    }
-    const int64_t gm = e * NP + im;
-    const double tm = MIX ? a.temp[gm] : 0.0;
-    const double wsm = s_ws[lm];
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k) qm[k] = a.q[k * N + gm];
-    double tp, wsp;
-    if (nb >= g_lo && nb < g_hi) {
-      const int lp = int(nb - g_lo);
-      const int ple = lp / NP;
-      tp = MIX ? a.temp[nb] : 0.0;
-      wsp = s_ws[lp];
-#pragma unroll
-      for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
-#pragma unroll
+    if constexpr (!STREAM) {
+      double fnm[NFLD], fnp[NFLD], qm[NFLD], qp[NFLD], shift[NFLD];
+  #pragma unroll
       for (int k = 0; k < NFLD; ++k) {
-        double c = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d)
-          c += (double(s_f[lp * NODEP + k * DIM + d]) +
-                (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
-        fnp[k] = c;
+        double c = 0.0, sh = 0.0;
+  #pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
+          c += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
+          sh += r0 * n[d];
+        }
+        fnm[k] = c;
+        shift[k] = sh;
       }
-    } else {
-      const bool bnd = nb < 0;
-      const int64_t b = bnd ? int64_t(-nb - 1) : 0;
-      Node<DIM, NSP> sp;
-      if (!bnd) {
-        load_q<DIM, NSP>(sp, a.q, N, nb);
+      const int64_t gm = e * NP + im;
+      const double tm = MIX ? a.temp[gm] : 0.0;
+      const double wsm = s_ws[lm];
+  #pragma unroll
+      for (int k = 0; k < NFLD; ++k) qm[k] = a.q[k * N + gm];
+      double tp, wsp;
+      if (nb >= g_lo && nb < g_hi) {
+        const int lp = int(nb - g_lo);
+        const int ple = lp / NP;
         tp = MIX ? a.temp[nb] : 0.0;
+        wsp = s_ws[lp];
+  #pragma unroll
+        for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
+  #pragma unroll
+        for (int k = 0; k < NFLD; ++k) {
+          double c = 0.0;
+  #pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            c += (double(s_f[lp * NODEP + k * DIM + d]) +
+                  (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
+          fnp[k] = c;
+        }
       } else {
-        load_q<DIM, NSP>(sp, a.qb, a.nbnd, b);
-        tp = MIX ? a.tb[b] : 0.0;
-      }
-      node_state<DIM, NSP, MIX, VISC>(sp, a.gas, tp);
-      wsp = sp.speed + sp.c;
-#pragma unroll
-      for (int k = 0; k < NFLD; ++k) qp[k] = sp.q[k];
-      if constexpr (MODE == kFlux) {
+        const bool bnd = nb < 0;
+        const int64_t b = bnd ? int64_t(-nb - 1) : 0;
+        Node<DIM, NSP> sp;
         if (!bnd) {
-#pragma unroll
-          for (int k = 0; k < NFLD; ++k) {
-            double c = 0.0;
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-              c += (MIXED ? double(a.fnet32[(k * DIM + d) * N + nb])
-                          : a.fnet[(k * DIM + d) * N + nb]) * n[d];
-            fnp[k] = c;
+          load_q<DIM, NSP>(sp, a.q, N, nb);
+          tp = MIX ? a.temp[nb] : 0.0;
+        } else {
+          load_q<DIM, NSP>(sp, a.qb, a.nbnd, b);
+          tp = MIX ? a.tb[b] : 0.0;
+        }
+        node_state<DIM, NSP, MIX, VISC>(sp, a.gas, tp);
+        wsp = sp.speed + sp.c;
+  #pragma unroll
+        for (int k = 0; k < NFLD; ++k) qp[k] = sp.q[k];
+        if constexpr (MODE == kFlux) {
+          if (!bnd) {
+  #pragma unroll
+            for (int k = 0; k < NFLD; ++k) {
+              double c = 0.0;
+  #pragma unroll
+              for (int d = 0; d < DIM; ++d)
+                c += (MIXED ? double(a.fnet32[(k * DIM + d) * N + nb])
+                            : a.fnet[(k * DIM + d) * N + nb]) * n[d];
+              fnp[k] = c;
+            }
+          } else {
+  #pragma unroll
+            for (int k = 0; k < NFLD; ++k) fnp[k] = a.fnb[k * a.nbnd + b];
           }
         } else {
-#pragma unroll
-          for (int k = 0; k < NFLD; ++k) fnp[k] = a.fnb[k * a.nbnd + b];
+          if (inv) {
+            inviscid_flux_n<DIM, NSP>(sp, n, fnp);
+          } else {
+  #pragma unroll
+            for (int k = 0; k < NFLD; ++k) fnp[k] = 0.0;
+          }
+          if constexpr (VISC) {
+            double gq[NFLD][DIM], gt[DIM], fv[NFLD];
+            load_grad<DIM, NSP>(gq, gt, a.grad, N, bnd ? e * NP + im : int64_t(nb));
+            if (bnd && (wall_flags(a, b) & kBcMengaldo)) {
+              // precomputed by wall_flux_kernel
+  #pragma unroll
+              for (int k = 0; k < NFLD; ++k) fv[k] = a.wwork[k * a.nbnd + b];
+            } else {
+              viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
+            }
+  #pragma unroll
+            for (int k = 0; k < NFLD; ++k) fnp[k] -= fv[k];
+          }
+        }
+      }
+      double fl[NFLD];
+      if constexpr (hll) {
+        Node<DIM, NSP> sm, sp;
+  #pragma unroll
+        for (int k = 0; k < NFLD; ++k) {
+          sm.q[k] = qm[k];
+          sp.q[k] = qp[k];
+        }
+        node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
+        node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
+        double fim[NFLD], fip[NFLD];
+        inviscid_flux_n<DIM, NSP>(sm, n, fim);
+        inviscid_flux_n<DIM, NSP>(sp, n, fip);
+        inviscid_numflux<DIM, NSP>(sm, sp, n, 1, fl);
+        // stored net fluxes exclude p_ref I: F_V.n = F_I.n - (Fn + p_ref n)
+  #pragma unroll
+        for (int k = 0; k < NFLD; ++k) {
+          const double pn = (k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0;
+          fl[k] -= ((fim[k] - fnm[k] - pn) + (fip[k] - fnp[k] - pn)) / 2 + pn;
         }
       } else {
-        if (inv) {
-          inviscid_flux_n<DIM, NSP>(sp, n, fnp);
-        } else {
-#pragma unroll
-          for (int k = 0; k < NFLD; ++k) fnp[k] = 0.0;
-        }
+        const double lam = (inv && a.flux_kind == 0) ? fmax(wsm, wsp) : 0.0;
+  #pragma unroll
+        for (int k = 0; k < NFLD; ++k) fl[k] = (fnm[k] + fnp[k] + lam * (qm[k] - qp[k])) / 2;
+      }
+  #pragma unroll
+      for (int k = 0; k < NFLD; ++k)
+        s_face[fn * FROW + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
+    } else {
+      // Rusanov / central, field by field
+      const int64_t gm = e * NP + im;
+      const double wsm = s_ws[lm];
+      const bool local = nb >= g_lo && nb < g_hi;
+      const bool bnd = nb < 0;
+      const int64_t b = bnd ? int64_t(-nb - 1) : 0;
+      const double* qsrc = bnd ? a.qb : a.q;
+      const int64_t qstr = bnd ? a.nbnd : N;
+      const int64_t qix = bnd ? b : int64_t(nb);
+      const int lp = local ? int(nb - g_lo) : 0;
+      const int ple = lp / NP;
+      double wsp;
+      Node<DIM, NSP> sp;
+      double fv[VISC ? NFLD : 1];
+      if (local) {
+        wsp = s_ws[lp];
+      } else {
+        load_q<DIM, NSP>(sp, qsrc, qstr, qix);
+        node_state<DIM, NSP, MIX, VISC>(sp, a.gas,
+                                        MIX ? (bnd ? a.tb[b] : a.temp[nb]) : 0.0);
+        wsp = sp.speed + sp.c;
         if constexpr (VISC) {
-          double gq[NFLD][DIM], gt[DIM], fv[NFLD];
+          double gq[NFLD][DIM], gt[DIM];
           load_grad<DIM, NSP>(gq, gt, a.grad, N, bnd ? e * NP + im : int64_t(nb));
           if (bnd && (wall_flags(a, b) & kBcMengaldo)) {
             // precomputed by wall_flux_kernel
@@ -538,39 +610,57 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
           } else {
             viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
           }
-#pragma unroll
-          for (int k = 0; k < NFLD; ++k) fnp[k] -= fv[k];
         }
       }
-    }
-    double fl[NFLD];
-    if constexpr (hll) {
-      Node<DIM, NSP> sm, sp;
+      double vnp = 0.0, mnp = 0.0;
+      if (!local) {
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) {
-        sm.q[k] = qm[k];
-        sp.q[k] = qp[k];
+        for (int d = 0; d < DIM; ++d) {
+          vnp += sp.v[d] * n[d];
+          mnp += sp.q[2 + d] * n[d];
+        }
       }
-      node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
-      node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
-      double fim[NFLD], fip[NFLD];
-      inviscid_flux_n<DIM, NSP>(sm, n, fim);
-      inviscid_flux_n<DIM, NSP>(sp, n, fip);
-      inviscid_numflux<DIM, NSP>(sm, sp, n, 1, fl);
-      // stored net fluxes exclude p_ref I: F_V.n = F_I.n - (Fn + p_ref n)
-#pragma unroll
-      for (int k = 0; k < NFLD; ++k) {
-        const double pn = (k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0;
-        fl[k] -= ((fim[k] - fnm[k] - pn) + (fip[k] - fnp[k] - pn)) / 2 + pn;
-      }
-    } else {
       const double lam = (inv && a.flux_kind == 0) ? fmax(wsm, wsp) : 0.0;
 #pragma unroll
-      for (int k = 0; k < NFLD; ++k) fl[k] = (fnm[k] + fnp[k] + lam * (qm[k] - qp[k])) / 2;
-    }
+      for (int k = 0; k < NFLD; ++k) {
+        double fm = 0.0, sh = 0.0;
 #pragma unroll
-    for (int k = 0; k < NFLD; ++k)
-      s_face[fn * FROW + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
+        for (int d = 0; d < DIM; ++d) {
+          const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
+          fm += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
+          sh += r0 * n[d];
+        }
+        const double qmk = a.q[k * N + gm];
+        const double qpk = local ? a.q[k * N + nb] : sp.q[k];
+        double fp = 0.0;
+        if (local) {
+#pragma unroll
+          for (int d = 0; d < DIM; ++d)
+            fp += (double(s_f[lp * NODEP + k * DIM + d]) +
+                   (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
+        } else if constexpr (MODE == kFlux) {
+          if (bnd) {
+            fp = a.fnb[k * a.nbnd + b];
+          } else {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              fp += (MIXED ? double(a.fnet32[(k * DIM + d) * N + nb])
+                           : a.fnet[(k * DIM + d) * N + nb]) * n[d];
+          }
+        } else {
+          if (inv) {
+            // component k of inviscid_flux_n (mgdg_physics.h)
+            if (k == 0) fp = mnp;
+            else if (k == 1) fp = vnp * (sp.q[1] + sp.p);
+            else if (k < 2 + DIM) fp = sp.q[0] * sp.v[k - 2] * vnp + sp.p * n[k - 2];
+            else fp = vnp * sp.q[k];
+          }
+          if constexpr (VISC) fp -= fv[k];
+        }
+        const double flk = (fm + fp + lam * (qmk - qpk)) / 2;
+        s_face[fn * FROW + le * NFLD + k] = R(-(fs * (flk - sh)));
+      }
+    }
   }
   __syncthreads();
 
@@ -1048,23 +1138,25 @@ __global__ __launch_bounds__(kBlock) void species_source_kernel(
     conc[k] = c < 0.0 ? 0.0 : c;
   }
   const double logt = log(t);
+  const double inv_t = 1.0 / t;
   double kf[3];
 #pragma unroll
   for (int r = 0; r < 3; ++r)
-    kf[r] = exp(c_mech.log_a[r] + c_mech.b[r] * logt - c_mech.ea_r[r] / t);
+    kf[r] = exp(c_mech.log_a[r] + c_mech.b[r] * logt - c_mech.ea_r[r] * inv_t);
   // reverse of R2 (CO + 0.5 O2 <=> CO2): 1/Kc = exp(dG/RT + 0.5 ln(p0/RT))
   double g_rt[MIX_NSP];
 #pragma unroll
   for (int k = 0; k < MIX_NSP; ++k) {
     double cp_r, h_rt;
-    nasa7_cp_h(k, t, cp_r, h_rt);
-    g_rt[k] = h_rt - nasa7_s_r(k, t);
+    nasa7_cp_h(k, t, inv_t, cp_r, h_rt);
+    g_rt[k] = h_rt - nasa7_s_r(k, t, logt);
   }
   const double c0 = log(c_mech.one_atm / (c_mech.gas_constant * t));
   const double log_rkc = g_rt[2] - (g_rt[3] + 0.5 * g_rt[1]) + 0.5 * c0;
-  const double r1 = kf[0] * pow(conc[0], 0.5) * pow(conc[1], 0.65);
-  const double r2 = kf[1] * (conc[3] * pow(conc[1], 0.5) - exp(log_rkc) * conc[2]);
-  const double r3 = kf[2] * pow(conc[5], 0.75) * pow(conc[1], 0.5);
+  const double sq_o2 = sqrt(conc[1]);
+  const double r1 = kf[0] * sqrt(conc[0]) * pow(conc[1], 0.65);
+  const double r2 = kf[1] * (conc[3] * sq_o2 - exp(log_rkc) * conc[2]);
+  const double r3 = kf[2] * pow(conc[5], 0.75) * sq_o2;
   const double w[MIX_NSP] = {-r1, -r1 - 0.5 * r2 - 0.5 * r3, r2, 2 * r1 - r2, r3,
                              2 * r1 - r3, 0.0};
 #pragma unroll
@@ -1395,6 +1487,22 @@ mg_status mg_load_mechanism(const double* packed, int64_t count) {
     return fail(MG_ERR_ARG, "mechanism size mismatch");
   hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(mgdg::c_mech), packed,
                                    sizeof(mgdg::MechDev), 0, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
+  mgdg::MechDev m;
+  std::memcpy(&m, packed, sizeof(m));
+  mgdg::PolyDev poly;
+  for (int r = 0; r < 2; ++r)
+    for (int k = 0; k < mgdg::MIX_NSP; ++k) {
+      const double* a = r ? m.nasa_hi[k] : m.nasa_lo[k];
+      for (int i = 0; i < 5; ++i) poly.cp[r][k][i] = a[i];
+      for (int i = 0; i < 5; ++i) poly.h[r][k][i] = a[i] / double(i + 1);
+      poly.h[r][k][5] = a[5];
+      poly.s[r][k][0] = a[0];
+      for (int i = 1; i < 5; ++i) poly.s[r][k][i] = a[i] / double(i);
+      poly.s[r][k][5] = a[6];
+    }
+  e = hipMemcpyToSymbol(HIP_SYMBOL(mgdg::c_poly), &poly, sizeof(poly), 0,
+                        hipMemcpyHostToDevice);
   if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
   g_mech_loaded = true;
   return MG_OK;

@@ -51,28 +51,44 @@ struct MechDev {
 
 extern __constant__ MechDev c_mech;
 
-__device__ __forceinline__ void nasa7_cp_h(int k, double t, double& cp_r,
+// NASA7 polynomials rearranged for Horner evaluation, derived on the host
+// from the mechanism's coefficients a0..a6 when it is loaded ([0] low range,
+// [1] high range):
+//   cp/R = a0 + a1 T + a2 T^2 + a3 T^3 + a4 T^4
+//   h/RT = a0 + a1/2 T + a2/3 T^2 + a3/4 T^3 + a4/5 T^4 + a5 / T
+//   s/R  = a0 ln T + a1 T + a2/2 T^2 + a3/3 T^3 + a4/4 T^4 + a6
+struct PolyDev {
+  double cp[2][MIX_NSP][5];
+  double h[2][MIX_NSP][6];
+  double s[2][MIX_NSP][6];
+};
+
+extern __constant__ PolyDev c_poly;
+
+__device__ __forceinline__ void nasa7_cp_h(int k, double t, double inv_t, double& cp_r,
                                            double& h_rt) {
-  const double* a = (t > c_mech.t_mid[k]) ? c_mech.nasa_hi[k] : c_mech.nasa_lo[k];
-  cp_r = a[0] + a[1] * t + a[2] * t * t + a[3] * t * t * t + a[4] * t * t * t * t;
-  h_rt = a[0] + a[1] * 0.5 * t + a[2] / 3.0 * t * t + a[3] * 0.25 * t * t * t +
-         a[4] * 0.2 * t * t * t * t + a[5] / t;
+  const int r = t > c_mech.t_mid[k] ? 1 : 0;
+  const double* c = c_poly.cp[r][k];
+  const double* h = c_poly.h[r][k];
+  cp_r = c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4])));
+  h_rt = h[0] + t * (h[1] + t * (h[2] + t * (h[3] + t * h[4]))) + h[5] * inv_t;
 }
 
-__device__ __forceinline__ double nasa7_s_r(int k, double t) {
-  const double* a = (t > c_mech.t_mid[k]) ? c_mech.nasa_hi[k] : c_mech.nasa_lo[k];
-  return a[0] * log(t) + a[1] * t + a[2] * 0.5 * t * t + a[3] / 3.0 * t * t * t +
-         a[4] * 0.25 * t * t * t * t + a[6];
+__device__ __forceinline__ double nasa7_s_r(int k, double t, double log_t) {
+  const int r = t > c_mech.t_mid[k] ? 1 : 0;
+  const double* c = c_poly.s[r][k];
+  return c[0] * log_t + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4]))) + c[5];
 }
 
 // Mixture internal energy and cv (mass basis) at T.
 __device__ __forceinline__ void mix_e_cv(double t, const double* y, double& e,
                                          double& cv) {
   double se = 0.0, sc = 0.0;
+  const double inv_t = 1.0 / t;
 #pragma unroll
   for (int k = 0; k < MIX_NSP; ++k) {
     double cp_r, h_rt;
-    nasa7_cp_h(k, t, cp_r, h_rt);
+    nasa7_cp_h(k, t, inv_t, cp_r, h_rt);
     se += c_mech.inv_wt[k] * (h_rt - 1.0) * y[k];
     sc += c_mech.inv_wt[k] * (cp_r - 1.0) * y[k];
   }
@@ -167,10 +183,11 @@ __device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
     static_assert(NSP == MIX_NSP, "mixture path is the 7-species mechanism");
     s.t = t_mix;
     double rsum = 0.0, cpsum = 0.0;
+    const double inv_t = 1.0 / t_mix;
 #pragma unroll
     for (int k = 0; k < MIX_NSP; ++k) {
       double cp_r, h_rt;
-      nasa7_cp_h(k, t_mix, cp_r, h_rt);
+      nasa7_cp_h(k, t_mix, inv_t, cp_r, h_rt);
       rsum += s.y[k] * c_mech.inv_wt[k];
       cpsum += c_mech.inv_wt[k] * cp_r * s.y[k];
       s.hk[k] = c_mech.gas_constant * c_mech.inv_wt[k] * t_mix * h_rt;

@@ -178,9 +178,9 @@ class PyrometheusMixture:
         _lib.ensure_mechanism()
         gas = _lib.MgGas(eos_kind=1, nspecies=7)
         n = cv.mass.numel()
-        _lib.check(_lib.load().mg_mixture_temperature(
-            gas, cv.dim, n, _lib.ptr(q), _lib.ptr(tseed), self.temperature_niter,
-            _lib.ptr(out), _lib.stream_handle()))
+        _lib.call("mg_mixture_temperature", gas, cv.dim, n, _lib.ptr(q),
+                  _lib.ptr(tseed), self.temperature_niter, _lib.ptr(out),
+                  _lib.stream_handle())
         return out
 
     def get_production_rates(self, cv, temperature):
@@ -198,8 +198,7 @@ class PyrometheusMixture:
         n = cv.mass.numel()
         _lib.ensure_mechanism()
         gas = _lib.MgGas(eos_kind=1, nspecies=7)
-        _lib.check(_lib.load().mg_species_source(
-            gas, cv.dim, n, _lib.ptr(q), _lib.ptr(t),
-            _lib.ptr(out[2 + cv.dim]), _lib.stream_handle()))
+        _lib.call("mg_species_source", gas, cv.dim, n, _lib.ptr(q), _lib.ptr(t),
+                  _lib.ptr(out[2 + cv.dim]), _lib.stream_handle())
         return ConservedVars(cv.dim, out)
 
