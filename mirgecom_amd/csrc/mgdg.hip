@@ -389,8 +389,11 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
     }
   }
 
-  // phase 1: net flux at the volume nodes (coalesced)
-  for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
+  // phase 1: net flux at the volume nodes (coalesced; fixed trip count)
+#pragma unroll
+  for (int it = 0; it < (EPB * NP + kBlock - 1) / kBlock; ++it) {
+    const int t = int(threadIdx.x) + it * kBlock;
+    if (t >= EPB * NP) continue;
     const int le = t / NP;
     const int i = t - le * NP;
     const int64_t e = e0 + le;
@@ -449,7 +452,12 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   // doubles per thread and cut occupancy to one wave per SIMD; with five
   // fields the arrays are cheap and let all loads issue up front.
   constexpr bool STREAM = !HLL && NFLD > 6;
-  for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
+  // fixed trip count, unrolled: the gathers of a thread's face nodes can
+  // issue together
+#pragma unroll
+  for (int it = 0; it < (EPB * NFN + kBlock - 1) / kBlock; ++it) {
+    const int t = int(threadIdx.x) + it * kBlock;
+    if (t >= EPB * NFN) continue;
     const int le = t / NFN;
     const int fn = t - le * NFN;
     const int64_t e = e0 + le;
@@ -843,8 +851,12 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   __syncthreads();
 
   // phase 2: central (interior) / exterior-value (boundary) traces x fscale;
-  // neighbours inside this workgroup are read from LDS
-  for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
+  // neighbours inside this workgroup are read from LDS (fixed trip count,
+  // unrolled, so that a thread's gathers can issue together)
+#pragma unroll
+  for (int it = 0; it < (EPB * NFN + kBlock - 1) / kBlock; ++it) {
+    const int t = int(threadIdx.x) + it * kBlock;
+    if (t >= EPB * NFN) continue;
     const int le = t / NFN;
     const int fn = t - le * NFN;
     const int64_t e = e0 + le;
@@ -954,8 +966,12 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   } else {
     if (a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, a.grad_out);
     const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
-    // phase 5: net flux F_I - F_V at every node, written coalesced
-    for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
+    // phase 5: net flux F_I - F_V at every node, written coalesced (fixed
+    // trip count)
+#pragma unroll
+    for (int it = 0; it < (EPB * NP + kBlock - 1) / kBlock; ++it) {
+      const int t = int(threadIdx.x) + it * kBlock;
+      if (t >= EPB * NP) continue;
       const int le = t / NP;
       const int i = t - le * NP;
       const int64_t e = e0 + le;

@@ -65,19 +65,36 @@ struct PolyDev {
 
 extern __constant__ PolyDev c_poly;
 
+// The temperature range is chosen per lane, the species index is
+// wave-uniform: both ranges' polynomials are evaluated with scalar-operand
+// coefficients and the result is selected per lane (indexing the table by
+// the range would turn every coefficient into a per-lane vector load).
+__device__ __forceinline__ double horner5(const double (&c)[5], double t) {
+  return c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4])));
+}
+
 __device__ __forceinline__ void nasa7_cp_h(int k, double t, double inv_t, double& cp_r,
                                            double& h_rt) {
-  const int r = t > c_mech.t_mid[k] ? 1 : 0;
-  const double* c = c_poly.cp[r][k];
-  const double* h = c_poly.h[r][k];
-  cp_r = c[0] + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4])));
-  h_rt = h[0] + t * (h[1] + t * (h[2] + t * (h[3] + t * h[4]))) + h[5] * inv_t;
+  const bool high = t > c_mech.t_mid[k];
+  const double cp_lo = horner5(c_poly.cp[0][k], t);
+  const double cp_hi = horner5(c_poly.cp[1][k], t);
+  const double* hl = c_poly.h[0][k];
+  const double* hh = c_poly.h[1][k];
+  const double h_lo = hl[0] + t * (hl[1] + t * (hl[2] + t * (hl[3] + t * hl[4]))) +
+                      hl[5] * inv_t;
+  const double h_hi = hh[0] + t * (hh[1] + t * (hh[2] + t * (hh[3] + t * hh[4]))) +
+                      hh[5] * inv_t;
+  cp_r = high ? cp_hi : cp_lo;
+  h_rt = high ? h_hi : h_lo;
 }
 
 __device__ __forceinline__ double nasa7_s_r(int k, double t, double log_t) {
-  const int r = t > c_mech.t_mid[k] ? 1 : 0;
-  const double* c = c_poly.s[r][k];
-  return c[0] * log_t + t * (c[1] + t * (c[2] + t * (c[3] + t * c[4]))) + c[5];
+  const bool high = t > c_mech.t_mid[k];
+  const double* sl = c_poly.s[0][k];
+  const double* sh = c_poly.s[1][k];
+  const double s_lo = sl[0] * log_t + t * (sl[1] + t * (sl[2] + t * (sl[3] + t * sl[4]))) + sl[5];
+  const double s_hi = sh[0] * log_t + t * (sh[1] + t * (sh[2] + t * (sh[3] + t * sh[4]))) + sh[5];
+  return high ? s_hi : s_lo;
 }
 
 // Mixture internal energy and cv (mass basis) at T.
