@@ -832,7 +832,10 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
     __syncthreads();
   }
 
-  // phase 1: conserved fields and T at the volume nodes
+  // phase 1: conserved fields and T at the volume nodes.  With one node per
+  // thread the node's state stays in registers for the flux epilogue.
+  constexpr bool KEEPQ = FLUX && EPB * NP <= kBlock;
+  double qkeep[KEEPQ ? NFLD : 1];
   for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
     const int le = t / NP;
     const int i = t - le * NP;
@@ -842,6 +845,10 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
     double qv[NFLD];
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) qv[f] = a.q[f * N + g];
+    if constexpr (KEEPQ) {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) qkeep[f] = qv[f];
+    }
     const double tn = node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g] : 0.0);
     const double* ref = s_ref + le * NS;
 #pragma unroll
@@ -978,7 +985,12 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
       if (e >= a.K) continue;
       const int64_t g = e * NP + i;
       Node<DIM, NSP> s;
-      load_q<DIM, NSP>(s, a.q, N, g);
+      if constexpr (KEEPQ) {
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) s.q[f] = qkeep[f];
+      } else {
+        load_q<DIM, NSP>(s, a.q, N, g);
+      }
       node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
       double gq[NFLD][DIM], gt[DIM], F[NFLD][DIM];
 #pragma unroll
