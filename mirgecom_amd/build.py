@@ -26,6 +26,10 @@ HIPCC_FLAGS = [
     # float code otherwise)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-munsafe-fp-atomics",
+    # MFMA accumulators in ordinary VGPRs: AGPRs would be allocated on top
+    # of the kernels' VGPR peak (unified register file) and cost a wave per
+    # SIMD in the mixed-precision kernels
+    "-mllvm", "-amdgpu-mfma-vgpr-form=1",
     "-Wno-unused-command-line-argument",
 ]
 

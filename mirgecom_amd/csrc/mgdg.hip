@@ -137,8 +137,18 @@ struct Ops {
   static constexpr int DVOL = kWaves * E::NP * DIM * RPW;
   static constexpr int LIFT = kWaves * E::NFN * RPW;
   static constexpr int TOTAL = DVOL + LIFT;
-  // bytes: TOTAL doubles + TOTAL floats
-  static constexpr int BYTES = TOTAL * 12;
+  // MFMA A-fragments (FP32, v_mfma_f32_16x16x4f32) of the same operators for
+  // the mixed-precision kernels: 16-row tiles of Dw_r and of each face's
+  // LIFT block, K in steps of 4, 64 floats per fragment in lane order
+  // (lane l holds A[l % 16][l / 16]); rows >= NP and k past the end are 0.
+  static constexpr int MT = (E::NP + 15) / 16;
+  static constexpr int KSV = (E::NP + 3) / 4;
+  static constexpr int KSF = (E::NFP + 3) / 4;
+  static constexpr int MFV = DIM * MT * KSV * 64;
+  static constexpr int MFL = E::NF * MT * KSF * 64;
+  static constexpr int MF = MFV + MFL;
+  // bytes: TOTAL doubles + TOTAL floats + MF floats
+  static constexpr int BYTES = TOTAL * 12 + MF * 4;
 };
 
 template <int DIM, int P>
@@ -168,8 +178,34 @@ __global__ __launch_bounds__(kBlock) void pack_ops_kernel(const double* dw,
   if (t < O::TOTAL) {
     ws[t] = v;
     reinterpret_cast<float*>(ws + O::TOTAL)[t] = float(v);
+  } else if (t < O::TOTAL + O::MF) {
+    constexpr int NFP = E::NFP, MT = O::MT, KSV = O::KSV, KSF = O::KSF;
+    int u = t - O::TOTAL;
+    const int lane = u % 64;
+    const int i0 = lane % 16, k0 = lane / 16;
+    double a = 0.0;
+    if (u < O::MFV) {
+      const int frag = u / 64;
+      const int st = frag % KSV, mt = (frag / KSV) % MT, r = frag / (KSV * MT);
+      const int i = mt * 16 + i0, j = st * 4 + k0;
+      if (i < NP && j < NP) a = dw[(r * NP + i) * NP + j];
+    } else {
+      const int frag = (u - O::MFV) / 64;
+      const int st = frag % KSF, mt = (frag / KSF) % MT, f = frag / (KSF * MT);
+      const int i = mt * 16 + i0, m = st * 4 + k0;
+      if (i < NP && m < NFP) a = lift[i * NFN + f * NFP + m];
+    }
+    (reinterpret_cast<float*>(ws + O::TOTAL) + O::TOTAL)[u] = float(a);
   }
 }
+
+// MFMA A-fragments of the packed operators (see Ops).
+template <int DIM, int P>
+__device__ __forceinline__ const float* mfma_ops(const double* ws) {
+  return reinterpret_cast<const float*>(ws + Ops<DIM, P>::TOTAL) + Ops<DIM, P>::TOTAL;
+}
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
 
 template <typename R, int DIM, int P>
 __device__ __forceinline__ const R* packed_ops(const double* ws) {
@@ -915,55 +951,130 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   }
   __syncthreads();
 
-  // phase 3: per (element, scalar) column: W_r = Dw_r u, L_f = LIFT_f (fs u*),
-  // grad_d = sum_f n_f[d] L_f - sum_r rx[r][d] W_r
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = threadIdx.x & 63;
-  const R* GV = ops + w * (NP * DIM * RPW);
-  const R* GL = ops + O::DVOL + w * (NFN * RPW);
-  R accw[RPW][DIM], accl[RPW][NF];
+  if constexpr (MIXED) {
+    // phase 3 on the matrix cores (v_mfma_f32_16x16x4f32): wave w owns the
+    // 16 columns [16w, 16w + 16); per 16-row tile of each output
+    // W_r = Dw_r u and L_f = LIFT_f (fs u*) the K dimension (nodes) runs in
+    // steps of 4.  Lane l supplies B[k = l / 16][col = l % 16] and holds
+    // D[4 (l / 16) + reg][l % 16] of each tile.
+    using O = Ops<DIM, P>;
+    constexpr int MT = O::MT, KSV = O::KSV, KSF = O::KSF;
+    const float* mf = mfma_ops<DIM, P>(ws);
+    const int col = w * 16 + (c & 15);
+    const int kq = c >> 4;
+    floatx4 aw[DIM][MT], al[NF][MT];
 #pragma unroll
-  for (int rr = 0; rr < RPW; ++rr) {
+    for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
-    for (int r = 0; r < DIM; ++r) accw[rr][r] = R(0);
+      for (int r = 0; r < DIM; ++r) aw[r][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int f = 0; f < NF; ++f) accl[rr][f] = R(0);
-  }
-#pragma unroll 2
-  for (int j = 0; j < NP; ++j) {
-    const R bv = s_b[j * ROW + c];
-#pragma unroll
-    for (int r = 0; r < DIM; ++r)
-#pragma unroll
-      for (int rr = 0; rr < RPW; ++rr) accw[rr][r] += GV[(j * DIM + r) * RPW + rr] * bv;
-  }
-#pragma unroll
-  for (int f = 0; f < NF; ++f) {
-#pragma unroll 2
-    for (int m = 0; m < NFP; ++m) {
-      const R bv = s_b[(NP + f * NFP + m) * ROW + c];
-#pragma unroll
-      for (int rr = 0; rr < RPW; ++rr) accl[rr][f] += GL[(f * NFP + m) * RPW + rr] * bv;
+      for (int f = 0; f < NF; ++f) al[f][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
     }
-  }
-  __syncthreads();
-  // phase 4: stage the gradients [NS][DIM][EPB*NP] (over the trace buffer)
-  if (c < EPB * NS) {
-    const int le = c / NS;
-    const int sc = c - le * NS;
-    const double* geo = s_geo + le * GEO;
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) {
+    for (int st = 0; st < KSV; ++st) {
+      const int j = st * 4 + kq;
+      const float bv = float(s_b[(j < NP ? j : NP - 1) * ROW + col]);  // A is 0 past NP
 #pragma unroll
-      for (int rr = 0; rr < RPW; ++rr) {
-        const int i = w * RPW + rr;
-        if (i >= NP) continue;
-        R vsum = R(0), fsum = R(0);
+      for (int r = 0; r < DIM; ++r)
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) vsum += R(geo[r * DIM + d]) * accw[rr][r];
+        for (int mt = 0; mt < MT; ++mt)
+          aw[r][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+              mf[((r * MT + mt) * KSV + st) * 64 + c], bv, aw[r][mt], 0, 0, 0);
+    }
+    // faces innermost: NF * MT independent accumulations per step
 #pragma unroll
-        for (int f = 0; f < NF; ++f) fsum += R(geo[DIM * DIM + f * DIM + d]) * accl[rr][f];
-        stage[(sc * DIM + d) * SP + le * NP + i] = fsum - vsum;
+    for (int st = 0; st < KSF; ++st) {
+      const int m = st * 4 + kq;
+      const int mm = m < NFP ? m : NFP - 1;
+#pragma unroll
+      for (int f = 0; f < NF; ++f) {
+        const float bv = float(s_b[(NP + f * NFP + mm) * ROW + col]);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+          al[f][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+              mf[O::MFV + ((f * MT + mt) * KSF + st) * 64 + c], bv, al[f][mt], 0, 0, 0);
+      }
+    }
+    __syncthreads();
+    // phase 4: stage the gradients [NS][DIM][EPB*NP] (over the trace buffer)
+    if (col < EPB * NS) {
+      const int le = col / NS;
+      const int sc = col - le * NS;
+      const double* geo = s_geo + le * GEO;
+      // one direction at a time: only its 7 geometric factors are live
+#pragma unroll 1
+      for (int d = 0; d < DIM; ++d) {
+        float rxd[DIM], nfd[NF];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) rxd[r] = float(geo[r * DIM + d]);
+#pragma unroll
+        for (int f = 0; f < NF; ++f) nfd[f] = float(geo[DIM * DIM + f * DIM + d]);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+          for (int reg = 0; reg < 4; ++reg) {
+            const int i = mt * 16 + 4 * kq + reg;
+            if (i >= NP) continue;
+            float vsum = 0.f, fsum = 0.f;
+#pragma unroll
+            for (int r = 0; r < DIM; ++r) vsum += rxd[r] * aw[r][mt][reg];
+#pragma unroll
+            for (int f = 0; f < NF; ++f) fsum += nfd[f] * al[f][mt][reg];
+            stage[(sc * DIM + d) * SP + le * NP + i] = R(fsum - vsum);
+          }
+      }
+    }
+  } else {
+    // phase 3: per (element, scalar) column: W_r = Dw_r u, L_f = LIFT_f (fs u*),
+    // grad_d = sum_f n_f[d] L_f - sum_r rx[r][d] W_r
+    const R* GV = ops + w * (NP * DIM * RPW);
+    const R* GL = ops + O::DVOL + w * (NFN * RPW);
+    R accw[RPW][DIM], accl[RPW][NF];
+  #pragma unroll
+    for (int rr = 0; rr < RPW; ++rr) {
+  #pragma unroll
+      for (int r = 0; r < DIM; ++r) accw[rr][r] = R(0);
+  #pragma unroll
+      for (int f = 0; f < NF; ++f) accl[rr][f] = R(0);
+    }
+  #pragma unroll 2
+    for (int j = 0; j < NP; ++j) {
+      const R bv = s_b[j * ROW + c];
+  #pragma unroll
+      for (int r = 0; r < DIM; ++r)
+  #pragma unroll
+        for (int rr = 0; rr < RPW; ++rr) accw[rr][r] += GV[(j * DIM + r) * RPW + rr] * bv;
+    }
+  #pragma unroll
+    for (int f = 0; f < NF; ++f) {
+  #pragma unroll 2
+      for (int m = 0; m < NFP; ++m) {
+        const R bv = s_b[(NP + f * NFP + m) * ROW + c];
+  #pragma unroll
+        for (int rr = 0; rr < RPW; ++rr) accl[rr][f] += GL[(f * NFP + m) * RPW + rr] * bv;
+      }
+    }
+    __syncthreads();
+    // phase 4: stage the gradients [NS][DIM][EPB*NP] (over the trace buffer)
+    if (c < EPB * NS) {
+      const int le = c / NS;
+      const int sc = c - le * NS;
+      const double* geo = s_geo + le * GEO;
+  #pragma unroll
+      for (int d = 0; d < DIM; ++d) {
+  #pragma unroll
+        for (int rr = 0; rr < RPW; ++rr) {
+          const int i = w * RPW + rr;
+          if (i >= NP) continue;
+          R vsum = R(0), fsum = R(0);
+  #pragma unroll
+          for (int r = 0; r < DIM; ++r) vsum += R(geo[r * DIM + d]) * accw[rr][r];
+  #pragma unroll
+          for (int f = 0; f < NF; ++f) fsum += R(geo[DIM * DIM + f * DIM + d]) * accl[rr][f];
+          stage[(sc * DIM + d) * SP + le * NP + i] = fsum - vsum;
+        }
       }
     }
   }
@@ -1419,7 +1530,7 @@ mg_status ensure_mech() {
 
 template <int D, int P>
 mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
-  const int n = mgdg::Ops<D, P>::TOTAL;
+  const int n = mgdg::Ops<D, P>::TOTAL + mgdg::Ops<D, P>::MF;
   auto kern = mgdg::pack_ops_kernel<D, P>;
   hipLaunchKernelGGL(kern, dim3((n + 255) / 256), dim3(256), 0, st, m->weak_diff,
                      m->lift, ws);

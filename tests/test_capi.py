@@ -106,4 +106,9 @@ def test_workspace_sizes(dim, order):
     m = _lib.MgMesh(dim=dim, order=order, np=np_, nfp=nfp, nfaces=dim + 1,
                     nelements=1)
     rpw = (np_ + 3) // 4
-    assert lib.mg_mesh_workspace_bytes(m) == 12 * 4 * rpw * (np_ * dim + (dim + 1) * nfp)
+    # FP64 + FP32 wave-packed operators, plus the FP32 MFMA A-fragments
+    # (16-row tiles, K in steps of 4, 64 lanes) of Dw_r and each face's LIFT
+    mt, ksv, ksf = (np_ + 15) // 16, (np_ + 3) // 4, (nfp + 3) // 4
+    mfma = 64 * mt * (dim * ksv + (dim + 1) * ksf)
+    assert lib.mg_mesh_workspace_bytes(m) == (
+        12 * 4 * rpw * (np_ * dim + (dim + 1) * nfp) + 4 * mfma)
