@@ -113,14 +113,47 @@ __device__ __forceinline__ void mix_e_cv(double t, const double* y, double& e,
   cv = c_mech.gas_constant * sc;
 }
 
+// e and cv of one temperature range (RANGE 0 low, 1 high) for all species.
+template <int RANGE>
+__device__ __forceinline__ void mix_e_cv_range(double t, const double* y, double& e,
+                                               double& cv) {
+  double se = 0.0, sc = 0.0;
+  const double inv_t = 1.0 / t;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    const double* h = c_poly.h[RANGE][k];
+    const double cp_r = horner5(c_poly.cp[RANGE][k], t);
+    const double h_rt = h[0] + t * (h[1] + t * (h[2] + t * (h[3] + t * h[4]))) +
+                        h[5] * inv_t;
+    se += c_mech.inv_wt[k] * (h_rt - 1.0) * y[k];
+    sc += c_mech.inv_wt[k] * (cp_r - 1.0) * y[k];
+  }
+  e = c_mech.gas_constant * t * se;
+  cv = c_mech.gas_constant * sc;
+}
+
 // Fixed-iteration Newton solve for T from mass-specific internal energy
-// (thermochemistry.py:126-167, update at :99-102).
+// (thermochemistry.py:126-167, update at :99-102).  For the standalone
+// kernel: when the whole wave is on one side of every species' T_mid (the
+// common case), only that range's polynomials are evaluated.
 __device__ __forceinline__ double mix_temperature(double e_in, double t_seed,
                                                   const double* y, int niter) {
+  double tmid_lo = c_mech.t_mid[0], tmid_hi = c_mech.t_mid[0];
+#pragma unroll
+  for (int k = 1; k < MIX_NSP; ++k) {
+    tmid_lo = fmin(tmid_lo, c_mech.t_mid[k]);
+    tmid_hi = fmax(tmid_hi, c_mech.t_mid[k]);
+  }
   double t = t_seed;
   for (int it = 0; it < niter; ++it) {
     double e, cv;
-    mix_e_cv(t, y, e, cv);
+    if (__all(t > tmid_hi)) {
+      mix_e_cv_range<1>(t, y, e, cv);
+    } else if (__all(!(t > tmid_lo))) {
+      mix_e_cv_range<0>(t, y, e, cv);
+    } else {
+      mix_e_cv(t, y, e, cv);
+    }
     t = t + (e_in - e) / cv;
   }
   return t;
