@@ -205,6 +205,21 @@ mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double
 mg_status mg_rk_axpy(int64_t n, const double* y, const double* k, double a,
                      double* out, double* acc, double w, void* stream);
 
+/* Bound-preserving limiter (mirgecom/limiter.py:39-125,
+ * bound_preserving_limiter): per element, u <- avg + theta (u - avg), theta the
+ * largest factor in [0, 1] that keeps the element's nodal values inside
+ * [mmin, mmax] (the upper bound only with MG_LIMIT_MAX).  field and out are
+ * [nfields][nelements][np] (out may equal field); weights[np] are the row sums
+ * of the reference mass matrix, which give the cell average on an affine
+ * simplex.  MG_LIMIT_MODIFY_AVERAGE clips the average into the bounds first
+ * (the reference's modify_average).  np is a simplex node count: 3, 4, 6, 10,
+ * 15, 20, 21, 35 or 56. */
+#define MG_LIMIT_MAX 1
+#define MG_LIMIT_MODIFY_AVERAGE 2
+mg_status mg_bound_limiter(int32_t np, int64_t nelements, int32_t nfields,
+                           const double* weights, const double* field, double* out,
+                           double mmin, double mmax, int32_t flags, void* stream);
+
 /* Upload the uiuc_7sp mechanism constants (NASA7 tables, molecular weights,
  * gas constant, standard pressure, SI Arrhenius parameters; 130 doubles in
  * the order of mirgecom_amd.mechanisms.packed_mechanism) to the device.

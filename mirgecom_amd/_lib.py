@@ -85,6 +85,8 @@ SIGNATURES = {
     "mg_local_timestep": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  _P, _P, _P, _I32, _P, _P]),
     "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
+    "mg_bound_limiter": (_I32, [_I32, _I64, _I32, _P, _P, _P, _D, _D, _I32,
+                                _P]),
     "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),
     "mg_mesh_workspace_bytes": (_I64, [ctypes.POINTER(MgMesh)]),
     "mg_mesh_prepare": (_I32, [ctypes.POINTER(MgMesh), _P, _P]),

@@ -1338,6 +1338,65 @@ __global__ __launch_bounds__(kBlock) void rk_axpy_kernel(
   if (acc) acc[g] = acc[g] + w * kv;
 }
 
+
+// Bound-preserving limiter (mirgecom/limiter.py:39-125; Zhang & Shu 2011,
+// eq. 2.9): u <- avg + theta (u - avg) per element, theta the largest factor
+// in [0, 1] that brings the element's extrema inside [mmin, mmax].  The cell
+// average is the mass-weighted mean: on an affine simplex the Jacobian is
+// constant and cancels, so avg = sum_j w_j u_j / sum_j w_j with w the row
+// sums of the reference mass matrix.  One thread per element; a block stages
+// kLimElems consecutive elements of one field through LDS so that loads and
+// stores stay contiguous (the element rows are np doubles long).
+constexpr int kLimElems = 128;
+constexpr int kLimFlagMax = 1;
+constexpr int kLimFlagModifyAvg = 2;
+
+template <int NP>
+__global__ __launch_bounds__(kLimElems) void bound_limiter_kernel(
+    int64_t nelements, const double* __restrict__ weights,
+    const double* __restrict__ field, double* out, double mmin, double mmax,
+    int flags) {
+  constexpr int S = NP | 1;  // odd LDS row stride
+  __shared__ double tile[kLimElems * S];
+  const int64_t e0 = int64_t(blockIdx.x) * kLimElems;
+  const int64_t ne = nelements - e0 < kLimElems ? nelements - e0 : kLimElems;
+  const int64_t base = (int64_t(blockIdx.y) * nelements + e0) * NP;
+  const int span = int(ne) * NP;
+  const int t = threadIdx.x;
+  for (int i = t; i < span; i += kLimElems) {
+    const int e = i / NP;
+    tile[e * S + (i - e * NP)] = field[base + i];
+  }
+  __syncthreads();
+  if (t < ne) {
+    double* u = tile + t * S;
+    double num = 0.0, den = 0.0, umin = u[0], umax = u[0];
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      num += weights[j] * u[j];
+      den += weights[j];
+      umin = fmin(umin, u[j]);
+      umax = fmax(umax, u[j]);
+    }
+    double avg = num / fabs(den);
+    const bool modify = (flags & kLimFlagModifyAvg) != 0;
+    if (modify && !(avg > mmin)) avg = mmin;
+    double theta = 1.0;
+    if (umin < mmin) theta = fmin(theta, fabs((mmin - avg) / (umin - avg + 1e-13)));
+    if (flags & kLimFlagMax) {
+      if (modify && avg > mmax) avg = mmax;
+      if (umax > mmax) theta = fmin(theta, fabs((mmax - avg) / (umax - avg + 1e-13)));
+    }
+#pragma unroll
+    for (int j = 0; j < NP; ++j) u[j] = theta * (u[j] - avg) + avg;
+  }
+  __syncthreads();
+  for (int i = t; i < span; i += kLimElems) {
+    const int e = i / NP;
+    out[base + i] = tile[e * S + (i - e * NP)];
+  }
+}
+
 // }}}
 
 }  // namespace mgdg
@@ -1931,6 +1990,35 @@ mg_status mg_rk_axpy(int64_t n, const double* y, const double* k, double a,
   hipLaunchKernelGGL(mgdg::rk_axpy_kernel, dim3(nblk), dim3(mgdg::kBlock), 0,
                      static_cast<hipStream_t>(stream), n, y, k, a, out, acc, w);
   return check_launch("rk_axpy_kernel");
+}
+
+
+mg_status mg_bound_limiter(int32_t np, int64_t nelements, int32_t nfields,
+                           const double* weights, const double* field, double* out,
+                           double mmin, double mmax, int32_t flags, void* stream) {
+  if (!weights || !field || !out || nelements < 0 || nfields < 0)
+    return fail(MG_ERR_ARG, "bad arguments");
+  if (flags & ~(mgdg::kLimFlagMax | mgdg::kLimFlagModifyAvg))
+    return fail(MG_ERR_ARG, "unknown limiter flags");
+  if (nfields > 65535) return fail(MG_ERR_ARG, "nfields > 65535");
+  if (nelements == 0 || nfields == 0) return MG_OK;
+  const int64_t nblk64 = (nelements + mgdg::kLimElems - 1) / mgdg::kLimElems;
+  if (nblk64 > 0x7fffffff) return fail(MG_ERR_ARG, "too many elements");
+  const dim3 grid{unsigned(nblk64), unsigned(nfields), 1u};
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define MG_LIM(N)                                                                   \
+  case N:                                                                           \
+    hipLaunchKernelGGL(mgdg::bound_limiter_kernel<N>, grid, dim3(mgdg::kLimElems), 0, \
+                       st, nelements, weights, field, out, mmin, mmax, flags);      \
+    break;
+  switch (np) {
+    MG_LIM(3) MG_LIM(4) MG_LIM(6) MG_LIM(10) MG_LIM(15) MG_LIM(20) MG_LIM(21)
+    MG_LIM(35) MG_LIM(56)
+    default:
+      return fail(MG_ERR_UNSUPPORTED, "np must be a simplex node count (3..56)");
+  }
+#undef MG_LIM
+  return check_launch("bound_limiter_kernel");
 }
 
 }  // extern "C"

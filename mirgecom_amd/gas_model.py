@@ -14,6 +14,7 @@ formulas.
 
 from __future__ import annotations
 
+import numpy as np
 import torch
 
 from mirgecom_amd import _lib
@@ -166,19 +167,33 @@ def make_fluid_state(cv, gas_model, temperature_seed=None, smoothness_mu=None,
                      limiter_dd=None):
     """mirgecom/gas_model.py:302-431 (GasModel branch).
 
-    Artificial-viscosity smoothness fields, limiters and porous media are
-    outside this build and raise ``NotImplementedError`` when given."""
+    ``limiter_func`` is called as in the reference (gas_model.py:373-383)
+    with ``cv``, ``temperature_seed``, ``gas_model``, ``dd`` (and
+    ``entropy_min`` when given) and returns either the limited CV or an
+    object array ``[cv, pressure, temperature]``; the state is then built on
+    what it returned.  Artificial-viscosity smoothness fields and porous media
+    are outside this build and raise ``NotImplementedError`` when given."""
     if not isinstance(gas_model, GasModel):
         raise NotImplementedError("only GasModel (no porous media)")
-    if limiter_func is not None:
-        raise NotImplementedError("limiter_func is not supported")
     if any(x is not None for x in (smoothness_mu, smoothness_kappa,
                                    smoothness_d, smoothness_beta)):
         raise NotImplementedError("artificial-viscosity fields not supported")
-    temperature = None
-    if getattr(gas_model.eos, "is_mixture", False):
+    pressure = temperature = None
+    if limiter_func is not None:
+        kw = dict(cv=cv, temperature_seed=temperature_seed,
+                  gas_model=gas_model, dd=limiter_dd)
+        if entropy_min is not None:
+            kw["entropy_min"] = entropy_min
+        rv = limiter_func(**kw)
+        if isinstance(rv, np.ndarray):
+            cv, pressure, temperature = rv
+        else:
+            cv = rv
+    if temperature is None and getattr(gas_model.eos, "is_mixture", False):
         temperature = gas_model.eos.temperature(cv, temperature_seed)
     dv = _LazyDV(cv, gas_model.eos, temperature)
+    if pressure is not None:
+        dv._cache["p"] = pressure
     if gas_model.transport is not None:
         return ViscousFluidState(cv, dv, gas_model)
     return FluidState(cv, dv, gas_model)

@@ -112,3 +112,30 @@ def test_workspace_sizes(dim, order):
     mfma = 64 * mt * (dim * ksv + (dim + 1) * ksf)
     assert lib.mg_mesh_workspace_bytes(m) == (
         12 * 4 * rpw * (np_ * dim + (dim + 1) * nfp) + 4 * mfma)
+
+
+def test_limiter_validation():
+    lib = _lib.load()
+    one = ctypes.c_void_p(1)
+    assert lib.mg_bound_limiter(20, 10, 1, None, one, one, 0.0, 1.0, 1,
+                                None) == 1
+    assert lib.mg_bound_limiter(20, 10, 1, one, one, one, 0.0, 1.0, 4,
+                                None) == 1
+    assert b"flags" in lib.mg_last_error()
+    assert lib.mg_bound_limiter(7, 10, 1, one, one, one, 0.0, 1.0, 1,
+                                None) == 2
+    assert b"np" in lib.mg_last_error()
+    # nothing to limit: accepted without a launch
+    assert lib.mg_bound_limiter(20, 0, 1, one, one, one, 0.0, 1.0, 1,
+                                None) == 0
+
+
+def test_limiter_rejects_host_tensors():
+    torch = pytest.importorskip("torch")
+    from mirgecom_amd.discretization import (
+        create_discretization_collection, generate_box_mesh)
+    from mirgecom_amd.limiter import bound_preserving_limiter
+    dcoll = create_discretization_collection(
+        generate_box_mesh(2, 0.0, 1.0, (2, 2)), 2)
+    with pytest.raises(TypeError):
+        bound_preserving_limiter(dcoll, torch.zeros(dcoll.nelements, 6))
