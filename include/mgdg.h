@@ -65,13 +65,19 @@ mg_status mg_mesh_prepare(mg_mesh* mesh, double* workspace, void* stream);
  * mirgecom/eos.py, mirgecom/transport.py). */
 typedef struct mg_gas {
   int32_t eos_kind;        /* 0 IdealSingleGas, 1 uiuc_7sp mixture          */
-  int32_t transport_kind;  /* 0 none, 1 SimpleTransport, 2 PowerLawTransport*/
+  int32_t transport_kind;  /* 0 none, 1 SimpleTransport, 2 PowerLawTransport,
+                              3 MixtureAveragedTransport (uiuc_7sp mixture)  */
   int32_t nspecies;
-  int32_t has_lewis;       /* power law: species_diffusivity holds Lewis nos */
+  int32_t has_lewis;       /* power law / mixture-averaged: species_diffusivity
+                              holds Lewis numbers                            */
   double gamma, gas_const; /* IdealSingleGas (eos.py:329)                   */
   double mu, mu_b, kappa;  /* SimpleTransport (transport.py:143-202)        */
   double pl_alpha, pl_beta, pl_sigma, pl_n; /* PowerLaw (transport.py:205-321)*/
   double species_diffusivity[8];
+  /* MixtureAveragedTransport (transport.py:324-502): bulk-viscosity ratio
+   * alpha, scaling factor, single-species threshold epsilon and the
+   * diffusivity used there; needs mg_load_transport.                       */
+  double ma_alpha, ma_factor, ma_epsilon, ma_singular_diffusivity;
 } mg_gas;
 
 /* Boundary data of Mengaldo-type boundaries (AdiabaticSlipBoundary,
@@ -227,6 +233,24 @@ mg_status mg_bound_limiter(int32_t np, int64_t nelements, int32_t nfields,
  * pyrometheus code generation of the mechanism class
  * (mirgecom/thermochemistry.py:186-219). */
 mg_status mg_load_mechanism(const double* packed, int64_t count);
+
+/* Upload the pure-species transport fits of the uiuc_7sp mixture (308
+ * doubles in the order of mirgecom_amd.transport_fits.packed_transport:
+ * sqrt(mu_k/sqrt(T)), lambda_k/sqrt(T), p D_kj/T^1.5 as degree-4 polynomials
+ * in ln T, and the Wilke factors).  Needed once per process before any call
+ * with transport_kind 3; it replaces the transport part of the pyrometheus
+ * mechanism class that MixtureAveragedTransport evaluates
+ * (mirgecom/transport.py:391-502). */
+mg_status mg_load_transport(const double* packed, int64_t count);
+
+/* Transport variables at n nodes (mirgecom/transport.py:66-83,
+ * TransportModel.transport_vars): mu, mu_B, kappa [n] and the species
+ * diffusivities [nspecies][n] from the conserved state q [F][n] (and the
+ * temperature [n] for mixtures), evaluated exactly as inside the
+ * Navier-Stokes kernels.  Ideal gas without species, or the uiuc_7sp mixture. */
+mg_status mg_transport_vars(const mg_gas* gas, int32_t dim, int64_t n,
+                            const double* q, const double* temperature, double* mu,
+                            double* mu_b, double* kappa, double* diff, void* stream);
 
 /* Message of the last failed call on this thread. */
 const char* mg_last_error(void);

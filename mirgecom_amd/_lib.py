@@ -44,6 +44,9 @@ class MgGas(ctypes.Structure):
         ("pl_alpha", ctypes.c_double), ("pl_beta", ctypes.c_double),
         ("pl_sigma", ctypes.c_double), ("pl_n", ctypes.c_double),
         ("species_diffusivity", ctypes.c_double * 8),
+        ("ma_alpha", ctypes.c_double), ("ma_factor", ctypes.c_double),
+        ("ma_epsilon", ctypes.c_double),
+        ("ma_singular_diffusivity", ctypes.c_double),
     ]
 
 
@@ -88,6 +91,9 @@ SIGNATURES = {
     "mg_bound_limiter": (_I32, [_I32, _I64, _I32, _P, _P, _P, _D, _D, _I32,
                                 _P]),
     "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),
+    "mg_load_transport": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),
+    "mg_transport_vars": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P, _P,
+                                 _P, _P, _P, _P]),
     "mg_mesh_workspace_bytes": (_I64, [ctypes.POINTER(MgMesh)]),
     "mg_mesh_prepare": (_I32, [ctypes.POINTER(MgMesh), _P, _P]),
     "mg_last_error": (ctypes.c_char_p, []),
@@ -125,6 +131,7 @@ def check(status):
 
 
 _mech_loaded = False
+_tran_loaded = False
 
 
 def ensure_mechanism():
@@ -136,6 +143,18 @@ def ensure_mechanism():
     check(load().mg_load_mechanism(
         arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), arr.size))
     _mech_loaded = True
+
+
+def ensure_transport():
+    """Upload the uiuc_7sp transport fits (once per process)."""
+    global _tran_loaded
+    if _tran_loaded:
+        return
+    from mirgecom_amd.transport_fits import packed_transport
+    arr = np.ascontiguousarray(packed_transport())
+    check(load().mg_load_transport(
+        arr.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), arr.size))
+    _tran_loaded = True
 
 
 # Optional per-entry-point timing: when ``timing`` is a dict, ``call`` records

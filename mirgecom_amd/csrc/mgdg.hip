@@ -26,6 +26,7 @@ namespace mgdg {
 
 __constant__ MechDev c_mech;
 __constant__ PolyDev c_poly;
+__constant__ TranDev c_tran;
 
 constexpr int kBlock = 256;
 constexpr int kLdsBudget = 64 * 1024;
@@ -257,7 +258,7 @@ constexpr int odd_stride(int n) { return n | 1; }
 // diagonal does not change the DG divergence (the weak divergence of a
 // constant flux cancels its lift exactly), but it removes the large
 // low-Mach background from the values that are rounded to FP32.
-template <int DIM, int NSP, bool MIX>
+template <int DIM, int NSP, int MIX>
 __device__ __forceinline__ double reference_pressure(const KArgs& a) {
   Node<DIM, NSP> s;
   load_q<DIM, NSP>(s, a.q, a.nnodes, 0);
@@ -277,7 +278,7 @@ __device__ __forceinline__ int wall_flags(const KArgs& a, int64_t b) {
 // Effective exterior viscous normal flux 2 F_V(bc).n - F_V(q-, grad-).n: the
 // face flux averages both sides, which then gives F_V(bc).n.  gq/gt hold the
 // interior gradients on entry and the boundary-condition gradients on exit.
-template <int DIM, int NSP, bool MIX>
+template <int DIM, int NSP, int MIX>
 __device__ void wall_viscous_flux_n(const KArgs& a, int64_t b, int64_t gm, int fl,
                                     const double* n, double (*gq)[DIM], double* gt,
                                     double* fv) {
@@ -359,7 +360,7 @@ __device__ void wall_viscous_flux_n(const KArgs& a, int64_t b, int64_t gm, int f
 
 constexpr int kEuler = 0, kGrad = 1, kFlux = 2;
 
-template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
+template <int DIM, int P, int NSP, int MIX, int MODE, typename R>
 struct RhsCfg {
   using E = Elem<DIM, P>;
   static constexpr int NFLD = 2 + DIM + NSP;
@@ -385,7 +386,7 @@ struct RhsCfg {
 
 // HLL selects the HLL face flux (compiled separately: its two full face
 // states cost registers that the Rusanov / central path does not need).
-template <int DIM, int P, int NSP, bool MIX, int MODE, typename R, bool HLL>
+template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
 __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
                                                      const double* __restrict__ ws,
                                                      double* __restrict__ out) {
@@ -778,7 +779,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
 // runs in FP32, the physics in FP64 and the nodal fluxes are stored in FP32
 // with p_ref removed from the momentum-flux diagonal.
 
-template <int DIM, int P, int NSP, bool MIX, typename R>
+template <int DIM, int P, int NSP, int MIX, typename R>
 struct GradCfg {
   using E = Elem<DIM, P>;
   static constexpr int NS = 3 + DIM + NSP;  // conserved fields + T
@@ -797,10 +798,10 @@ struct GradCfg {
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
 
-template <int DIM, int NSP, bool MIX>
+template <int DIM, int NSP, int MIX>
 __device__ __forceinline__ double node_temperature(const double* qv,
                                                    const GasDev& g, double tmix) {
-  if constexpr (MIX) {
+  if constexpr (MIX != 0) {
     return tmix;
   } else {
     double mm = 0.0;
@@ -812,7 +813,7 @@ __device__ __forceinline__ double node_temperature(const double* qv,
   }
 }
 
-template <int DIM, int P, int NSP, bool MIX, bool FLUX, typename R>
+template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
 __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
                                                       const double* __restrict__ ws,
                                                       double* __restrict__ out) {
@@ -1196,7 +1197,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 // for the mixed path) is written to fnb_out.  Otherwise the gradients come
 // from the full gradient array and the effective exterior viscous flux
 // 2 F_V(bc).n - F_V(q-).n is written to wwork for rhs_kernel<kGrad>.
-template <int DIM, int NSP, bool MIX, bool FUSED>
+template <int DIM, int NSP, int MIX, bool FUSED>
 __global__ __launch_bounds__(kBlock) void wall_flux_kernel(KArgs a, int shift_pref) {
   constexpr int NFLD = 2 + DIM + NSP;
   const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
@@ -1306,7 +1307,7 @@ __global__ __launch_bounds__(kBlock) void species_source_kernel(
 //   dt = h_e / (|v| + c)                                   inviscid
 //   dt = h_e / (|v| + c + (mu / rho + max_k d_k) / h_e)     viscous
 // with h_e the characteristic length of the node's element.
-template <int DIM, int NSP, bool MIX>
+template <int DIM, int NSP, int MIX>
 __global__ __launch_bounds__(kBlock) void local_timestep_kernel(
     int64_t n, int np, const double* __restrict__ q, const double* __restrict__ temp,
     const double* __restrict__ hlen, GasDev gas, int viscous,
@@ -1326,6 +1327,26 @@ __global__ __launch_bounds__(kBlock) void local_timestep_kernel(
     rate += (s.mu * s.inv_rho + dmax) / h;
   }
   dt[g] = h / rate;
+}
+
+// Transport variables of every node, from the same node_state /
+// node_transport code the Navier-Stokes kernels run.
+template <int DIM, int NSP, int MIX>
+__global__ __launch_bounds__(kBlock) void transport_vars_kernel(
+    GasDev gas, int64_t n, const double* __restrict__ q,
+    const double* __restrict__ temp, double* mu, double* mu_b, double* kappa,
+    double* diff) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  Node<DIM, NSP> s;
+#pragma unroll
+  for (int f = 0; f < Node<DIM, NSP>::NFLD; ++f) s.q[f] = q[f * n + g];
+  node_state<DIM, NSP, MIX, true>(s, gas, MIX ? temp[g] : 0.0);
+  mu[g] = s.mu;
+  mu_b[g] = s.mu_b;
+  kappa[g] = s.kappa;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) diff[k * n + g] = s.d[k];
 }
 
 __global__ __launch_bounds__(kBlock) void rk_axpy_kernel(
@@ -1434,6 +1455,10 @@ mgdg::GasDev to_dev(const mg_gas* g) {
   d.pl_n = g->pl_n;
   d.has_lewis = g->has_lewis;
   for (int k = 0; k < 8; ++k) d.diff[k] = g->species_diffusivity[k];
+  d.ma_alpha = g->ma_alpha;
+  d.ma_factor = g->ma_factor;
+  d.ma_epsilon = g->ma_epsilon;
+  d.ma_singular = g->ma_singular_diffusivity;
   return d;
 }
 
@@ -1483,7 +1508,7 @@ mg_status set_lds(Kern kern, size_t lds) {
   return MG_OK;
 }
 
-template <int DIM, int P, int NSP, bool MIX, int MODE, typename R, bool HLL>
+template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
 mg_status launch_rhs_kernel(const mgdg::KArgs& a, hipStream_t st) {
   using C = mgdg::RhsCfg<DIM, P, NSP, MIX, MODE, R>;
   auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, MODE, R, HLL>;
@@ -1499,7 +1524,7 @@ mg_status launch_rhs_kernel(const mgdg::KArgs& a, hipStream_t st) {
   return check_launch("rhs_kernel");
 }
 
-template <int DIM, int P, int NSP, bool MIX, int MODE, typename R>
+template <int DIM, int P, int NSP, int MIX, int MODE, typename R>
 mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
   const bool inv = MODE == mgdg::kEuler || a.inviscid_on;
   if (inv && a.flux_kind == MG_FLUX_HLL)
@@ -1507,7 +1532,7 @@ mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
   return launch_rhs_kernel<DIM, P, NSP, MIX, MODE, R, false>(a, st);
 }
 
-template <int DIM, int P, int NSP, bool MIX, bool FLUX, typename R>
+template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
 mg_status launch_grad(const mgdg::KArgs& a, hipStream_t st) {
   using C = mgdg::GradCfg<DIM, P, NSP, MIX, R>;
   auto kern = mgdg::grad_kernel<DIM, P, NSP, MIX, FLUX, R>;
@@ -1524,26 +1549,34 @@ mg_status launch_grad(const mgdg::KArgs& a, hipStream_t st) {
 }
 
 // compiled (species count, EOS) combinations: (0, ideal), (7, ideal), (7, mixture)
+// mix: 0 ideal gas, 1 mixture, 2 mixture with mixture-averaged transport
+// (a separate instantiation only where the kernel evaluates transport).
 template <int DIM, int P, int MODE, typename R>
-mg_status rhs_species(const mgdg::KArgs& a, int nsp, bool mix, hipStream_t st) {
-  if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, false, MODE, R>(a, st);
-  if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, false, MODE, R>(a, st);
-  if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, true, MODE, R>(a, st);
+mg_status rhs_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
+  if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, 0, MODE, R>(a, st);
+  if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, 0, MODE, R>(a, st);
+  if constexpr (MODE == mgdg::kGrad) {
+    if (nsp == 7 && mix == 2) return launch_rhs<DIM, P, 7, 2, MODE, R>(a, st);
+  }
+  if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, 1, MODE, R>(a, st);
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
 template <int DIM, int P, bool FLUX, typename R>
-mg_status grad_species(const mgdg::KArgs& a, int nsp, bool mix, hipStream_t st) {
-  if (nsp == 0 && !mix) return launch_grad<DIM, P, 0, false, FLUX, R>(a, st);
-  if (nsp == 7 && !mix) return launch_grad<DIM, P, 7, false, FLUX, R>(a, st);
-  if (nsp == 7 && mix) return launch_grad<DIM, P, 7, true, FLUX, R>(a, st);
+mg_status grad_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
+  if (nsp == 0 && !mix) return launch_grad<DIM, P, 0, 0, FLUX, R>(a, st);
+  if (nsp == 7 && !mix) return launch_grad<DIM, P, 7, 0, FLUX, R>(a, st);
+  if constexpr (FLUX) {
+    if (nsp == 7 && mix == 2) return launch_grad<DIM, P, 7, 2, FLUX, R>(a, st);
+  }
+  if (nsp == 7 && mix) return launch_grad<DIM, P, 7, 1, FLUX, R>(a, st);
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
 template <int DIM, int P>
-mg_status dispatch_p(const mgdg::KArgs& a, int nsp, bool mix, int what,
+mg_status dispatch_p(const mgdg::KArgs& a, int nsp, int mix, int what,
                      hipStream_t st) {
   switch (what) {
     case 0: return rhs_species<DIM, P, mgdg::kEuler, double>(a, nsp, mix, st);
@@ -1559,7 +1592,7 @@ mg_status dispatch_p(const mgdg::KArgs& a, int nsp, bool mix, int what,
 
 // what: 0 Euler RHS, 1 NS RHS from gradients, 2 NS RHS from nodal fluxes,
 // 3 gradients, 4 gradients + nodal fluxes, 5/6 the mixed-precision 4/2
-mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, bool mix,
+mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, int mix,
                    int what, hipStream_t st) {
 #define MG_CASE(D, P) \
   if (m->dim == D && m->order == P) return dispatch_p<D, P>(a, nsp, mix, what, st);
@@ -1569,8 +1602,22 @@ mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, bool mix,
   return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }
 
+bool g_tran_loaded = false;
+
+// Kernel variant of a gas model: 0 ideal gas, 1 uiuc_7sp mixture, 2 mixture
+// with mixture-averaged transport.
+int gas_variant(const mg_gas* g) {
+  return g->eos_kind != 1 ? 0 : (g->transport_kind == 3 ? 2 : 1);
+}
+
 mg_status check_gas(const mg_gas* g, bool need_transport) {
   if (!g) return fail(MG_ERR_ARG, "gas is NULL");
+  if (g->transport_kind < 0 || g->transport_kind > 3)
+    return fail(MG_ERR_ARG, "transport_kind");
+  if (g->transport_kind == 3 && g->eos_kind != 1)
+    return fail(MG_ERR_ARG, "mixture-averaged transport needs the mixture EOS");
+  if (g->transport_kind == 3 && !g_tran_loaded)
+    return fail(MG_ERR_ARG, "transport fits not loaded (mg_load_transport)");
   if (g->eos_kind == 1 && g->nspecies != 7)
     return fail(MG_ERR_ARG, "the mixture EOS needs the 7 uiuc_7sp species");
   if (g->eos_kind != 0 && g->eos_kind != 1) return fail(MG_ERR_ARG, "eos_kind");
@@ -1618,7 +1665,7 @@ mg_status set_wall(mgdg::KArgs& a, const mg_mesh* m, const mg_wall* w, bool mix)
   return MG_OK;
 }
 
-template <int DIM, int NSP, bool MIX, bool FUSED>
+template <int DIM, int NSP, int MIX, bool FUSED>
 mg_status launch_wall(const mgdg::KArgs& a, int shift, hipStream_t st) {
   const unsigned nblk = unsigned((a.nbnd + mgdg::kBlock - 1) / mgdg::kBlock);
   hipLaunchKernelGGL((mgdg::wall_flux_kernel<DIM, NSP, MIX, FUSED>), dim3(nblk),
@@ -1629,23 +1676,25 @@ mg_status launch_wall(const mgdg::KArgs& a, int shift, hipStream_t st) {
 // Mengaldo boundary nodes: after grad_kernel<FLUX> (fused) or before
 // rhs_kernel<kGrad>; nothing to do without wall data.
 template <bool FUSED>
-mg_status wall_pass(const mgdg::KArgs& a, int dim, int nsp, bool mix, int shift,
+mg_status wall_pass(const mgdg::KArgs& a, int dim, int nsp, int mix, int shift,
                     hipStream_t st) {
   if (!a.wflag) return MG_OK;
   if (dim == 2) {
-    if (nsp == 0 && !mix) return launch_wall<2, 0, false, FUSED>(a, shift, st);
-    if (nsp == 7 && !mix) return launch_wall<2, 7, false, FUSED>(a, shift, st);
-    if (nsp == 7 && mix) return launch_wall<2, 7, true, FUSED>(a, shift, st);
+    if (nsp == 0 && !mix) return launch_wall<2, 0, 0, FUSED>(a, shift, st);
+    if (nsp == 7 && !mix) return launch_wall<2, 7, 0, FUSED>(a, shift, st);
+    if (nsp == 7 && mix == 1) return launch_wall<2, 7, 1, FUSED>(a, shift, st);
+    if (nsp == 7 && mix == 2) return launch_wall<2, 7, 2, FUSED>(a, shift, st);
   } else {
-    if (nsp == 0 && !mix) return launch_wall<3, 0, false, FUSED>(a, shift, st);
-    if (nsp == 7 && !mix) return launch_wall<3, 7, false, FUSED>(a, shift, st);
-    if (nsp == 7 && mix) return launch_wall<3, 7, true, FUSED>(a, shift, st);
+    if (nsp == 0 && !mix) return launch_wall<3, 0, 0, FUSED>(a, shift, st);
+    if (nsp == 7 && !mix) return launch_wall<3, 7, 0, FUSED>(a, shift, st);
+    if (nsp == 7 && mix == 1) return launch_wall<3, 7, 1, FUSED>(a, shift, st);
+    if (nsp == 7 && mix == 2) return launch_wall<3, 7, 2, FUSED>(a, shift, st);
   }
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
-template <int DIM, int NSP, bool MIX>
+template <int DIM, int NSP, int MIX>
 mg_status launch_timestep(const mg_mesh* m, const mgdg::GasDev& g, const double* q,
                           const double* t, const double* h, int viscous, double* dt,
                           hipStream_t st) {
@@ -1661,13 +1710,15 @@ mg_status timestep_species(const mg_mesh* m, const mg_gas* gas, const double* q,
                            const double* t, const double* h, int viscous, double* dt,
                            hipStream_t st) {
   const mgdg::GasDev g = to_dev(gas);
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (gas->nspecies == 0 && !mix)
-    return launch_timestep<DIM, 0, false>(m, g, q, t, h, viscous, dt, st);
+    return launch_timestep<DIM, 0, 0>(m, g, q, t, h, viscous, dt, st);
   if (gas->nspecies == 7 && !mix)
-    return launch_timestep<DIM, 7, false>(m, g, q, t, h, viscous, dt, st);
-  if (gas->nspecies == 7 && mix)
-    return launch_timestep<DIM, 7, true>(m, g, q, t, h, viscous, dt, st);
+    return launch_timestep<DIM, 7, 0>(m, g, q, t, h, viscous, dt, st);
+  if (gas->nspecies == 7 && mix == 1)
+    return launch_timestep<DIM, 7, 1>(m, g, q, t, h, viscous, dt, st);
+  if (gas->nspecies == 7 && mix == 2)
+    return launch_timestep<DIM, 7, 2>(m, g, q, t, h, viscous, dt, st);
   return fail(MG_ERR_UNSUPPORTED,
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
@@ -1676,7 +1727,7 @@ extern "C" {
 
 const char* mg_last_error(void) { return g_err.c_str(); }
 
-int32_t mg_abi_version(void) { return 2; }
+int32_t mg_abi_version(void) { return 3; }
 
 /* Upload the uiuc_7sp mechanism constants (host array of 7*7*2 + 7 + 7 + 7
  * + 2 + 9 doubles, packed as in mgdg::MechDev). */
@@ -1703,6 +1754,16 @@ mg_status mg_load_mechanism(const double* packed, int64_t count) {
                         hipMemcpyHostToDevice);
   if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
   g_mech_loaded = true;
+  return MG_OK;
+}
+
+mg_status mg_load_transport(const double* packed, int64_t count) {
+  if (!packed || count != int64_t(sizeof(mgdg::TranDev) / sizeof(double)))
+    return fail(MG_ERR_ARG, "transport size mismatch");
+  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(mgdg::c_tran), packed,
+                                   sizeof(mgdg::TranDev), 0, hipMemcpyHostToDevice);
+  if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
+  g_tran_loaded = true;
   return MG_OK;
 }
 
@@ -1738,7 +1799,7 @@ mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, false))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (mix && (s = ensure_mech())) return s;
   if (!q || !rhs) return fail(MG_ERR_ARG, "q/rhs NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
@@ -1763,7 +1824,7 @@ mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas, const double* q
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, false))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (!q || !grad) return fail(MG_ERR_ARG, "q/grad NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
   if (mesh->nbnd_nodes > 0 && (!q_bnd || (mix && !t_bnd)))
@@ -1785,7 +1846,7 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (mix && (s = ensure_mech())) return s;
   if (!q || !grad || !rhs) return fail(MG_ERR_ARG, "q/grad/rhs NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
@@ -1814,7 +1875,7 @@ mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (mix && (s = ensure_mech())) return s;
   if (!q || !fnet) return fail(MG_ERR_ARG, "q/fnet NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
@@ -1843,7 +1904,7 @@ mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (mix && (s = ensure_mech())) return s;
   if (!q || !fnet || !rhs) return fail(MG_ERR_ARG, "q/fnet/rhs NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
@@ -1871,7 +1932,7 @@ mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas, const double*
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (mix && (s = ensure_mech())) return s;
   if (!q || !fnet) return fail(MG_ERR_ARG, "q/fnet NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
@@ -1899,7 +1960,7 @@ mg_status mg_ns_div_mixed(const mg_mesh* mesh, const mg_gas* gas, const double* 
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (mix && (s = ensure_mech())) return s;
   if (!q || !fnet || !rhs) return fail(MG_ERR_ARG, "q/fnet/rhs NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
@@ -1970,7 +2031,7 @@ mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, viscous != 0))) return s;
-  const bool mix = gas->eos_kind == 1;
+  const int mix = gas_variant(gas);
   if (mix && (s = ensure_mech())) return s;
   if (!q || !lengthscale || !dt) return fail(MG_ERR_ARG, "q/lengthscale/dt NULL");
   if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
@@ -1980,6 +2041,37 @@ mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double
                                dt, st);
   return timestep_species<3>(mesh, gas, q, temperature, lengthscale, viscous ? 1 : 0,
                              dt, st);
+}
+
+mg_status mg_transport_vars(const mg_gas* gas, int32_t dim, int64_t n,
+                            const double* q, const double* temperature, double* mu,
+                            double* mu_b, double* kappa, double* diff, void* stream) {
+  mg_status s;
+  if ((s = check_gas(gas, true))) return s;
+  const int mix = gas_variant(gas);
+  if (mix && (s = ensure_mech())) return s;
+  if (!mix && gas->nspecies != 0)
+    return fail(MG_ERR_UNSUPPORTED, "ideal gas with species");
+  if (!q || !mu || !mu_b || !kappa || n < 0 || (mix && (!temperature || !diff)))
+    return fail(MG_ERR_ARG, "bad arguments");
+  if (dim != 2 && dim != 3) return fail(MG_ERR_ARG, "dim");
+  if (n == 0) return MG_OK;
+  const unsigned nblk = unsigned((n + mgdg::kBlock - 1) / mgdg::kBlock);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  const mgdg::GasDev g = to_dev(gas);
+#define MG_TV(D, NS, MX)                                                          \
+  hipLaunchKernelGGL((mgdg::transport_vars_kernel<D, NS, MX>), dim3(nblk),        \
+                     dim3(mgdg::kBlock), 0, st, g, n, q, temperature, mu, mu_b, \
+                     kappa, diff)
+  if (mix == 2) {
+    if (dim == 2) MG_TV(2, 7, 2); else MG_TV(3, 7, 2);
+  } else if (mix) {
+    if (dim == 2) MG_TV(2, 7, 1); else MG_TV(3, 7, 1);
+  } else {
+    if (dim == 2) MG_TV(2, 0, 0); else MG_TV(3, 0, 0);
+  }
+#undef MG_TV
+  return check_launch("transport_vars_kernel");
 }
 
 mg_status mg_rk_axpy(int64_t n, const double* y, const double* k, double a,

@@ -31,6 +31,7 @@ struct GasDev {
   double pl_alpha, pl_beta, pl_sigma, pl_n;
   int has_lewis;
   double diff[8];      // species diffusivities, or Lewis numbers if has_lewis
+  double ma_alpha, ma_factor, ma_epsilon, ma_singular;  // mixture-averaged
 };
 
 // {{{ uiuc_7sp thermochemistry (species order C2H4, O2, CO2, CO, H2O, H2, N2)
@@ -159,6 +160,24 @@ __device__ __forceinline__ double mix_temperature(double e_in, double t_seed,
   return t;
 }
 
+// Pure-species transport fits (degree-4 polynomials in ln T, lowest power
+// first), derived on the host from the mechanism's Lennard-Jones parameters:
+//   sqrt(mu_k / sqrt(T)),  lambda_k / sqrt(T),  p D_kj / T^1.5
+// and the Wilke factors phi_a = (1 + W_k/W_j)^-1/2 / sqrt(8),
+// phi_b = (W_j/W_k)^1/4.
+constexpr int MIX_NPAIR = MIX_NSP * (MIX_NSP - 1) / 2;
+
+struct TranDev {
+  double visc[MIX_NSP][5];
+  double cond[MIX_NSP][5];
+  double diff[MIX_NPAIR][5];  // pairs k < j, row-major upper triangle
+  double dself[MIX_NSP][5];
+  double phi_a[MIX_NSP][MIX_NSP];
+  double phi_b[MIX_NSP][MIX_NSP];
+};
+
+extern __constant__ TranDev c_tran;
+
 // }}}
 
 template <int DIM, int NSP>
@@ -175,9 +194,92 @@ struct Node {
   double d[NS1];
 };
 
-// Transport coefficients of a node (SimpleTransport / PowerLawTransport).
-template <int DIM, int NSP, bool MIX>
+// MixtureAveragedTransport (mirgecom/transport.py:324-502) with the mixture
+// rules of the pyrometheus mechanism class: Wilke's viscosity
+// mu = sum_k X_k mu_k / sum_j X_j phi_kj, where sqrt(mu_k/mu_j) is the ratio
+// of the fitted square roots; kappa = (sum X_k lambda_k + 1/sum X_k/lambda_k)/2;
+// d_k = (1 - Y_k) / (p sum_{j != k} X_j / D_kj(p = 1)), the self diffusion
+// D_kk/p where that sum vanishes, the singular diffusivity where
+// 1 - Y_k < epsilon; or kappa/(rho Le_k cp) with Lewis numbers.
+template <int DIM, int NSP>
+__device__ __forceinline__ void mixavg_transport(Node<DIM, NSP>& s, const GasDev& g) {
+  const double t = s.t;
+  const double lt = log(t);
+  const double st = sqrt(t);
+  double ysw = 0.0;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) ysw += s.y[k] * c_mech.inv_wt[k];
+  const double mmw = 1.0 / ysw;
+  double x[MIX_NSP], sv[MIX_NSP], isv[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    x[k] = s.y[k] * mmw * c_mech.inv_wt[k];
+    sv[k] = horner5(c_tran.visc[k], lt);
+    isv[k] = 1.0 / sv[k];
+  }
+  double mu = 0.0;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    double den = 0.0;
+#pragma unroll
+    for (int j = 0; j < MIX_NSP; ++j) {
+      const double r = 1.0 + sv[k] * isv[j] * c_tran.phi_b[k][j];
+      den += x[j] * (c_tran.phi_a[k][j] * (r * r));
+    }
+    mu += x[k] * (st * sv[k] * sv[k]) / den;
+  }
+  mu *= g.ma_factor;
+  double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    const double lam = st * horner5(c_tran.cond[k], lt);
+    s1 += x[k] * lam;
+    s2 += x[k] / lam;
+  }
+  const double kappa = g.ma_factor * (0.5 * (s1 + 1.0 / s2));
+  s.mu = mu;
+  s.mu_b = g.ma_alpha * mu;
+  s.kappa = kappa;
+  if (g.has_lewis) {
+    // cp of the mixture from gamma and R_mix = p / (rho T)
+    const double cp = s.gamma * s.p * s.inv_rho / (t * (s.gamma - 1.0));
+#pragma unroll
+    for (int k = 0; k < MIX_NSP; ++k) s.d[k] = kappa * s.inv_rho / (g.diff[k] * cp);
+    return;
+  }
+  const double t15 = t * st;
+  double dn[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) dn[k] = 0.0;
+#pragma unroll
+  for (int k = 0, m = 0; k < MIX_NSP; ++k) {
+#pragma unroll
+    for (int j = k + 1; j < MIX_NSP; ++j, ++m) {
+      const double inv_d = 1.0 / (t15 * horner5(c_tran.diff[m], lt));
+      dn[k] += x[j] * inv_d;
+      dn[j] += x[k] * inv_d;
+    }
+  }
+  const double inv_p = 1.0 / s.p;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    double d = dn[k] > 0.0 ? (1.0 - s.y[k]) * inv_p / dn[k]
+                           : t15 * horner5(c_tran.dself[k], lt) * inv_p;
+    if (1.0 - s.y[k] < g.ma_epsilon) d = g.ma_singular;
+    s.d[k] = g.ma_factor * d;
+  }
+}
+
+// Transport coefficients of a node: SimpleTransport / PowerLawTransport by
+// transport_kind, MixtureAveragedTransport in the MIX == 2 instantiations
+// (0 ideal gas, 1 mixture, 2 mixture with mixture-averaged transport; kernels
+// that evaluate no transport use 1 for both mixture cases).
+template <int DIM, int NSP, int MIX>
 __device__ __forceinline__ void node_transport(Node<DIM, NSP>& s, const GasDev& g) {
+  if constexpr (MIX == 2) {
+    mixavg_transport<DIM, NSP>(s, g);
+    return;
+  }
   const double irho = s.inv_rho;
   if (g.transport_kind == 1) {
     s.mu = g.mu;
@@ -190,7 +292,7 @@ __device__ __forceinline__ void node_transport(Node<DIM, NSP>& s, const GasDev& 
     s.mu = mu;
     s.mu_b = g.pl_alpha * mu;
     double cv;
-    if constexpr (!MIX) {
+    if constexpr (MIX == 0) {
       cv = g.gas_const / (g.gamma - 1.0);
     } else {
       double e;
@@ -206,7 +308,7 @@ __device__ __forceinline__ void node_transport(Node<DIM, NSP>& s, const GasDev& 
 
 // Fill the dependent variables of a node from q (and T for mixtures); the
 // transport coefficients only if TRANSPORT.
-template <int DIM, int NSP, bool MIX, bool TRANSPORT = true>
+template <int DIM, int NSP, int MIX, bool TRANSPORT = true>
 __device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
                                            double t_mix) {
   const double rho = s.q[0];
@@ -224,7 +326,7 @@ __device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
   for (int d = 0; d < DIM; ++d) vv += s.v[d] * s.v[d];
   s.speed = sqrt(vv);
   const double e_int = s.q[1] - 0.5 * mm * irho;
-  if constexpr (!MIX) {
+  if constexpr (MIX == 0) {
     s.gamma = g.gamma;
     s.p = e_int * (g.gamma - 1.0);
     s.t = ((g.gamma - 1.0) / g.gas_const) * e_int * irho;
@@ -327,7 +429,7 @@ __device__ __forceinline__ void inviscid_numflux(const Node<DIM, NSP>& a,
 
 // Viscous flux F_V[f][d] (viscous.py:232-293). gq[f][d] = grad of cv field
 // f, gt[d] = grad T.
-template <int DIM, int NSP, bool MIX>
+template <int DIM, int NSP, int MIX>
 __device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
                                              const double (*gq)[DIM],
                                              const double* gt, double (*FV)[DIM]) {
@@ -359,7 +461,7 @@ __device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
 #pragma unroll
     for (int d = 0; d < DIM; ++d) {
       double sum = 0.0;
-      if constexpr (MIX) {
+      if constexpr (MIX != 0) {
 #pragma unroll
         for (int k = 0; k < NSP; ++k) sum += s.d[k] * gy[k][d];
       }
@@ -375,7 +477,7 @@ __device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
 #pragma unroll
     for (int j = 0; j < DIM; ++j) tv += tau[d][j] * s.v[j];
     double qd = 0.0;
-    if constexpr (MIX) {
+    if constexpr (MIX != 0) {
 #pragma unroll
       for (int k = 0; k < NSP; ++k) qd += s.hk[k] * jd[k][d];
     }
@@ -388,7 +490,7 @@ __device__ __forceinline__ void viscous_flux(const Node<DIM, NSP>& s,
 }
 
 // Normal viscous flux F_V . n without materialising F_V[f][d].
-template <int DIM, int NSP, bool MIX>
+template <int DIM, int NSP, int MIX>
 __device__ __forceinline__ void viscous_flux_n(const Node<DIM, NSP>& s,
                                                const double (*gq)[DIM],
                                                const double* gt, const double* n,

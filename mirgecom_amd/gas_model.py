@@ -45,6 +45,9 @@ class GasModel:
         if nspecies > 8:
             raise NotImplementedError("at most 8 species")
         if self.transport is not None and with_transport:
+            if getattr(self.transport, "kind", 0) == 3 and g.eos_kind != 1:
+                raise ValueError("mixture-averaged transport needs the "
+                                 "mixture EOS")
             prm = self.transport.device_params(nspecies)
             diff = prm.pop("diff")
             for k, v in prm.items():

@@ -76,6 +76,39 @@ _REACTIONS = [(1.0e10, 0.0, 36000.0, 1.15),
               (7.0e8, 0.0, 12000.0, 1.25)]
 
 
+# Lennard-Jones transport parameters of each species (the yaml's ``transport``
+# blocks): geometry, diameter [Angstrom], well depth eps/k_B [K], dipole moment
+# [Debye], polarizability [Angstrom^3], rotational relaxation number at 298 K.
+_TRANSPORT = {
+    "C2H4": ("nonlinear", 3.971, 280.8, 0.0, 0.0, 1.5),
+    "O2": ("linear", 3.458, 107.4, 0.0, 1.6, 3.8),
+    "CO2": ("linear", 3.763, 244.0, 0.0, 2.65, 2.1),
+    "CO": ("linear", 3.65, 98.1, 0.0, 1.95, 1.8),
+    "H2O": ("nonlinear", 2.605, 572.4, 1.844, 0.0, 4.0),
+    "H2": ("linear", 2.92, 38.0, 0.0, 0.79, 280.0),
+    "N2": ("linear", 3.621, 97.53, 0.0, 1.76, 4.0),
+}
+
+# NASA7 validity of each species ([T_min, T_max]); the phase is valid on the
+# intersection, which is where the transport fits are made.
+_T_RANGE = {"C2H4": (200.0, 3500.0), "O2": (200.0, 3500.0),
+            "CO2": (200.0, 3500.0), "CO": (200.0, 3500.0),
+            "H2O": (200.0, 3500.0), "H2": (300.0, 5000.0),
+            "N2": (300.0, 5000.0)}
+
+
+def transport_parameters():
+    """Per-species ``(geometry, sigma [A], eps/k_B [K], dipole [D],
+    polarizability [A^3], z_rot(298))`` in species order."""
+    return [_TRANSPORT[s] for s in SPECIES]
+
+
+def phase_temperature_range():
+    """``(T_min, T_max)`` where every species' thermo is valid."""
+    return (max(_T_RANGE[s][0] for s in SPECIES),
+            min(_T_RANGE[s][1] for s in SPECIES))
+
+
 def molecular_weights():
     return np.array([sum(_ATOMIC[el] * n for el, n in _COMPOSITION[s].items())
                      for s in SPECIES])

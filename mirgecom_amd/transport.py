@@ -1,8 +1,11 @@
-"""Transport models: ``SimpleTransport`` and ``PowerLawTransport``.
+"""Transport models: ``SimpleTransport``, ``PowerLawTransport`` and
+``MixtureAveragedTransport``.
 
-Call surface and formulas of mirgecom/transport.py:143-321.  The DG kernels
+Call surface and formulas of mirgecom/transport.py:143-502.  The DG kernels
 evaluate them pointwise in registers from these parameters; the tensor
-methods here serve callers that want the transport variables themselves.
+methods here serve callers that want the transport variables themselves
+(the mixture-averaged ones through the kernel entry point
+``mg_transport_vars``, i.e. the same code the Navier-Stokes kernels run).
 """
 
 from __future__ import annotations
@@ -89,3 +92,80 @@ class PowerLawTransport:
         return GasTransportVars(bulk_viscosity=self._alpha * mu, viscosity=mu,
                                 thermal_conductivity=kappa,
                                 species_diffusivity=d)
+
+
+class MixtureAveragedTransport:
+    """Mixture-averaged transport of the uiuc_7sp mixture
+    (mirgecom/transport.py:324-502): Wilke's rule for the viscosity, the
+    average of the arithmetic and harmonic mole-fraction means for the
+    conductivity, ``d_k = (1 - Y_k) / sum_{j != k} X_j / D_kj`` for the
+    species diffusivities (``singular_diffusivity`` where ``1 - Y_k <
+    epsilon``), or ``kappa / (rho Le_k c_p)`` with Lewis numbers;
+    ``mu_B = alpha mu``, everything scaled by ``factor``.  The pure-species
+    fits come from ``mirgecom_amd.transport_fits``."""
+
+    kind = 3
+
+    def __init__(self, pyrometheus_mech=None, alpha=0.6, factor=1.0,
+                 lewis=None, epsilon=1e-4, singular_diffusivity=1e-6):
+        nspecies = getattr(pyrometheus_mech, "num_species", 7)
+        if pyrometheus_mech not in (None, "uiuc_7sp") and nspecies != 7:
+            raise NotImplementedError("only the uiuc_7sp mechanism is built in")
+        self._alpha = float(alpha)
+        self._factor = float(factor)
+        self._lewis = lewis
+        self._epsilon = float(epsilon)
+        self._singular_diffusivity = float(singular_diffusivity)
+        if lewis is not None and len(lewis) != 7:
+            raise ValueError("Lewis number should match number of species")
+
+    def device_params(self, nspecies):
+        from mirgecom_amd import _lib
+        _lib.ensure_transport()
+        d = np.zeros(8)
+        if self._lewis is not None:
+            d[:7] = np.asarray(self._lewis, dtype=np.float64)
+        return dict(transport_kind=3, ma_alpha=self._alpha,
+                    ma_factor=self._factor, ma_epsilon=self._epsilon,
+                    ma_singular_diffusivity=self._singular_diffusivity,
+                    has_lewis=int(self._lewis is not None), diff=d)
+
+    def _eval(self, cv, dv, eos):
+        from mirgecom_amd import _lib
+        from mirgecom_amd.gas_model import GasModel
+        if not getattr(eos, "is_mixture", False):
+            raise ValueError("mixture-averaged transport needs the mixture EOS")
+        gas = GasModel(eos=eos, transport=self).device_params(cv.nspecies)
+        q = cv.q.contiguous()
+        t = dv.temperature.to(dtype=q.dtype).expand_as(cv.mass).contiguous()
+        shape = cv.mass.shape
+        mu = torch.empty(shape, dtype=q.dtype, device=q.device)
+        mu_b = torch.empty_like(mu)
+        kappa = torch.empty_like(mu)
+        diff = torch.empty((cv.nspecies,) + tuple(shape), dtype=q.dtype,
+                           device=q.device)
+        _lib.ensure_mechanism()
+        _lib.call("mg_transport_vars", gas, cv.dim, cv.mass.numel(),
+                  _lib.ptr(q), _lib.ptr(t), _lib.ptr(mu), _lib.ptr(mu_b),
+                  _lib.ptr(kappa), _lib.ptr(diff), _lib.stream_handle())
+        return GasTransportVars(bulk_viscosity=mu_b, viscosity=mu,
+                                thermal_conductivity=kappa,
+                                species_diffusivity=diff)
+
+    def transport_vars(self, cv, dv, eos):
+        return self._eval(cv, dv, eos)
+
+    def viscosity(self, cv, dv, eos=None):
+        return self._eval(cv, dv, eos).viscosity
+
+    def bulk_viscosity(self, cv, dv, eos=None):
+        return self._alpha * self.viscosity(cv, dv, eos)
+
+    def volume_viscosity(self, cv, dv, eos=None):
+        return (self._alpha - 2.0 / 3.0) * self.viscosity(cv, dv, eos)
+
+    def thermal_conductivity(self, cv, dv, eos):
+        return self._eval(cv, dv, eos).thermal_conductivity
+
+    def species_diffusivity(self, cv, dv, eos):
+        return self._eval(cv, dv, eos).species_diffusivity

@@ -31,6 +31,15 @@ def test_tables_match_yaml():
         assert th["temperature-ranges"][1] == mech.T_MID
         assert np.array_equal(np.array(th["data"][0]), lo[k])
         assert np.array_equal(np.array(th["data"][1]), hi[k])
+        tr = sp["transport"]
+        geo, sig, eps, dip, pol, zrot = mech._TRANSPORT[name]
+        assert tr["model"] == "gas" and tr["geometry"] == geo
+        assert (tr["diameter"], tr["well-depth"]) == (sig, eps)
+        assert tr.get("dipole", 0.0) == dip
+        assert tr.get("polarizability", 0.0) == pol
+        assert tr.get("rotational-relaxation", 0.0) == zrot
+        assert tuple(th["temperature-ranges"][::2]) == mech._T_RANGE[name]
+    assert phase["transport"] == "mixture-averaged"
     rxns = doc["reactions"]
     assert len(rxns) == 3
     for r, (a, b, ea, order) in zip(rxns, mech._REACTIONS):
