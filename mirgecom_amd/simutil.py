@@ -78,3 +78,28 @@ def compare_fluid_solutions(dcoll, red_state, blue_state, *, dd=DD_VOLUME_ALL):
     component, as a list."""
     resid = red_state.q - blue_state.q
     return componentwise_norms(dcoll, resid, dd=dd).tolist()
+
+
+def write_visfile(dcoll, io_fields, visualizer, vizname, step=0, t=0,
+                  overwrite=False, vis_timer=None, comm=None):
+    """mirgecom/simutil.py:309-386: one ``.vtu`` per rank named
+    ``vizname-<step>-<rank>.vtu`` and the ``vizname-<step>.pvtu`` manifest;
+    rank 0 creates the directory."""
+    import os
+    from contextlib import nullcontext
+
+    from mirgecom_amd.io import make_par_fname, make_rank_fname
+    rank = comm.Get_rank() if comm is not None else 0
+    rank_fn = make_rank_fname(basename=vizname, rank=rank, step=step, t=t)
+    if rank == 0:
+        viz_dir = os.path.dirname(rank_fn)
+        if viz_dir and not os.path.exists(viz_dir):
+            os.makedirs(viz_dir)
+    if comm is not None:
+        comm.barrier()
+    ctm = vis_timer.start_sub_timer() if vis_timer else nullcontext()
+    with ctm:
+        visualizer.write_parallel_vtk_file(
+            comm, rank_fn, io_fields, overwrite=overwrite,
+            par_manifest_filename=make_par_fname(basename=vizname, step=step,
+                                                 t=t))
