@@ -141,7 +141,9 @@ mg_status mg_ns_rhs(const mg_mesh* mesh, const mg_gas* gas,
  * conserved fields and of T (as mg_ns_gradient) feed, in the same kernel,
  * the pointwise net flux F_I - F_V at every node, written to fnet
  * [F*dim][N] (F_I omitted if inviscid_on == 0).  The gradients themselves
- * are written only if grad != NULL.  For boundary face nodes fnet_bnd
+ * are written only if grad != NULL; for the uiuc_7sp mixture grad
+ * [(F+1)*dim][N] is required: the mixture path stores the gradients there
+ * and forms the net fluxes in a node-parallel pass.  For boundary face nodes fnet_bnd
  * [F][nbnd_nodes] receives (F_I - F_V)(q_ext, grad q-) . n, the exterior side
  * of the boundary fluxes (mirgecom/boundary.py:944-970); on Mengaldo nodes
  * its viscous part is 2 F_V(bc).n - F_V(q-, grad q-).n, so that the central

@@ -1189,6 +1189,86 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 
 // }}}
 
+// {{{ nodal net flux from stored gradients
+//
+// The pointwise phases (5 and 6) of grad_kernel<FLUX> as node-parallel
+// passes over gradients that grad_kernel<FLUX=false> stored.  Used for the
+// 12-field mixture: its flux kernel holds only 4 elements (40 nodes at p = 2)
+// per workgroup, so the pointwise phase, which carries the NASA7 and
+// transport work, kept 40 of 256 lanes busy.
+
+template <int DIM, int NSP, int MIX>
+__global__ __launch_bounds__(kBlock, 2) void node_flux_kernel(KArgs a) {
+  constexpr int NFLD = 2 + DIM + NSP;
+  const int64_t N = a.nnodes;
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= N) return;
+  Node<DIM, NSP> s;
+  load_q<DIM, NSP>(s, a.q, N, g);
+  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
+  double gq[NFLD][DIM], gt[DIM], F[NFLD][DIM];
+  load_grad<DIM, NSP>(gq, gt, a.grad, N, g);
+  viscous_flux<DIM, NSP, MIX>(s, gq, gt, F);
+  double FI[NFLD][DIM];
+  if (a.inviscid_on) {
+    inviscid_flux<DIM, NSP>(s, FI);
+  } else {
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) FI[f][d] = 0.0;
+  }
+#pragma unroll
+  for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) a.out[(f * DIM + d) * N + g] = FI[f][d] - F[f][d];
+}
+
+// Boundary face nodes (one thread per element face node; interior ones
+// return at once): the exterior net normal flux with the interior gradients
+// (boundary.py:944-970), or, on Mengaldo nodes, the interior gradients
+// copied to wwork for wall_flux_kernel.
+template <int DIM, int P, int NSP, int MIX>
+__global__ __launch_bounds__(kBlock) void bnd_node_flux_kernel(KArgs a) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NFN = E::NFN, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + NSP, NS = NFLD + 1;
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= a.K * NFN) return;
+  const int32_t nb = a.vmap_p[t];
+  if (nb >= 0) return;
+  const int64_t e = t / NFN;
+  const int fn = int(t - e * NFN);
+  const int64_t b = -int64_t(nb) - 1;
+  const int f = fn / NFP;
+  const int64_t N = a.nnodes;
+  const int64_t gi = e * NP + a.fmask[fn];
+  if (wall_flags(a, b) & kBcMengaldo) {
+#pragma unroll
+    for (int k = 0; k < NS * DIM; ++k) a.wwork[k * a.nbnd + b] = a.grad[k * N + gi];
+    return;
+  }
+  double n[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + f) * DIM + d];
+  Node<DIM, NSP> s;
+  load_q<DIM, NSP>(s, a.qb, a.nbnd, b);
+  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.tb[b] : 0.0);
+  double gq[NFLD][DIM], gt[DIM], fv[NFLD], fi[NFLD];
+  load_grad<DIM, NSP>(gq, gt, a.grad, N, gi);
+  if (a.inviscid_on) {
+    inviscid_flux_n<DIM, NSP>(s, n, fi);
+  } else {
+#pragma unroll
+    for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
+  }
+  viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
+#pragma unroll
+  for (int k = 0; k < NFLD; ++k) a.fnb_out[k * a.nbnd + b] = fi[k] - fv[k];
+}
+
+// }}}
+
 // {{{ Mengaldo boundary nodes
 //
 // One thread per boundary node; nodes without MG_BC_MENGALDO return at once.
@@ -1575,6 +1655,31 @@ mg_status grad_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
               "unsupported species count / EOS combination (nspecies 0 or 7)");
 }
 
+// Nodal net fluxes of the uiuc_7sp mixture from stored gradients (a.grad).
+template <int DIM, int P>
+mg_status node_flux_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
+  if (nsp != 7 || !mix)
+    return fail(MG_ERR_UNSUPPORTED, "node-parallel net flux is the mixture path");
+  const unsigned nblk = unsigned((a.nnodes + mgdg::kBlock - 1) / mgdg::kBlock);
+  if (mix == 2)
+    hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 2>), dim3(nblk),
+                       dim3(mgdg::kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 1>), dim3(nblk),
+                       dim3(mgdg::kBlock), 0, st, a);
+  mg_status s = check_launch("node_flux_kernel");
+  if (s || a.nbnd == 0) return s;
+  const int64_t nt = a.K * mgdg::Elem<DIM, P>::NFN;
+  const unsigned nb = unsigned((nt + mgdg::kBlock - 1) / mgdg::kBlock);
+  if (mix == 2)
+    hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 2>), dim3(nb),
+                       dim3(mgdg::kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 1>), dim3(nb),
+                       dim3(mgdg::kBlock), 0, st, a);
+  return check_launch("bnd_node_flux_kernel");
+}
+
 template <int DIM, int P>
 mg_status dispatch_p(const mgdg::KArgs& a, int nsp, int mix, int what,
                      hipStream_t st) {
@@ -1586,12 +1691,14 @@ mg_status dispatch_p(const mgdg::KArgs& a, int nsp, int mix, int what,
     case 4: return grad_species<DIM, P, true, double>(a, nsp, mix, st);
     case 5: return grad_species<DIM, P, true, float>(a, nsp, mix, st);
     case 6: return rhs_species<DIM, P, mgdg::kFlux, float>(a, nsp, mix, st);
+    case 7: return node_flux_species<DIM, P>(a, nsp, mix, st);
   }
   return fail(MG_ERR_ARG, "kernel selector");
 }
 
 // what: 0 Euler RHS, 1 NS RHS from gradients, 2 NS RHS from nodal fluxes,
-// 3 gradients, 4 gradients + nodal fluxes, 5/6 the mixed-precision 4/2
+// 3 gradients, 4 gradients + nodal fluxes, 5/6 the mixed-precision 4/2,
+// 7 nodal fluxes from stored gradients (mixture)
 mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, int mix,
                    int what, hipStream_t st) {
 #define MG_CASE(D, P) \
@@ -1892,7 +1999,18 @@ mg_status mg_ns_flux(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   a.fnb_out = fnet_bnd;
   a.inviscid_on = inviscid_on ? 1 : 0;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if ((s = dispatch(mesh, a, gas->nspecies, mix, 4, st))) return s;
+  if (mix) {
+    // mixture: gradients through `grad`, then node-parallel net fluxes
+    if (!grad) return fail(MG_ERR_ARG, "the mixture path needs grad (gradient staging)");
+    a.out = grad;
+    a.grad_out = nullptr;
+    if ((s = dispatch(mesh, a, gas->nspecies, mix, 3, st))) return s;
+    a.out = fnet;
+    a.grad = grad;
+    if ((s = dispatch(mesh, a, gas->nspecies, mix, 7, st))) return s;
+  } else if ((s = dispatch(mesh, a, gas->nspecies, mix, 4, st))) {
+    return s;
+  }
   return wall_pass<true>(a, mesh->dim, gas->nspecies, mix, 0, st);
 }
 

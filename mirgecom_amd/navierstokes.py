@@ -130,7 +130,8 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
             fnb = torch.empty((nf, prep.qb.shape[1]), dtype=prep.q.dtype,
                               device=prep.q.device)
         grad = None
-        if return_gradients:
+        if return_gradients or state.is_mixture:
+            # the mixture path stages its gradients through this buffer
             grad = torch.empty((nf + 1, dim) + shape, dtype=prep.q.dtype,
                                device=prep.q.device)
         _lib.call("mg_ns_flux", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
