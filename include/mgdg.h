@@ -228,6 +228,31 @@ mg_status mg_bound_limiter(int32_t np, int64_t nelements, int32_t nfields,
                            const double* weights, const double* field, double* out,
                            double mmin, double mmax, int32_t flags, void* stream);
 
+/* Operators of the quadrature discretization (overintegration,
+ * DISCR_TAG_QUAD; mirgecom/discretization.py:65-83): nq volume points and
+ * nfq points per face, row-major device arrays. */
+typedef struct mg_quad {
+  int32_t nq, nfq;
+  const double* interp;       /* [nq][np]  nodal values -> volume points       */
+  const double* weak_diff;    /* [dim][np][nq]  M^-1 D_q,r^T W                 */
+  const double* face_interp;  /* [nfaces][nfq][nfp]  face nodes -> face points */
+  const double* lift;         /* [np][nfaces*nfq]  M^-1 E_q                    */
+} mg_quad;
+
+/* Euler RHS on the quadrature discretization (euler_operator with
+ * quadrature_tag=DISCR_TAG_QUAD, mirgecom/euler.py:367-416): the state is
+ * interpolated to the volume points, where the inviscid flux is formed, and
+ * both face traces to the face points, where the numerical flux (flux_kind)
+ * is formed; weak divergence and lift integrate them with the quadrature
+ * weights.  Ideal gas (passive species allowed).  q_bnd_quad [F][nbnd_faces
+ * * nfq] holds the boundary exterior states at the boundary faces' points
+ * (face by face, in the order of the boundary nodes); NULL without
+ * boundaries. */
+mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
+                            const mg_quad* quad, const double* q,
+                            const double* q_bnd_quad, int32_t flux_kind, double* rhs,
+                            void* stream);
+
 /* Upload the uiuc_7sp mechanism constants (NASA7 tables, molecular weights,
  * gas constant, standard pressure, SI Arrhenius parameters; 130 doubles in
  * the order of mirgecom_amd.mechanisms.packed_mechanism) to the device.

@@ -50,6 +50,14 @@ class MgGas(ctypes.Structure):
     ]
 
 
+class MgQuad(ctypes.Structure):
+    _fields_ = [
+        ("nq", ctypes.c_int32), ("nfq", ctypes.c_int32),
+        ("interp", ctypes.c_void_p), ("weak_diff", ctypes.c_void_p),
+        ("face_interp", ctypes.c_void_p), ("lift", ctypes.c_void_p),
+    ]
+
+
 # exported symbol -> (restype, argtypes)
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -88,6 +96,8 @@ SIGNATURES = {
     "mg_local_timestep": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  _P, _P, _P, _I32, _P, _P]),
     "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
+    "mg_euler_rhs_quad": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                                 ctypes.POINTER(MgQuad), _P, _P, _I32, _P, _P]),
     "mg_bound_limiter": (_I32, [_I32, _I64, _I32, _P, _P, _P, _D, _D, _I32,
                                 _P]),
     "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),

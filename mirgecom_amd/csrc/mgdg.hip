@@ -1189,6 +1189,141 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 
 // }}}
 
+// {{{ Euler RHS on the quadrature discretization (overintegration)
+//
+// A workgroup owns epb elements (host-chosen from the LDS budget).  LDS:
+// s_q [F][epb][NP] the state, s_g [F*DIM][epb][nq] the inviscid flux in
+// reference directions (rx . F) at the volume points, s_f [F][epb][NF*nfq]
+// the numerical flux times the face Jacobian ratio at the face points.
+// Lanes own quadrature points in phases 2-3 and (field, node) outputs in
+// phase 4; operator rows vary per lane and are read through the caches.
+
+struct QuadArgs {
+  int nq, nfq, epb;
+  const double* interp;   // [nq][NP]
+  const double* wdiff;    // [DIM][NP][nq]
+  const double* finterp;  // [NF][nfq][NFP]
+  const double* lift;     // [NP][NF*nfq]
+  const double* qbq;      // [F][nbq] exterior states at boundary face points
+  int64_t nbq;
+};
+
+template <int DIM, int P, int NSP>
+__global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + NSP;
+  extern __shared__ double lds[];
+  const int nq = qa.nq, nfq = qa.nfq, epb = qa.epb, nfq_all = NF * qa.nfq;
+  double* s_q = lds;
+  double* s_g = s_q + NFLD * epb * NP;
+  double* s_f = s_g + NFLD * DIM * epb * nq;
+  const int64_t e0 = int64_t(blockIdx.x) * epb;
+  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
+  const int64_t N = a.nnodes;
+  // 1: the elements' state (each field's rows are contiguous in HBM)
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int f = t / (ne * NP);
+    const int r = t - f * ne * NP;
+    s_q[f * epb * NP + r] = a.q[f * N + e0 * NP + r];
+  }
+  __syncthreads();
+  // 2: inviscid flux at the volume points, in reference directions
+  for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
+    const int le = t / nq;
+    const int m = t - le * nq;
+    const double* w = qa.interp + m * NP;
+    Node<DIM, NSP> s;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) {
+      const double* u = s_q + (f * epb + le) * NP;
+      double v = 0.0;
+#pragma unroll
+      for (int i = 0; i < NP; ++i) v += w[i] * u[i];
+      s.q[f] = v;
+    }
+    node_state<DIM, NSP, 0, false>(s, a.gas, 0.0);
+    double F[NFLD][DIM];
+    inviscid_flux<DIM, NSP>(s, F);
+    const double* rx = a.rx + (e0 + le) * DIM * DIM;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) {
+        double g = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) g += rx[r * DIM + d] * F[f][d];
+        s_g[((f * DIM + r) * epb + le) * nq + m] = g;
+      }
+  }
+  // 3: numerical flux at the face points (both traces interpolated from the
+  // face nodes in this element's face-node order)
+  for (int t = threadIdx.x; t < ne * nfq_all; t += blockDim.x) {
+    const int le = t / nfq_all;
+    const int rem = t - le * nfq_all;
+    const int fc = rem / nfq;
+    const int m = rem - fc * nfq;
+    const int64_t e = e0 + le;
+    const double* w = qa.finterp + (fc * nfq + m) * NFP;
+    const int32_t* vp = a.vmap_p + (e * NF + fc) * NFP;
+    const int* fm = a.fmask + fc * NFP;
+    Node<DIM, NSP> sm, sp;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) {
+      const double* u = s_q + (f * epb + le) * NP;
+      double v = 0.0;
+#pragma unroll
+      for (int j = 0; j < NFP; ++j) v += w[j] * u[fm[j]];
+      sm.q[f] = v;
+    }
+    const int32_t v0 = vp[0];
+    if (v0 >= 0) {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < NFP; ++j) v += w[j] * a.q[f * N + vp[j]];
+        sp.q[f] = v;
+      }
+    } else {
+      const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + m;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) sp.q[f] = qa.qbq[f * qa.nbq + bq];
+    }
+    node_state<DIM, NSP, 0, false>(sm, a.gas, 0.0);
+    node_state<DIM, NSP, 0, false>(sp, a.gas, 0.0);
+    double n[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
+    double fs[NFLD];
+    inviscid_numflux<DIM, NSP>(sm, sp, n, a.flux_kind, fs);
+    const double sc = a.fscale[e * NF + fc];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_f[(f * epb + le) * nfq_all + rem] = sc * fs[f];
+  }
+  __syncthreads();
+  // 4: dQ/dt = sum_r Wq_r (rx . F)_r - Lq (fscale F*)
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int f = t / (ne * NP);
+    const int r = t - f * ne * NP;
+    const int le = r / NP;
+    const int i = r - le * NP;
+    double acc = 0.0;
+#pragma unroll
+    for (int rr = 0; rr < DIM; ++rr) {
+      const double* wd = qa.wdiff + (rr * NP + i) * nq;
+      const double* g = s_g + ((f * DIM + rr) * epb + le) * nq;
+      for (int m = 0; m < nq; ++m) acc += wd[m] * g[m];
+    }
+    const double* lf = qa.lift + i * nfq_all;
+    const double* fl = s_f + (f * epb + le) * nfq_all;
+    for (int m = 0; m < nfq_all; ++m) acc -= lf[m] * fl[m];
+    a.out[f * N + e0 * NP + r] = acc;
+  }
+}
+
+// }}}
+
 // {{{ nodal net flux from stored gradients
 //
 // The pointwise phases (5 and 6) of grad_kernel<FLUX> as node-parallel
@@ -2159,6 +2294,65 @@ mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double
                                dt, st);
   return timestep_species<3>(mesh, gas, q, temperature, lengthscale, viscous ? 1 : 0,
                              dt, st);
+}
+
+mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
+                            const mg_quad* quad, const double* q,
+                            const double* q_bnd_quad, int32_t flux_kind, double* rhs,
+                            void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, false))) return s;
+  if (gas->eos_kind != 0)
+    return fail(MG_ERR_UNSUPPORTED, "the quadrature path is built for the ideal gas");
+  if (gas->nspecies != 0 && gas->nspecies != 7)
+    return fail(MG_ERR_UNSUPPORTED, "nspecies must be 0 or 7");
+  if (!quad || !quad->interp || !quad->weak_diff || !quad->face_interp || !quad->lift ||
+      quad->nq < 1 || quad->nfq < 1)
+    return fail(MG_ERR_ARG, "mg_quad incomplete");
+  if (!q || !rhs) return fail(MG_ERR_ARG, "q/rhs NULL");
+  if (mesh->nbnd_nodes > 0 && !q_bnd_quad)
+    return fail(MG_ERR_ARG, "boundary exterior state missing");
+  if (mesh->nbnd_nodes % mesh->nfp) return fail(MG_ERR_ARG, "nbnd_nodes % nfp");
+  if (flux_kind < 0 || flux_kind > 2) return fail(MG_ERR_ARG, "flux_kind");
+  const int nfld = 2 + mesh->dim + gas->nspecies;
+  const size_t per_elem =
+      sizeof(double) * size_t(nfld) *
+      (size_t(mesh->np) + size_t(mesh->dim) * quad->nq + size_t(mesh->nfaces) * quad->nfq);
+  constexpr size_t kBudget = 128 * 1024;
+  int epb = int(kBudget / per_elem);
+  if (epb < 1) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS");
+  if (epb > 16) epb = 16;
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.out = rhs;
+  a.flux_kind = flux_kind;
+  mgdg::QuadArgs qa{};
+  qa.nq = quad->nq;
+  qa.nfq = quad->nfq;
+  qa.epb = epb;
+  qa.interp = quad->interp;
+  qa.wdiff = quad->weak_diff;
+  qa.finterp = quad->face_interp;
+  qa.lift = quad->lift;
+  qa.qbq = q_bnd_quad;
+  qa.nbq = mesh->nbnd_nodes / mesh->nfp * quad->nfq;
+  const size_t lds = per_elem * epb;
+  const unsigned nblk = unsigned((a.K + epb - 1) / epb);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+#define MG_QCASE(D, P, NS)                                                        \
+  if (mesh->dim == D && mesh->order == P && gas->nspecies == NS) {                \
+    auto kern = mgdg::euler_quad_kernel<D, P, NS>;                                \
+    if ((s = set_lds(kern, kBudget))) return s;                                   \
+    hipLaunchKernelGGL(kern, dim3(nblk), dim3(mgdg::kBlock), lds, st, a, qa);     \
+    return check_launch("euler_quad_kernel");                                     \
+  }
+  MG_QCASE(2, 1, 0) MG_QCASE(2, 2, 0) MG_QCASE(2, 3, 0) MG_QCASE(2, 4, 0)
+  MG_QCASE(3, 1, 0) MG_QCASE(3, 2, 0) MG_QCASE(3, 3, 0) MG_QCASE(3, 4, 0)
+  MG_QCASE(2, 1, 7) MG_QCASE(2, 2, 7) MG_QCASE(2, 3, 7) MG_QCASE(2, 4, 7)
+  MG_QCASE(3, 1, 7) MG_QCASE(3, 2, 7) MG_QCASE(3, 3, 7) MG_QCASE(3, 4, 7)
+#undef MG_QCASE
+  return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }
 
 mg_status mg_transport_vars(const mg_gas* gas, int32_t dim, int64_t n,

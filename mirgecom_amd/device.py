@@ -70,13 +70,62 @@ class DeviceDiscretization:
         _lib.check(lib.mg_mesh_prepare(self.mesh, _lib.ptr(self.workspace),
                                        _lib.stream_handle()))
 
-    def boundary_index(self, tag):
-        """Indices (into the boundary-node arrays) of a boundary tag."""
-        if tag not in self._tag_index:
-            mask = self.host.boundary_node_slice_mask(tag)
-            self._tag_index[tag] = torch.as_tensor(
+    def boundary_index(self, tag, quad=False):
+        """Indices (into the boundary-node arrays, or the boundary
+        quadrature-point arrays) of a boundary tag."""
+        key = (tag, quad)
+        if key not in self._tag_index:
+            mask = self.host.boundary_node_slice_mask(tag, quad)
+            self._tag_index[key] = torch.as_tensor(
                 np.nonzero(mask)[0], dtype=torch.int64, device=self.device)
-        return self._tag_index[tag]
+        return self._tag_index[key]
+
+    # {{{ quadrature discretization (DISCR_TAG_QUAD)
+
+    def _quad(self):
+        if not hasattr(self, "_qcache"):
+            qo = self.host.quadrature
+            f64 = dict(dtype=torch.float64, device=self.device)
+            c = {k: torch.as_tensor(np.ascontiguousarray(getattr(qo, k)), **f64)
+                 for k in ("interp", "weak_diff", "face_interp", "lift")}
+            xq, nq = self.host.quad_boundary_geometry()
+            c["bnd_x"] = torch.as_tensor(np.ascontiguousarray(xq), **f64)
+            c["bnd_n"] = torch.as_tensor(np.ascontiguousarray(nq), **f64)
+            bf = self.host.bnd_faces
+            c["bnd_face"] = torch.as_tensor(
+                bf[:, 1] if len(bf) else np.zeros(0, dtype=np.int64),
+                dtype=torch.int64, device=self.device)
+            c["ops"] = _lib.MgQuad(
+                nq=qo.nq, nfq=qo.nfq, interp=c["interp"].data_ptr(),
+                weak_diff=c["weak_diff"].data_ptr(),
+                face_interp=c["face_interp"].data_ptr(),
+                lift=c["lift"].data_ptr())
+            self._qcache = c
+        return self._qcache
+
+    def quad_ops(self):
+        """The ``mg_quad`` struct of the quadrature operators."""
+        return self._quad()["ops"]
+
+    def quad_nodes(self):
+        """Quadrature point coordinates ``(dim, K, nq)``."""
+        return torch.einsum("qn,dkn->dkq", self._quad()["interp"], self.nodes)
+
+    def bnd_quad_geometry(self):
+        c = self._quad()
+        return c["bnd_x"], c["bnd_n"]
+
+    def to_bnd_quad(self, trace):
+        """Boundary-node values ``(..., Nbf*Nfp)`` -> boundary quadrature
+        points ``(..., Nbf*nfq)`` (each face's own interpolation)."""
+        c = self._quad()
+        nfp = self.host.refel.nface_nodes
+        lead = trace.shape[:-1]
+        t = trace.reshape(lead + (-1, nfp))
+        out = torch.einsum("bmj,...bj->...bm", c["face_interp"][c["bnd_face"]], t)
+        return out.reshape(lead + (-1,))
+
+    # }}}
 
 
 def get_device_discretization(dcoll: Discretization, device=None):

@@ -156,6 +156,7 @@ class Discretization:
     bnd_face_tags: list        # tag of each boundary face
     bnd_nodes: np.ndarray      # (dim, Nbf*Nfp)
     bnd_normals: np.ndarray    # (dim, Nbf*Nfp)
+    quadrature_order: int = 0  # degree of DISCR_TAG_QUAD (2p by default)
     _device_cache: dict = field(default_factory=dict, repr=False)
     _host_cache: dict = field(default_factory=dict, repr=False)
 
@@ -187,6 +188,34 @@ class Discretization:
     def nbnd_nodes(self):
         return self.bnd_nodes.shape[1]
 
+    @property
+    def quadrature(self):
+        """Element operators of the quadrature discretization
+        (``refelem.QuadratureOperators`` of degree ``quadrature_order``)."""
+        from mirgecom_amd.refelem import get_quadrature_operators
+        return get_quadrature_operators(self.dim, self.order,
+                                        self.quadrature_order)
+
+    def quad_boundary_geometry(self):
+        """Coordinates and unit normals ``(dim, Nbf*nfq)`` of the boundary
+        faces' quadrature points, face by face in the order of the boundary
+        nodes."""
+        if "bq" not in self._host_cache:
+            qo, re = self.quadrature, self.refel
+            bf = self.bnd_faces
+            if len(bf) == 0:
+                z = np.zeros((self.dim, 0))
+                self._host_cache["bq"] = (z, z)
+            else:
+                xf = self.host_nodes[:, bf[:, 0:1], re.face_mask[bf[:, 1]]]
+                xq = np.einsum("bmj,dbj->dbm", qo.face_interp[bf[:, 1]], xf)
+                nq = np.repeat(self.normals[bf[:, 0], bf[:, 1]][:, None, :],
+                               qo.nfq, axis=1)
+                self._host_cache["bq"] = (
+                    xq.reshape(self.dim, -1),
+                    np.moveaxis(nq, -1, 0).reshape(self.dim, -1))
+        return self._host_cache["bq"]
+
     def boundary_tags(self):
         return sorted(set(self.bnd_face_tags))
 
@@ -197,15 +226,21 @@ class Discretization:
         from mirgecom_amd.device import get_device_discretization
         dev = get_device_discretization(self)
         tag = _boundary_tag_of(dd)
+        quad = _is_quad(dd)
         if tag is None:
-            return dev.nodes
+            return dev.quad_nodes() if quad else dev.nodes
+        if quad:
+            return dev.bnd_quad_geometry()[0][:, dev.boundary_index(tag, True)]
         return dev.bnd_nodes[:, dev.boundary_index(tag)]
 
     def normal(self, dd):
         """Outward unit normals ``(dim, n)`` on the nodes of boundary ``dd``."""
         from mirgecom_amd.device import get_device_discretization
         dev = get_device_discretization(self)
-        return dev.bnd_normals[:, dev.boundary_index(_boundary_tag_of(dd))]
+        tag = _boundary_tag_of(dd)
+        if _is_quad(dd):
+            return dev.bnd_quad_geometry()[1][:, dev.boundary_index(tag, True)]
+        return dev.bnd_normals[:, dev.boundary_index(tag)]
 
     def discr_from_dd(self, dd):
         """Object with a ``nodes()`` method (grudge's ``discr_from_dd``)."""
@@ -216,14 +251,20 @@ class Discretization:
                 return dcoll.nodes(dd)
         return _D()
 
-    def boundary_node_slice_mask(self, tag):
-        """Boolean mask over boundary nodes for a boundary tag."""
-        nfp = self.refel.nface_nodes
+    def boundary_node_slice_mask(self, tag, quad=False):
+        """Boolean mask over boundary nodes (or boundary quadrature points)
+        for a boundary tag."""
+        nfp = self.quadrature.nfq if quad else self.refel.nface_nodes
         if tag == BTAG_ALL:
             fm = np.ones(len(self.bnd_face_tags), dtype=bool)
         else:
             fm = np.array([t == tag for t in self.bnd_face_tags], dtype=bool)
         return np.repeat(fm, nfp)
+
+
+def _is_quad(dd):
+    from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
+    return getattr(dd, "discretization_tag", None) == DISCR_TAG_QUAD
 
 
 def _boundary_tag_of(dd):
@@ -359,16 +400,20 @@ def _connectivity(mesh: BoxMesh, refel: ReferenceElement, nodes, normals):
     return vmap_p.astype(np.int32), bf.astype(np.int64), tags, bnodes, bnormals
 
 
-def create_discretization_collection(mesh: BoxMesh, order: int) -> Discretization:
+def create_discretization_collection(mesh: BoxMesh, order: int, *,
+                                     quadrature_order: int = -1) -> Discretization:
     """Build the nodal DG discretization of ``mesh`` at polynomial ``order``
-    (call surface of mirgecom/discretization.py:41-83)."""
+    (call surface of mirgecom/discretization.py:41-83); the quadrature
+    discretization (DISCR_TAG_QUAD) integrates to degree ``quadrature_order``,
+    ``2 order`` by default as in the reference."""
     refel = get_reference_element(mesh.dim, order)
     nodes, rx, jac, normals, face_jac, fscale = _geometry(mesh, refel)
     vmap_p, bf, tags, bnodes, bnormals = _connectivity(mesh, refel, nodes, normals)
     return Discretization(
         mesh=mesh, refel=refel, host_nodes=nodes, rx=rx, jac=jac, normals=normals,
         face_jac=face_jac, fscale=fscale, vmap_p=vmap_p, bnd_faces=bf,
-        bnd_face_tags=tags, bnd_nodes=bnodes, bnd_normals=bnormals)
+        bnd_face_tags=tags, bnd_nodes=bnodes, bnd_normals=bnormals,
+        quadrature_order=2 * order if quadrature_order < 0 else int(quadrature_order))
 
 
 def characteristic_lengthscales(dcoll: Discretization):

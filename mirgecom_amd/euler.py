@@ -13,7 +13,7 @@ import torch
 from mirgecom_amd import _lib
 from mirgecom_amd._opcommon import (
     Prepared, check_dd, check_unsupported, warn_mixture_limiter)
-from mirgecom_amd.dof_desc import DD_VOLUME_ALL, DISCR_TAG_BASE
+from mirgecom_amd.dof_desc import DD_VOLUME_ALL, DISCR_TAG_BASE, DISCR_TAG_QUAD
 from mirgecom_amd.fluid import ConservedVars
 from mirgecom_amd.inviscid import flux_kind, inviscid_facial_flux_rusanov
 
@@ -26,21 +26,45 @@ def euler_operator(dcoll, state, gas_model, boundaries, time=0.0,
                    entropy_min=None):
     r"""RHS of the Euler equations, :math:`\dot{q} = -\nabla\cdot F`.
 
-    Same arguments and errors as the reference (mirgecom/euler.py:313-372);
-    ESDG, overintegration and limiters are outside the device path and raise
-    ``NotImplementedError``."""
+    Same arguments and errors as the reference (mirgecom/euler.py:313-372).
+    ``quadrature_tag=DISCR_TAG_QUAD`` (overintegration, ideal gas) runs
+    ``mg_euler_rhs_quad``: fluxes formed and integrated at the quadrature
+    points of ``dcoll.quadrature``.  ESDG and limiters are outside the device
+    path and raise ``NotImplementedError``."""
     check_dd(dd)
     check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
-                      limiter_func, entropy_min)
+                      limiter_func, entropy_min, allow_quad=True)
     warn_mixture_limiter(state, limiter_func)
     if inviscid_numerical_flux_func is None:
         warn("inviscid_numerical_flux_func unspecified, defaulting to "
              "inviscid_facial_flux_rusanov.", stacklevel=2)
         inviscid_numerical_flux_func = inviscid_facial_flux_rusanov
     kind = flux_kind(inviscid_numerical_flux_func)
+    if quadrature_tag == DISCR_TAG_QUAD:
+        return _euler_quad(dcoll, state, gas_model, boundaries, time, kind)
     prep = Prepared(dcoll, state, gas_model, boundaries, time)
     rhs = torch.empty_like(prep.q)
     _lib.call("mg_euler_rhs", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
               _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb), kind,
               _lib.ptr(rhs), prep.stream)
+    return ConservedVars(state.dim, rhs)
+
+
+def _euler_quad(dcoll, state, gas_model, boundaries, time, kind):
+    from mirgecom_amd._opcommon import quad_boundary_states
+    from mirgecom_amd.device import get_device_discretization
+    if getattr(gas_model.eos, "is_mixture", False):
+        raise NotImplementedError("overintegration is built for the ideal gas")
+    q = state.cv.q
+    if not q.is_cuda or q.dtype != torch.float64:
+        raise TypeError("the state must be a float64 tensor on the GPU")
+    dev = get_device_discretization(dcoll, q.device)
+    q = q.contiguous()
+    qbq = quad_boundary_states(dcoll, dev, q, state.dim, gas_model, boundaries,
+                               time)
+    rhs = torch.empty_like(q)
+    _lib.call("mg_euler_rhs_quad", dev.mesh,
+              gas_model.device_params(state.cv.nspecies, with_transport=False),
+              dev.quad_ops(), _lib.ptr(q), _lib.ptr(qbq), kind, _lib.ptr(rhs),
+              _lib.stream_handle())
     return ConservedVars(state.dim, rhs)

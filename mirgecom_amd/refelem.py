@@ -476,3 +476,120 @@ def get_reference_element(dim: int, order: int) -> ReferenceElement:
         dim=dim, order=order, nodes=nodes, tuples=tuples, vdm=vdm, mass=mass,
         inv_mass=inv_mass, diff=diff, weak_diff=weak_diff,
         face_mask=face_mask, face_mass=face_mass, lift=lift)
+
+
+# {{{ quadrature (overintegration)
+
+def simplex_quadrature(dim, degree):
+    """Collapsed Gauss-Jacobi (Stroud conical product) rule on the biunit
+    ``dim``-simplex, exact for total degree ``<= degree``: ``n =
+    ceil((degree + 1) / 2)`` points per collapsed direction with the Jacobi
+    weights ``(1-b)^1``, ``(1-c)^2`` absorbing the Duffy Jacobian.  Returns
+    ``(x (nq, dim), w (nq,))``; ``sum(w)`` is the simplex volume."""
+    from scipy.special import roots_jacobi
+    n = max(1, (degree + 2) // 2)
+    a, wa = roots_jacobi(n, 0.0, 0.0)
+    if dim == 1:
+        return a[:, None], wa
+    b, wb = roots_jacobi(n, 1.0, 0.0)
+    if dim == 2:
+        aa, bb = np.meshgrid(a, b, indexing="ij")
+        r = 0.5 * (1 + aa) * (1 - bb) - 1
+        w = 0.5 * np.outer(wa, wb)
+        return np.stack([r.ravel(), bb.ravel()], axis=1), w.ravel()
+    if dim == 3:
+        c, wc = roots_jacobi(n, 2.0, 0.0)
+        aa, bb, cc = np.meshgrid(a, b, c, indexing="ij")
+        s = 0.5 * (1 + bb) * (1 - cc) - 1
+        r = 0.25 * (1 + aa) * (1 - bb) * (1 - cc) - 1
+        w = 0.125 * np.einsum("i,j,k->ijk", wa, wb, wc)
+        return np.stack([r.ravel(), s.ravel(), cc.ravel()], axis=1), w.ravel()
+    raise ValueError("dim must be 1, 2 or 3")
+
+
+def face_quadrature(dim, degree):
+    """Rule on the biunit (dim-1)-simplex of an element face, exact to
+    ``degree`` and invariant under every permutation of the face's vertices,
+    so that the two elements sharing a face evaluate the numerical flux at
+    the same physical points whatever their local face orientation:
+    Gauss-Legendre on segments; on triangles the collapsed rule (already
+    symmetric in the two vertices of its base edge) taken with each of the
+    three vertices as the collapsed one, weights divided by three."""
+    x, w = simplex_quadrature(dim - 1, degree)
+    if dim - 1 < 2:
+        return x, w
+    lam = np.stack([-(x[:, 0] + x[:, 1]) / 2, (1 + x[:, 0]) / 2,
+                    (1 + x[:, 1]) / 2], axis=1)             # barycentric
+    verts = np.array([[-1.0, -1.0], [1.0, -1.0], [-1.0, 1.0]])
+    pts = [np.roll(lam, k, axis=1) @ verts for k in range(3)]
+    return np.concatenate(pts), np.concatenate([w / 3.0] * 3)
+
+
+@dataclass(frozen=True)
+class QuadratureOperators:
+    """Element operators of the quadrature discretization (grudge's
+    DISCR_TAG_QUAD, mirgecom/discretization.py:65-83) for a nodal element:
+
+    * ``points (nq, dim)``, ``weights (nq,)``: the volume rule
+    * ``interp (nq, Np)``: nodal values -> quadrature points
+    * ``weak_diff (dim, Np, nq)``: ``M^-1 D_q,r^T W`` -- applied to flux
+      values at the points, the weak derivative ``M^-1 (d phi/dr, F)``
+    * ``face_points (nfq, dim-1)``, ``face_weights (nfq,)``: the face rule on
+      the biunit (dim-1)-simplex (face-local coordinates; ``face_quadrature``)
+    * ``face_interp (Nf, nfq, Nfp)``: face-node values (in face_mask order)
+      -> face quadrature points
+    * ``face_xref (Nf, nfq, dim)``: those points in element coordinates
+    * ``lift (Np, Nf*nfq)``: ``M^-1 E_q`` with ``E_q[i, (f, m)] = w_m
+      phi_i(x_fm)`` on the biunit face
+    """
+
+    degree: int
+    points: np.ndarray
+    weights: np.ndarray
+    interp: np.ndarray
+    weak_diff: np.ndarray
+    face_points: np.ndarray
+    face_weights: np.ndarray
+    face_interp: np.ndarray
+    face_xref: np.ndarray
+    lift: np.ndarray
+
+    @property
+    def nq(self):
+        return self.points.shape[0]
+
+    @property
+    def nfq(self):
+        return self.face_points.shape[0]
+
+
+@lru_cache(maxsize=None)
+def get_quadrature_operators(dim: int, order: int,
+                             degree: int) -> QuadratureOperators:
+    re = get_reference_element(dim, order)
+    vinv = np.linalg.inv(re.vdm)
+    xq, wq = simplex_quadrature(dim, degree)
+    interp = simplex_basis(dim, order, xq) @ vinv
+    gq = simplex_basis_grad(dim, order, xq)
+    weak_diff = np.stack([re.inv_mass @ (gq[r] @ vinv).T @ np.diag(wq)
+                          for r in range(dim)])
+    xf, wf = face_quadrature(dim, degree)
+    nf, nfq = dim + 1, xf.shape[0]
+    face_interp = np.zeros((nf, nfq, re.nface_nodes))
+    face_xref = np.zeros((nf, nfq, dim))
+    emb = np.zeros((re.nnodes, nf * nfq))
+    for f in range(nf):
+        verts = REF_VERTICES[dim][list(FACE_VERTICES[dim][f])]
+        xe = verts[0] + 0.5 * (1 + xf) @ (verts[1:] - verts[0])
+        face_xref[f] = xe
+        xn = face_local_coords(dim, f, re.nodes[re.face_mask[f]])
+        face_interp[f] = (simplex_basis(dim - 1, order, xf)
+                          @ np.linalg.inv(simplex_basis(dim - 1, order, xn)))
+        emb[:, f * nfq:(f + 1) * nfq] = (simplex_basis(dim, order, xe)
+                                         @ vinv).T * wf[None, :]
+    return QuadratureOperators(
+        degree=degree, points=xq, weights=wq, interp=interp,
+        weak_diff=weak_diff, face_points=xf, face_weights=wf,
+        face_interp=face_interp, face_xref=face_xref, lift=re.inv_mass @ emb)
+
+# }}}

@@ -1,0 +1,134 @@
+"""GPU parity of the overintegrated Euler operator (euler_operator with
+quadrature_tag=DISCR_TAG_QUAD; mg_euler_rhs_quad) against the oracle
+(oracle/overintegration.py): periodic boxes at p = 1..4 in 2-D and 3-D with
+the Rusanov, HLL and central fluxes, passive species, and prescribed /
+dummy boundaries evaluated at the boundary faces' quadrature points.
+Tolerance 1e-11 relative per field (1e-10 at p = 4: the oracle evaluates
+its Lagrange basis through monomials, whose round-off grows with p)."""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from tests.cases import Problem, rel_err, smooth_perturbed_state  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def _dev(a):
+    return torch.as_tensor(np.ascontiguousarray(a), dtype=torch.float64,
+                           device="cuda")
+
+
+def _host(t):
+    return t.detach().cpu().numpy()
+
+
+def _tol(order):
+    return 1e-10 if order >= 4 else 1e-11
+
+
+def _fluxes(name):
+    from mirgecom_amd import inviscid as gi
+    from oracle import physics as ph
+    return {"rusanov": (gi.inviscid_facial_flux_rusanov,
+                        ph.inviscid_facial_flux_rusanov),
+            "hll": (gi.inviscid_facial_flux_hll, ph.inviscid_facial_flux_hll),
+            "central": (gi.inviscid_facial_flux_central,
+                        ph.inviscid_facial_flux_central)}[name]
+
+
+@pytest.mark.parametrize("dim,order", [(2, 1), (2, 2), (2, 3), (2, 4), (3, 1),
+                                       (3, 2), (3, 3), (3, 4)])
+@pytest.mark.parametrize("flux", ["rusanov", "hll", "central"])
+def test_periodic(dim, order, flux):
+    from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
+    from mirgecom_amd.eos import IdealSingleGas
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from oracle import physics as ph
+    from oracle.overintegration import euler_operator_quad
+    n = (4, 3) if dim == 2 else (3, 3, 3)
+    pb = Problem(dim, order, n, 0.0, (1.0, 0.8, 0.9)[:dim], (True,) * dim)
+    q = smooth_perturbed_state(pb.x, seed=5 + order, amp=0.1)
+    gflux, oflux = _fluxes(flux)
+    gm = GasModel(eos=IdealSingleGas(gas_const=1.0))
+    print(f"overintegration periodic dim={dim} p={order} flux={flux}", flush=True)
+    st = make_fluid_state(ConservedVars(dim, _dev(q)), gm)
+    rhs = euler_operator(pb.dcoll, st, gm, {}, inviscid_numerical_flux_func=gflux,
+                         quadrature_tag=DISCR_TAG_QUAD)
+    ref = euler_operator_quad(pb.odisc, pb.box, q,
+                              ph.GasModel(ph.IdealSingleGas(gas_const=1.0)), {},
+                              numerical_flux_func=oflux)
+    err = rel_err(_host(rhs.q), ref)
+    print("err", err)
+    assert err.max() < _tol(order)
+
+
+def test_passive_species():
+    from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
+    from mirgecom_amd.eos import IdealSingleGas
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+    from oracle import physics as ph
+    from oracle.overintegration import euler_operator_quad
+    pb = Problem(3, 2, (3, 3, 3), 0.0, 1.0, (True,) * 3)
+    q = smooth_perturbed_state(pb.x, seed=3, nspecies=7, amp=0.1)
+    gm = GasModel(eos=IdealSingleGas())
+    st = make_fluid_state(ConservedVars(3, _dev(q)), gm)
+    rhs = euler_operator(pb.dcoll, st, gm, {},
+                         inviscid_numerical_flux_func=inviscid_facial_flux_rusanov,
+                         quadrature_tag=DISCR_TAG_QUAD)
+    ref = euler_operator_quad(pb.odisc, pb.box, q, ph.GasModel(ph.IdealSingleGas()),
+                              {})
+    assert rel_err(_host(rhs.q), ref).max() < 1e-11
+
+
+@pytest.mark.parametrize("order", [2, 3])
+@pytest.mark.parametrize("kind", ["prescribed", "dummy"])
+def test_boundaries(order, kind):
+    """The steady vortex of test/test_euler.py:407-483 with its exact state
+    prescribed at the boundary faces' quadrature points (or a dummy
+    boundary)."""
+    from mirgecom_amd.boundary import DummyBoundary, PrescribedFluidBoundary
+    from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
+    from mirgecom_amd.eos import IdealSingleGas
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from mirgecom_amd.initializers import Vortex2D
+    from mirgecom_amd.inviscid import inviscid_facial_flux_hll
+    from oracle import initializers as oini, operators as oop, physics as ph
+    from oracle.overintegration import euler_operator_quad
+    pb = Problem(2, order, (6, 5), -5.0, 5.0, (False, False))
+    gm = GasModel(eos=IdealSingleGas())
+    ogm = ph.GasModel(ph.IdealSingleGas())
+    vortex = Vortex2D(center=[0, 0], velocity=[0, 0])
+    q = oini.vortex2d(pb.x, beta=5)
+
+    def gstate(dcoll, dd_bdry, gas_model, state_minus, time=0.0, **kw):
+        return make_fluid_state(vortex(dcoll.nodes(dd_bdry), time=time,
+                                       eos=gas_model.eos), gas_model)
+
+    def ostate(xb, sm, t):
+        return ph.make_fluid_state(oini.vortex2d(xb, t, beta=5), ogm, 2)
+
+    if kind == "prescribed":
+        gb = {"all": PrescribedFluidBoundary(boundary_state_func=gstate)}
+        ob = {"all": oop.PrescribedFluidBoundary(ostate)}
+    else:
+        gb, ob = {"all": DummyBoundary()}, {"all": oop.DummyBoundary()}
+    print(f"overintegration boundaries p={order} kind={kind}", flush=True)
+    st = make_fluid_state(ConservedVars(2, _dev(q)), gm)
+    rhs = euler_operator(pb.dcoll, st, gm, gb,
+                         inviscid_numerical_flux_func=inviscid_facial_flux_hll,
+                         quadrature_tag=DISCR_TAG_QUAD)
+    ref = euler_operator_quad(pb.odisc, pb.box, q, ogm, ob,
+                              numerical_flux_func=ph.inviscid_facial_flux_hll)
+    err = rel_err(_host(rhs.q), ref)
+    print("err", err)
+    assert err.max() < 1e-11
