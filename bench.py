@@ -202,6 +202,50 @@ def measure_reacting(args, n):
                 "ms_per_rhs": ms_ns, "value": dofs / (ms_ns * 1e-3)}}
 
 
+def measure_overintegration(args, dcoll, gm, cv):
+    """Euler RHS of the same state on the nodal and on the quadrature
+    (degree 2p) discretization, FP64 (overintegration's cost)."""
+    import torch
+
+    from mirgecom_amd import _lib
+    from mirgecom_amd.dof_desc import DISCR_TAG_BASE, DISCR_TAG_QUAD
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.gas_model import make_fluid_state
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+    st = make_fluid_state(cv, _inviscid_model(gm))
+    out = {}
+    for tag in (DISCR_TAG_BASE, DISCR_TAG_QUAD):
+        def go():
+            return euler_operator(dcoll, st, st.gas_model, {},
+                                  inviscid_numerical_flux_func=inviscid_facial_flux_rusanov,
+                                  quadrature_tag=tag)
+        for _ in range(args.warmup):
+            go()
+        torch.cuda.synchronize()
+        _lib.timing = {}
+        for _ in range(args.steps):
+            go()
+        torch.cuda.synchronize()
+        t = {k: sum(a.elapsed_time(b) for a, b in v) / len(v)
+             for k, v in _lib.timing.items()}
+        _lib.timing = None
+        out[tag] = t
+    qo = dcoll.quadrature
+    res = {"workload": "Euler RHS of the same state (Rusanov), nodal vs "
+                       f"quadrature of degree {qo.degree} ({qo.nq} volume, "
+                       f"{qo.nfq} points per face)",
+           "nodal_ms": out[DISCR_TAG_BASE].get("mg_euler_rhs"),
+           "quad_ms": out[DISCR_TAG_QUAD].get("mg_euler_rhs_quad")}
+    print(f"# overintegration: {json.dumps(res)}", file=sys.stderr, flush=True)
+    return res
+
+
+def _inviscid_model(gm):
+    """The gas model without its transport (for the Euler operator)."""
+    from mirgecom_amd.gas_model import GasModel
+    return GasModel(eos=gm.eos)
+
+
 def measured_traffic(entry, n, order):
     """HBM bytes per launch of ``entry`` on the 24^3x6 p=3 workload from the
     committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary (copy-calibrated),
@@ -374,6 +418,7 @@ def main():
         print("# copy calibration: 3 x copy of 268435456 bytes", file=sys.stderr,
               flush=True)
     reacting = measure_reacting(args, args.reacting_n) if args.reacting_n else None
+    overint = measure_overintegration(args, dcoll, gm, cv)
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n, order)
@@ -414,7 +459,8 @@ def main():
                              "rel_err_rhs": err_rhs,
                              "rel_err_state": err_state,
                              "state_steps": nsteps, "dt": dt},
-                   "reacting": reacting},
+                   "reacting": reacting,
+                   "overintegration": overint},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,

@@ -230,13 +230,18 @@ mg_status mg_bound_limiter(int32_t np, int64_t nelements, int32_t nfields,
 
 /* Operators of the quadrature discretization (overintegration,
  * DISCR_TAG_QUAD; mirgecom/discretization.py:65-83): nq volume points and
- * nfq points per face, row-major device arrays. */
+ * nfq points per face, device arrays stored transposed (index of the
+ * point or node the kernel's lanes run over last):
+ *   interp_t[i][m]      = I[m][i],  I: nodal values -> volume points
+ *   weak_diff_t[r][m][i] = (M^-1 D_q,r^T W)[i][m]
+ *   face_interp_t[f][j][m] = face nodes (face_mask order) -> face points
+ *   lift_t[f*nfq+m][i]  = (M^-1 E_q)[i][f*nfq+m] */
 typedef struct mg_quad {
   int32_t nq, nfq;
-  const double* interp;       /* [nq][np]  nodal values -> volume points       */
-  const double* weak_diff;    /* [dim][np][nq]  M^-1 D_q,r^T W                 */
-  const double* face_interp;  /* [nfaces][nfq][nfp]  face nodes -> face points */
-  const double* lift;         /* [np][nfaces*nfq]  M^-1 E_q                    */
+  const double* interp;       /* [np][nq]            */
+  const double* weak_diff;    /* [dim][nq][np]       */
+  const double* face_interp;  /* [nfaces][nfp][nfq]  */
+  const double* lift;         /* [nfaces*nfq][np]    */
 } mg_quad;
 
 /* Euler RHS on the quadrature discretization (euler_operator with

@@ -1200,10 +1200,12 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 
 struct QuadArgs {
   int nq, nfq, epb;
-  const double* interp;   // [nq][NP]
-  const double* wdiff;    // [DIM][NP][nq]
-  const double* finterp;  // [NF][nfq][NFP]
-  const double* lift;     // [NP][NF*nfq]
+  // operators transposed so that consecutive lanes (consecutive points in
+  // phases 2-3, consecutive nodes in phase 4) read consecutive addresses
+  const double* interp;   // [NP][nq]
+  const double* wdiff;    // [DIM][nq][NP]
+  const double* finterp;  // [NF][NFP][nfq]
+  const double* lift;     // [NF*nfq][NP]
   const double* qbq;      // [F][nbq] exterior states at boundary face points
   int64_t nbq;
 };
@@ -1218,29 +1220,40 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
   double* s_q = lds;
   double* s_g = s_q + NFLD * epb * NP;
   double* s_f = s_g + NFLD * DIM * epb * nq;
+  double* s_p = s_f + NFLD * epb * nfq_all;   // [F][epb][NF*NFP] exterior
+                                              // face-node values
   const int64_t e0 = int64_t(blockIdx.x) * epb;
   const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
   const int64_t N = a.nnodes;
-  // 1: the elements' state (each field's rows are contiguous in HBM)
+  // 1: the elements' state (each field's rows are contiguous in HBM) and
+  // the neighbours' values at this element's face nodes (interior faces)
   for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
     const int f = t / (ne * NP);
     const int r = t - f * ne * NP;
     s_q[f * epb * NP + r] = a.q[f * N + e0 * NP + r];
+  }
+  for (int t = threadIdx.x; t < ne * NF * NFP; t += blockDim.x) {
+    const int le = t / (NF * NFP);
+    const int fn = t - le * NF * NFP;
+    const int32_t nb = a.vmap_p[(e0 + le) * NF * NFP + fn];
+    if (nb < 0) continue;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_p[(f * epb + le) * NF * NFP + fn] = a.q[f * N + nb];
   }
   __syncthreads();
   // 2: inviscid flux at the volume points, in reference directions
   for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
     const int le = t / nq;
     const int m = t - le * nq;
-    const double* w = qa.interp + m * NP;
+    const double* w = qa.interp + m;
     Node<DIM, NSP> s;
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) {
-      const double* u = s_q + (f * epb + le) * NP;
-      double v = 0.0;
+    for (int f = 0; f < NFLD; ++f) s.q[f] = 0.0;
 #pragma unroll
-      for (int i = 0; i < NP; ++i) v += w[i] * u[i];
-      s.q[f] = v;
+    for (int i = 0; i < NP; ++i) {
+      const double wi = w[i * nq];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s.q[f] += wi * s_q[(f * epb + le) * NP + i];
     }
     node_state<DIM, NSP, 0, false>(s, a.gas, 0.0);
     double F[NFLD][DIM];
@@ -1264,26 +1277,29 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
     const int fc = rem / nfq;
     const int m = rem - fc * nfq;
     const int64_t e = e0 + le;
-    const double* w = qa.finterp + (fc * nfq + m) * NFP;
+    const double* w = qa.finterp + fc * NFP * nfq + m;
     const int32_t* vp = a.vmap_p + (e * NF + fc) * NFP;
     const int* fm = a.fmask + fc * NFP;
     Node<DIM, NSP> sm, sp;
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) {
-      const double* u = s_q + (f * epb + le) * NP;
-      double v = 0.0;
+    for (int f = 0; f < NFLD; ++f) sm.q[f] = 0.0;
 #pragma unroll
-      for (int j = 0; j < NFP; ++j) v += w[j] * u[fm[j]];
-      sm.q[f] = v;
+    for (int j = 0; j < NFP; ++j) {
+      const double wj = w[j * nfq];
+      const int ij = fm[j];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) sm.q[f] += wj * s_q[(f * epb + le) * NP + ij];
     }
     const int32_t v0 = vp[0];
     if (v0 >= 0) {
 #pragma unroll
-      for (int f = 0; f < NFLD; ++f) {
-        double v = 0.0;
+      for (int f = 0; f < NFLD; ++f) sp.q[f] = 0.0;
 #pragma unroll
-        for (int j = 0; j < NFP; ++j) v += w[j] * a.q[f * N + vp[j]];
-        sp.q[f] = v;
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f)
+          sp.q[f] += wj * s_p[(f * epb + le) * NF * NFP + fc * NFP + j];
       }
     } else {
       const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + m;
@@ -1302,23 +1318,33 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
     for (int f = 0; f < NFLD; ++f) s_f[(f * epb + le) * nfq_all + rem] = sc * fs[f];
   }
   __syncthreads();
-  // 4: dQ/dt = sum_r Wq_r (rx . F)_r - Lq (fscale F*)
-  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
-    const int f = t / (ne * NP);
-    const int r = t - f * ne * NP;
-    const int le = r / NP;
-    const int i = r - le * NP;
-    double acc = 0.0;
+  // 4: dQ/dt = sum_r Wq_r (rx . F)_r - Lq (fscale F*), all fields of one
+  // node per lane (each operator entry read once for the NFLD fields)
+  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
+    const int le = t / NP;
+    const int i = t - le * NP;
+    double acc[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
 #pragma unroll
     for (int rr = 0; rr < DIM; ++rr) {
-      const double* wd = qa.wdiff + (rr * NP + i) * nq;
-      const double* g = s_g + ((f * DIM + rr) * epb + le) * nq;
-      for (int m = 0; m < nq; ++m) acc += wd[m] * g[m];
+      const double* wd = qa.wdiff + rr * nq * NP + i;
+      const double* g = s_g + (rr * epb + le) * nq;
+      for (int m = 0; m < nq; ++m) {
+        const double w = wd[m * NP];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) acc[f] += w * g[f * DIM * epb * nq + m];
+      }
     }
-    const double* lf = qa.lift + i * nfq_all;
-    const double* fl = s_f + (f * epb + le) * nfq_all;
-    for (int m = 0; m < nfq_all; ++m) acc -= lf[m] * fl[m];
-    a.out[f * N + e0 * NP + r] = acc;
+    const double* lf = qa.lift + i;
+    const double* fl = s_f + le * nfq_all;
+    for (int m = 0; m < nfq_all; ++m) {
+      const double w = lf[m * NP];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) acc[f] -= w * fl[f * epb * nfq_all + m];
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) a.out[f * N + e0 * NP + t] = acc[f];
   }
 }
 
@@ -2318,10 +2344,14 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
   const int nfld = 2 + mesh->dim + gas->nspecies;
   const size_t per_elem =
       sizeof(double) * size_t(nfld) *
-      (size_t(mesh->np) + size_t(mesh->dim) * quad->nq + size_t(mesh->nfaces) * quad->nfq);
+      (size_t(mesh->np) + size_t(mesh->dim) * quad->nq +
+       size_t(mesh->nfaces) * (quad->nfq + mesh->nfp));
+  // up to 80 KiB per workgroup: two workgroups per CU, which is what the
+  // kernel's registers allow (at least one element; at most 128 KiB)
   constexpr size_t kBudget = 128 * 1024;
-  int epb = int(kBudget / per_elem);
-  if (epb < 1) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS");
+  if (per_elem > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS");
+  int epb = int((80 * 1024) / per_elem);
+  if (epb < 1) epb = 1;
   if (epb > 16) epb = 16;
   mgdg::KArgs a = make_args(mesh, gas);
   a.q = q;

@@ -88,6 +88,11 @@ class DeviceDiscretization:
             f64 = dict(dtype=torch.float64, device=self.device)
             c = {k: torch.as_tensor(np.ascontiguousarray(getattr(qo, k)), **f64)
                  for k in ("interp", "weak_diff", "face_interp", "lift")}
+            # the kernel's layouts: transposed so that lanes read consecutively
+            c["interp_t"] = c["interp"].T.contiguous()
+            c["weak_diff_t"] = c["weak_diff"].transpose(1, 2).contiguous()
+            c["face_interp_t"] = c["face_interp"].transpose(1, 2).contiguous()
+            c["lift_t"] = c["lift"].T.contiguous()
             xq, nq = self.host.quad_boundary_geometry()
             c["bnd_x"] = torch.as_tensor(np.ascontiguousarray(xq), **f64)
             c["bnd_n"] = torch.as_tensor(np.ascontiguousarray(nq), **f64)
@@ -96,10 +101,10 @@ class DeviceDiscretization:
                 bf[:, 1] if len(bf) else np.zeros(0, dtype=np.int64),
                 dtype=torch.int64, device=self.device)
             c["ops"] = _lib.MgQuad(
-                nq=qo.nq, nfq=qo.nfq, interp=c["interp"].data_ptr(),
-                weak_diff=c["weak_diff"].data_ptr(),
-                face_interp=c["face_interp"].data_ptr(),
-                lift=c["lift"].data_ptr())
+                nq=qo.nq, nfq=qo.nfq, interp=c["interp_t"].data_ptr(),
+                weak_diff=c["weak_diff_t"].data_ptr(),
+                face_interp=c["face_interp_t"].data_ptr(),
+                lift=c["lift_t"].data_ptr())
             self._qcache = c
         return self._qcache
 
