@@ -139,3 +139,52 @@ def test_limiter_rejects_host_tensors():
         generate_box_mesh(2, 0.0, 1.0, (2, 2)), 2)
     with pytest.raises(TypeError):
         bound_preserving_limiter(dcoll, torch.zeros(dcoll.nelements, 6))
+
+
+def test_transport_entry_points_validation():
+    lib = _lib.load()
+    arr = (ctypes.c_double * 5)()
+    assert lib.mg_load_transport(arr, 5) == 1
+    assert b"size" in lib.mg_last_error()
+    from mirgecom_amd.transport_fits import packed_transport
+    assert packed_transport().size == 308
+    one = ctypes.c_void_p(1)
+    # mixture-averaged transport with the ideal-gas EOS is rejected
+    g = _lib.MgGas(eos_kind=0, transport_kind=3, nspecies=0)
+    assert lib.mg_transport_vars(g, 3, 10, one, one, one, one, one, one,
+                                 None) == 1
+    assert b"mixture" in lib.mg_last_error()
+    g = _lib.MgGas(eos_kind=0, transport_kind=5)
+    assert lib.mg_transport_vars(g, 3, 10, one, one, one, one, one, one,
+                                 None) == 1
+    assert b"transport_kind" in lib.mg_last_error()
+    g = _lib.MgGas(eos_kind=0, transport_kind=1, gamma=1.4, gas_const=1.0)
+    assert lib.mg_transport_vars(g, 4, 10, one, None, one, one, one, None,
+                                 None) == 1
+
+
+def test_quadrature_entry_point_validation():
+    lib = _lib.load()
+    m = _lib.MgMesh(dim=2, order=2, np=6, nfp=3, nfaces=3, nelements=4,
+                    nbnd_nodes=0, rx=1, normals=1, fscale=1, vmap_p=1,
+                    weak_diff=1, lift=1, face_mask=1, workspace=1)
+    gas = _lib.MgGas(eos_kind=0, gamma=1.4, gas_const=1.0)
+    one = ctypes.c_void_p(1)
+    # incomplete mg_quad
+    qd = _lib.MgQuad(nq=9, nfq=3, interp=1, weak_diff=None, face_interp=1,
+                     lift=1)
+    assert lib.mg_euler_rhs_quad(m, gas, qd, one, None, 0, one, None) == 1
+    assert b"mg_quad" in lib.mg_last_error()
+    qd = _lib.MgQuad(nq=9, nfq=3, interp=1, weak_diff=1, face_interp=1, lift=1)
+    # mixtures are not built for the quadrature path
+    mix = _lib.MgGas(eos_kind=1, nspecies=7)
+    assert lib.mg_euler_rhs_quad(m, mix, qd, one, None, 0, one, None) == 2
+    # boundary exterior states required when the mesh has boundary nodes
+    m.nbnd_nodes = 6
+    assert lib.mg_euler_rhs_quad(m, gas, qd, one, None, 0, one, None) == 1
+    assert b"boundary" in lib.mg_last_error()
+    # an LDS footprint above 128 KiB per element is refused
+    m.nbnd_nodes = 0
+    big = _lib.MgQuad(nq=20000, nfq=3, interp=1, weak_diff=1, face_interp=1,
+                      lift=1)
+    assert lib.mg_euler_rhs_quad(m, gas, big, one, None, 0, one, None) == 2
