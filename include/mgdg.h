@@ -258,6 +258,41 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
                             const double* q_bnd_quad, int32_t flux_kind, double* rhs,
                             void* stream);
 
+/* Hybridized summation-by-parts operators of the entropy-stable path (on
+ * the quadrature of an mg_quad; nh = nq + nfaces*nfq hybridized points:
+ * volume points, then each face's points), device arrays:
+ *   proj_t[m][i]   = (M^-1 V_q^T W)[i][m]          [nq][np]
+ *   vh_t[i][h]     = V_h[h][i]                     [np][nh]
+ *   skew_t[r][j][h] = (Q_r - Q_r^T)[h][j], Q_r = W V_q D_r P_q   [dim][nq][nq]
+ *   ext[m][h]      = E[m][h], E = V_f P_q          [nfaces*nfq][nq]
+ *   ext_col[h][m]  = E[m][h]                       [nq][nfaces*nfq]
+ *   apply_t[h][i]  = (M^-1 V_h^T)[i][h]            [nh][np]
+ *   face_weights   = the face rule's weights       [nfq]                  */
+typedef struct mg_esdg {
+  int32_t nh, pad0;
+  const double* proj;
+  const double* vh;
+  const double* skew;
+  const double* ext;
+  const double* ext_col;
+  const double* apply;
+  const double* face_weights;
+} mg_esdg;
+
+/* Entropy-stable Euler RHS by flux differencing (euler_operator with
+ * use_esdg=True and quadrature_tag=DISCR_TAG_QUAD, mirgecom/euler.py:108-310)
+ * for the ideal gas: entropy variables projected onto the element space
+ * (stored in ev_work [F][N], caller-owned), the state at the hybridized
+ * points from them, Chandrashekar's entropy-conserving two-point flux
+ * (mirgecom/inviscid.py:451-518) in (Q_h - Q_h^T) o F_S, and the
+ * entropy-stable face flux (inviscid.py:601-711): with dissipation != 0 the
+ * Rusanov penalty alpha = max wavespeed, else none.  q_bnd_quad as for
+ * mg_euler_rhs_quad. */
+mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
+                            const mg_quad* quad, const mg_esdg* esdg, const double* q,
+                            const double* q_bnd_quad, int32_t dissipation,
+                            double* ev_work, double* rhs, void* stream);
+
 /* Upload the uiuc_7sp mechanism constants (NASA7 tables, molecular weights,
  * gas constant, standard pressure, SI Arrhenius parameters; 130 doubles in
  * the order of mirgecom_amd.mechanisms.packed_mechanism) to the device.

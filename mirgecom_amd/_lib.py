@@ -58,6 +58,16 @@ class MgQuad(ctypes.Structure):
     ]
 
 
+class MgEsdg(ctypes.Structure):
+    _fields_ = [
+        ("nh", ctypes.c_int32), ("pad0", ctypes.c_int32),
+        ("proj", ctypes.c_void_p), ("vh", ctypes.c_void_p),
+        ("skew", ctypes.c_void_p), ("ext", ctypes.c_void_p),
+        ("ext_col", ctypes.c_void_p), ("apply", ctypes.c_void_p),
+        ("face_weights", ctypes.c_void_p),
+    ]
+
+
 # exported symbol -> (restype, argtypes)
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -98,6 +108,9 @@ SIGNATURES = {
     "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
     "mg_euler_rhs_quad": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  ctypes.POINTER(MgQuad), _P, _P, _I32, _P, _P]),
+    "mg_euler_rhs_esdg": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                                 ctypes.POINTER(MgQuad), ctypes.POINTER(MgEsdg),
+                                 _P, _P, _I32, _P, _P, _P]),
     "mg_bound_limiter": (_I32, [_I32, _I64, _I32, _P, _P, _P, _D, _D, _I32,
                                 _P]),
     "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),

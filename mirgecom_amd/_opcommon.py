@@ -28,9 +28,10 @@ def check_dd(dd):
 def check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
                       limiter_func, entropy_min, comm_tag=None,
                       allow_quad=False):
-    if use_esdg:
-        raise NotImplementedError("the entropy-stable (ESDG) operator is not "
-                                  "part of the device path")
+    if use_esdg and not (allow_quad and quadrature_tag == DISCR_TAG_QUAD):
+        raise NotImplementedError("the entropy-stable (ESDG) operator is built "
+                                  "for euler_operator with "
+                                  "quadrature_tag=DISCR_TAG_QUAD")
     if quadrature_tag not in ((DISCR_TAG_BASE, DISCR_TAG_QUAD) if allow_quad
                               else (DISCR_TAG_BASE,)):
         raise NotImplementedError("overintegration is built for euler_operator "

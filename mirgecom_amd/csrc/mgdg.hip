@@ -14,6 +14,7 @@
 // The reference matrices (Dw_a = M^-1 D_a^T M, LIFT = M^-1 E) live in LDS,
 // stored transposed so that consecutive nodes read consecutive banks.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include <cstdio>
 #include <cstring>
@@ -1350,6 +1351,276 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
 
 // }}}
 
+// {{{ entropy-stable Euler RHS (flux differencing, ideal gas)
+//
+// esdg_project_kernel: projected entropy variables v~ = P_q v(V_q u) per
+// element (grudge's volume_quadrature_project), written to ev.
+// esdg_kernel: the state at the nh hybridized points from v~ (own element)
+// and the face fluxes from both sides' v~; each lane then owns one
+// hybridized point h of one element and sums over the points j its row of
+// (Q_h - Q_h^T) o F_S (volume rows: all nh points, face rows: the nq volume
+// points -- the face-face block vanishes); the sums are mapped back with
+// M^-1 V_h^T.  Geometric factors enter as rx and n * fscale (the Jacobian
+// cancels against the mass matrix on affine elements).
+
+struct EsdgArgs {
+  int nq, nfq, nh, epb;
+  const double* interp;   // [NP][nq]
+  const double* proj;     // [nq][NP]  P_q transposed
+  const double* vh;       // [NP][nh]  V_h transposed
+  const double* skew;     // [DIM][nq(j)][nq(h)]: (Q_r - Q_r^T)[h][j]
+  const double* ext;      // [NF*nfq][nq]  E
+  const double* ext_col;  // [nq][NF*nfq]  E transposed
+  const double* apply;    // [nh][NP]  (M^-1 V_h^T) transposed
+  const double* finterp;  // [NF][NFP][nfq]
+  const double* wf;       // [nfq]
+  const double* qbq;      // [F][nbq]
+  int64_t nbq;
+  double* ev;             // [F][N] projected entropy variables
+  int dissipation;
+};
+
+template <int DIM, int P, int NSP>
+__global__ __launch_bounds__(kBlock) void esdg_project_kernel(KArgs a, EsdgArgs ea) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP;
+  constexpr int NFLD = 2 + DIM + NSP;
+  extern __shared__ double lds[];
+  const int nq = ea.nq, epb = ea.epb;
+  double* s_e = lds;  // [F][epb][nq]
+  const int64_t e0 = int64_t(blockIdx.x) * epb;
+  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
+  const int64_t N = a.nnodes;
+  const double gamma = a.gas.gamma;
+  for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
+    const int le = t / nq;
+    const int m = t - le * nq;
+    const int64_t base = (e0 + le) * NP;
+    double u[NFLD], v[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) u[f] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const double w = ea.interp[i * nq + m];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) u[f] += w * a.q[f * N + base + i];
+    }
+    cons_to_entropy<DIM, NSP>(u, gamma, v);
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_e[(f * epb + le) * nq + m] = v[f];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
+    const int le = t / NP;
+    const int i = t - le * NP;
+    double v[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
+    for (int m = 0; m < nq; ++m) {
+      const double w = ea.proj[m * NP + i];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) v[f] += w * s_e[(f * epb + le) * nq + m];
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) ea.ev[f * N + e0 * NP + t] = v[f];
+  }
+}
+
+template <int DIM, int P, int NSP>
+__global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + NSP;
+  using Prim = EcPrim<DIM, NSP>;
+  extern __shared__ double lds[];
+  const int nq = ea.nq, nfq = ea.nfq, nh = ea.nh, epb = ea.epb, nft = NF * ea.nfq;
+  Prim* s_prim = reinterpret_cast<Prim*>(lds);             // [epb][nh]
+  double* s_v = reinterpret_cast<double*>(s_prim + epb * nh);  // [F][epb][NP]
+  double* s_q = s_v + NFLD * epb * NP;                      // [F][epb][NP]
+  double* s_vp = s_q + NFLD * epb * NP;                     // [F][epb][NF*NFP]
+  double* s_qf = s_vp + NFLD * epb * NF * NFP;              // [F][epb][nft]
+  double* s_fs = s_qf + NFLD * epb * nft;                   // [F][epb][nft]
+  double* s_acc = s_fs + NFLD * epb * nft;                  // [F][epb][nh]
+  const int64_t e0 = int64_t(blockIdx.x) * epb;
+  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
+  const int64_t N = a.nnodes;
+  const double gamma = a.gas.gamma;
+  // 1: own v~ and q, the neighbours' v~ at this element's face nodes
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int f = t / (ne * NP);
+    const int r = t - f * ne * NP;
+    s_v[f * epb * NP + r] = ea.ev[f * N + e0 * NP + r];
+    s_q[f * epb * NP + r] = a.q[f * N + e0 * NP + r];
+  }
+  for (int t = threadIdx.x; t < ne * NF * NFP; t += blockDim.x) {
+    const int le = t / (NF * NFP);
+    const int fn = t - le * NF * NFP;
+    const int32_t nb = a.vmap_p[(e0 + le) * NF * NFP + fn];
+    if (nb < 0) continue;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f)
+      s_vp[(f * epb + le) * NF * NFP + fn] = ea.ev[f * N + nb];
+  }
+  __syncthreads();
+  // 2: the state at the hybridized points
+  for (int t = threadIdx.x; t < ne * nh; t += blockDim.x) {
+    const int le = t / nh;
+    const int h = t - le * nh;
+    double v[NFLD], u[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const double w = ea.vh[i * nh + h];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) v[f] += w * s_v[(f * epb + le) * NP + i];
+    }
+    entropy_to_cons<DIM, NSP>(v, gamma, u);
+    ec_prim<DIM, NSP>(u, gamma, s_prim[le * nh + h]);
+    if (h >= nq) {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s_qf[(f * epb + le) * nft + (h - nq)] = u[f];
+    }
+  }
+  __syncthreads();
+  // 3: entropy-stable face flux at the face points, times fscale * w
+  for (int t = threadIdx.x; t < ne * nft; t += blockDim.x) {
+    const int le = t / nft;
+    const int m = t - le * nft;
+    const int fc = m / nfq;
+    const int mm = m - fc * nfq;
+    const int64_t e = e0 + le;
+    const double* w = ea.finterp + fc * NFP * nfq + mm;
+    const int32_t v0 = a.vmap_p[(e * NF + fc) * NFP];
+    double qm[NFLD], qp[NFLD];
+    if (v0 >= 0) {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) qm[f] = s_qf[(f * epb + le) * nft + m];
+      double vp[NFLD];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) vp[f] = 0.0;
+#pragma unroll
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f)
+          vp[f] += wj * s_vp[(f * epb + le) * NF * NFP + fc * NFP + j];
+      }
+      entropy_to_cons<DIM, NSP>(vp, gamma, qp);
+    } else {
+      // boundary: the interpolated interior state (through the entropy
+      // variables and back, as the reference's projection does) and the
+      // boundary condition's exterior state
+      double ui[NFLD], vi[NFLD];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) ui[f] = 0.0;
+#pragma unroll
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+        const int ij = a.fmask[fc * NFP + j];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) ui[f] += wj * s_q[(f * epb + le) * NP + ij];
+      }
+      cons_to_entropy<DIM, NSP>(ui, gamma, vi);
+      entropy_to_cons<DIM, NSP>(vi, gamma, qm);
+      const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + mm;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) qp[f] = ea.qbq[f * ea.nbq + bq];
+    }
+    Prim pm, pp;
+    ec_prim<DIM, NSP>(qm, gamma, pm);
+    ec_prim<DIM, NSP>(qp, gamma, pp);
+    double n[DIM], fs[NFLD];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) fs[f] = 0.0;
+    ec_flux_dot<DIM, NSP>(pm, pp, n, gamma, fs);
+    if (ea.dissipation) {
+      const double alpha = fmax(pm.c, pp.c);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) fs[f] -= 0.5 * alpha * (qp[f] - qm[f]);
+    }
+    const double sc = a.fscale[e * NF + fc] * ea.wf[mm];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_fs[(f * epb + le) * nft + m] = sc * fs[f];
+  }
+  __syncthreads();
+  // 4: rows of (Q_h - Q_h^T) o F_S, plus the face flux on face rows
+  for (int t = threadIdx.x; t < ne * nh; t += blockDim.x) {
+    const int le = t / nh;
+    const int h = t - le * nh;
+    const int64_t e = e0 + le;
+    const Prim ph = s_prim[le * nh + h];
+    const Prim* pj = s_prim + le * nh;
+    double rx[DIM * DIM];
+#pragma unroll
+    for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
+    double acc[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
+    if (h < nq) {
+      for (int j = 0; j < nq; ++j) {
+        double sr[DIM], sd[DIM];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) sr[r] = ea.skew[(r * nq + j) * nq + h];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          double v = 0.0;
+#pragma unroll
+          for (int r = 0; r < DIM; ++r) v += rx[r * DIM + d] * sr[r];
+          sd[d] = v;
+        }
+        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
+      }
+      for (int m = 0; m < nft; ++m) {
+        const int fc = m / nfq;
+        const double b = ea.ext[m * nq + h] * a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
+        double sd[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sd[d] = b * a.normals[(e * NF + fc) * DIM + d];
+        ec_flux_dot<DIM, NSP>(ph, pj[nq + m], sd, gamma, acc);
+      }
+    } else {
+      const int m = h - nq;
+      const int fc = m / nfq;
+      const double bsc = -a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
+      double nrm[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) nrm[d] = bsc * a.normals[(e * NF + fc) * DIM + d];
+      for (int j = 0; j < nq; ++j) {
+        const double ej = ea.ext_col[j * nft + m];
+        double sd[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sd[d] = ej * nrm[d];
+        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
+      }
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) acc[f] += s_fs[(f * epb + le) * nft + m];
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_acc[(f * epb + le) * nh + h] = acc[f];
+  }
+  __syncthreads();
+  // 5: dQ/dt = -M^-1 V_h^T acc
+  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
+    const int le = t / NP;
+    const int i = t - le * NP;
+    double out[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) out[f] = 0.0;
+    for (int h = 0; h < nh; ++h) {
+      const double w = ea.apply[h * NP + i];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) out[f] += w * s_acc[(f * epb + le) * nh + h];
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) a.out[f * N + e0 * NP + t] = -out[f];
+  }
+}
+
+// }}}
+
 // {{{ nodal net flux from stored gradients
 //
 // The pointwise phases (5 and 6) of grad_kernel<FLUX> as node-parallel
@@ -2382,6 +2653,83 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
   MG_QCASE(2, 1, 7) MG_QCASE(2, 2, 7) MG_QCASE(2, 3, 7) MG_QCASE(2, 4, 7)
   MG_QCASE(3, 1, 7) MG_QCASE(3, 2, 7) MG_QCASE(3, 3, 7) MG_QCASE(3, 4, 7)
 #undef MG_QCASE
+  return fail(MG_ERR_UNSUPPORTED, "dim/order");
+}
+
+mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
+                            const mg_quad* quad, const mg_esdg* esdg, const double* q,
+                            const double* q_bnd_quad, int32_t dissipation,
+                            double* ev_work, double* rhs, void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, false))) return s;
+  if (gas->eos_kind != 0)
+    return fail(MG_ERR_UNSUPPORTED, "the entropy-stable path is built for the ideal gas");
+  if (gas->nspecies != 0 && gas->nspecies != 7)
+    return fail(MG_ERR_UNSUPPORTED, "nspecies must be 0 or 7");
+  if (!quad || !quad->interp || !quad->face_interp || quad->nq < 1 || quad->nfq < 1)
+    return fail(MG_ERR_ARG, "mg_quad incomplete");
+  if (!esdg || !esdg->proj || !esdg->vh || !esdg->skew || !esdg->ext || !esdg->ext_col ||
+      !esdg->apply || !esdg->face_weights ||
+      esdg->nh != quad->nq + mesh->nfaces * quad->nfq)
+    return fail(MG_ERR_ARG, "mg_esdg incomplete or nh != nq + nfaces*nfq");
+  if (!q || !rhs || !ev_work) return fail(MG_ERR_ARG, "q/rhs/ev_work NULL");
+  if (mesh->nbnd_nodes > 0 && !q_bnd_quad)
+    return fail(MG_ERR_ARG, "boundary exterior state missing");
+  if (mesh->nbnd_nodes % mesh->nfp) return fail(MG_ERR_ARG, "nbnd_nodes % nfp");
+  const int nfld = 2 + mesh->dim + gas->nspecies;
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.out = rhs;
+  mgdg::EsdgArgs ea{};
+  ea.nq = quad->nq;
+  ea.nfq = quad->nfq;
+  ea.nh = esdg->nh;
+  ea.interp = quad->interp;
+  ea.finterp = quad->face_interp;
+  ea.proj = esdg->proj;
+  ea.vh = esdg->vh;
+  ea.skew = esdg->skew;
+  ea.ext = esdg->ext;
+  ea.ext_col = esdg->ext_col;
+  ea.apply = esdg->apply;
+  ea.wf = esdg->face_weights;
+  ea.qbq = q_bnd_quad;
+  ea.nbq = mesh->nbnd_nodes / mesh->nfp * quad->nfq;
+  ea.ev = ev_work;
+  ea.dissipation = dissipation ? 1 : 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  constexpr size_t kBudget = 128 * 1024;
+  const size_t nft = size_t(mesh->nfaces) * quad->nfq;
+#define MG_ECASE(D, P, NS)                                                              \
+  if (mesh->dim == D && mesh->order == P && gas->nspecies == NS) {                      \
+    const size_t per_p = sizeof(double) * nfld * quad->nq;                              \
+    if (per_p > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
+    int epb_p = int(std::max<size_t>(1, std::min<size_t>(8, 256 / quad->nq)));        \
+    ea.epb = epb_p;                                                                     \
+    auto kp = mgdg::esdg_project_kernel<D, P, NS>;                                      \
+    if ((s = set_lds(kp, kBudget))) return s;                                           \
+    hipLaunchKernelGGL(kp, dim3(unsigned((a.K + epb_p - 1) / epb_p)),                   \
+                       dim3(mgdg::kBlock), per_p * epb_p, st, a, ea);                  \
+    if ((s = check_launch("esdg_project_kernel"))) return s;                            \
+    const size_t per_e = sizeof(mgdg::EcPrim<D, NS>) * esdg->nh +                       \
+        sizeof(double) * nfld *                                                         \
+            (2 * size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp + 2 * nft +        \
+             size_t(esdg->nh));                                                         \
+    if (per_e > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
+    int epb = int(std::max<size_t>(1, std::min<size_t>(16, (64 * 1024) / per_e)));     \
+    ea.epb = epb;                                                                       \
+    auto kern = mgdg::esdg_kernel<D, P, NS>;                                            \
+    if ((s = set_lds(kern, kBudget))) return s;                                         \
+    hipLaunchKernelGGL(kern, dim3(unsigned((a.K + epb - 1) / epb)), dim3(mgdg::kBlock), \
+                       per_e * epb, st, a, ea);                                         \
+    return check_launch("esdg_kernel");                                                 \
+  }
+  MG_ECASE(2, 1, 0) MG_ECASE(2, 2, 0) MG_ECASE(2, 3, 0) MG_ECASE(2, 4, 0)
+  MG_ECASE(3, 1, 0) MG_ECASE(3, 2, 0) MG_ECASE(3, 3, 0) MG_ECASE(3, 4, 0)
+  MG_ECASE(2, 1, 7) MG_ECASE(2, 2, 7) MG_ECASE(2, 3, 7) MG_ECASE(2, 4, 7)
+  MG_ECASE(3, 1, 7) MG_ECASE(3, 2, 7) MG_ECASE(3, 3, 7) MG_ECASE(3, 4, 7)
+#undef MG_ECASE
   return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }
 

@@ -387,6 +387,121 @@ __device__ __forceinline__ void inviscid_flux_n(const Node<DIM, NSP>& s,
   for (int k = 0; k < NSP; ++k) fn[2 + DIM + k] = vn * s.q[2 + DIM + k];
 }
 
+// {{{ entropy-stable DG pieces (ideal gas)
+
+// Entropy variables (mirgecom/gas_model.py:1026-1060, Chandrashekar 2013
+// eq. 4.5) and their inverse (gas_model.py:1063-1100, Hughes, Franca &
+// Mallet 1986).
+template <int DIM, int NSP>
+__device__ __forceinline__ void cons_to_entropy(const double* q, double gamma,
+                                                double* v) {
+  const double rho = q[0];
+  const double irho = 1.0 / rho;
+  double u2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) u2 += (q[2 + d] * irho) * (q[2 + d] * irho);
+  const double p = (gamma - 1.0) * (q[1] - 0.5 * rho * u2);
+  const double s = log(p) - gamma * log(rho);
+  const double rho_p = rho / p;
+  v[0] = (gamma - s) / (gamma - 1.0) - 0.5 * rho_p * u2;
+  v[1] = -rho_p;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) v[2 + d] = rho_p * (q[2 + d] * irho);
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) v[2 + DIM + k] = q[2 + DIM + k] * irho / (gamma - 1.0);
+}
+
+template <int DIM, int NSP>
+__device__ __forceinline__ void entropy_to_cons(const double* ev, double gamma,
+                                                double* q) {
+  const double gm1 = gamma - 1.0;
+  const double v1 = ev[0] * gm1, v5 = ev[1] * gm1;
+  double vv = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) vv += (ev[2 + d] * gm1) * (ev[2 + d] * gm1);
+  const double s = gamma - v1 + vv / (2.0 * v5);
+  const double iota = pow(gm1 / pow(-v5, gamma), 1.0 / gm1);
+  const double rho_iota = iota * exp(-s / gm1);
+  const double mass = -rho_iota * v5;
+  q[0] = mass;
+  q[1] = rho_iota * (1.0 - vv / (2.0 * v5));
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) q[2 + d] = rho_iota * (ev[2 + d] * gm1);
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) q[2 + DIM + k] = mass * (ev[2 + DIM + k] * gm1);
+}
+
+// Per-point quantities of the entropy-conserving flux, with the logarithms
+// of rho and beta = rho/(2p) taken once per point.
+template <int DIM, int NSP>
+struct EcPrim {
+  double rho, lrho, beta, lbeta, kin, p, c;
+  double u[DIM];
+  double y[NSP > 0 ? NSP : 1];
+};
+
+template <int DIM, int NSP>
+__device__ __forceinline__ void ec_prim(const double* q, double gamma,
+                                        EcPrim<DIM, NSP>& a) {
+  a.rho = q[0];
+  const double irho = 1.0 / q[0];
+  double u2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    a.u[d] = q[2 + d] * irho;
+    u2 += a.u[d] * a.u[d];
+  }
+  a.kin = 0.5 * u2;
+  a.p = (gamma - 1.0) * (q[1] - 0.5 * q[0] * u2);
+  a.beta = 0.5 * q[0] / a.p;
+  a.lrho = log(a.rho);
+  a.lbeta = log(a.beta);
+  a.c = sqrt(u2) + sqrt(gamma * a.p * irho);   // wavespeed |u| + c
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) a.y[k] = q[2 + DIM + k] * irho;
+}
+
+// Logarithmic mean (mirgecom/inviscid.py:470-476) from the values and their
+// logarithms.
+__device__ __forceinline__ double ln_mean(double x, double lx, double y, double ly) {
+  const double f2 = (x * (x - 2.0 * y) + y * y) / (x * (x + 2.0 * y) + y * y);
+  if (f2 < 1e-4)
+    return (x + y) / (2.0 + f2 * 2.0 / 3.0 + f2 * f2 * 2.0 / 5.0 +
+                      f2 * f2 * f2 * 2.0 / 7.0);
+  return (y - x) / (ly - lx);
+}
+
+// sum_d s_d F_S,d(a, b) of Chandrashekar's entropy-conserving, kinetic-
+// energy-preserving flux (mirgecom/inviscid.py:451-518), added to acc.
+template <int DIM, int NSP>
+__device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
+                                            const EcPrim<DIM, NSP>& b,
+                                            const double* s, double gamma,
+                                            double* acc) {
+  const double rho_mean = ln_mean(a.rho, a.lrho, b.rho, b.lrho);
+  const double beta_mean = ln_mean(a.beta, a.lbeta, b.beta, b.lbeta);
+  const double rho_avg = 0.5 * (a.rho + b.rho);
+  const double beta_avg = 0.5 * (a.beta + b.beta);
+  const double p_mean = 0.5 * rho_avg / beta_avg;
+  double un = 0.0, ua2 = 0.0, ua[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    ua[d] = 0.5 * (a.u[d] + b.u[d]);
+    un += s[d] * ua[d];
+    ua2 += ua[d] * ua[d];
+  }
+  const double mflux = rho_mean * un;
+  acc[0] += mflux;
+  acc[1] += mflux * (0.5 * (1.0 / ((gamma - 1.0) * beta_mean) - (a.kin + b.kin)) + ua2) +
+            p_mean * un;
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) acc[2 + i] += mflux * ua[i] + s[i] * p_mean;
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) acc[2 + DIM + k] += mflux * 0.5 * (a.y[k] + b.y[k]);
+}
+
+// }}}
+
 // Numerical inviscid flux: 0 Rusanov, 1 HLL, 2 central.
 template <int DIM, int NSP>
 __device__ __forceinline__ void inviscid_numflux(const Node<DIM, NSP>& a,

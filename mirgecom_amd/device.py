@@ -108,6 +108,27 @@ class DeviceDiscretization:
             self._qcache = c
         return self._qcache
 
+    def esdg_ops(self):
+        """The ``mg_esdg`` struct of the hybridized SBP operators."""
+        c = self._quad()
+        if "esdg" not in c:
+            from mirgecom_amd.refelem import get_hybrid_operators
+            host = self.host
+            ho = get_hybrid_operators(host.dim, host.order, host.quadrature_order)
+            f64 = dict(dtype=torch.float64, device=self.device)
+
+            def t(a):
+                return torch.as_tensor(np.ascontiguousarray(a), **f64)
+            c["esdg_arrays"] = arr = {
+                "proj": t(ho.proj.T), "vh": t(ho.vh.T),
+                "skew": t(np.transpose(ho.skew, (0, 2, 1))),
+                "ext": t(ho.extrap), "ext_col": t(ho.extrap.T),
+                "apply": t(ho.apply.T),
+                "face_weights": t(host.quadrature.face_weights)}
+            c["esdg"] = _lib.MgEsdg(
+                nh=ho.nh, pad0=0, **{k: v.data_ptr() for k, v in arr.items()})
+        return c["esdg"]
+
     def quad_ops(self):
         """The ``mg_quad`` struct of the quadrature operators."""
         return self._quad()["ops"]

@@ -29,12 +29,18 @@ def euler_operator(dcoll, state, gas_model, boundaries, time=0.0,
     Same arguments and errors as the reference (mirgecom/euler.py:313-372).
     ``quadrature_tag=DISCR_TAG_QUAD`` (overintegration, ideal gas) runs
     ``mg_euler_rhs_quad``: fluxes formed and integrated at the quadrature
-    points of ``dcoll.quadrature``.  ESDG and limiters are outside the device
+    points of ``dcoll.quadrature``; with ``use_esdg=True`` as well, the
+    entropy-stable flux-differencing operator (``mg_euler_rhs_esdg``,
+    mirgecom/euler.py:108-310; ideal gas).  Limiters are outside the device
     path and raise ``NotImplementedError``."""
     check_dd(dd)
     check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
                       limiter_func, entropy_min, allow_quad=True)
     warn_mixture_limiter(state, limiter_func)
+    if use_esdg:
+        return _euler_esdg(dcoll, state, gas_model, boundaries, time,
+                           inviscid_numerical_flux_func,
+                           entropy_conserving_flux_func)
     if inviscid_numerical_flux_func is None:
         warn("inviscid_numerical_flux_func unspecified, defaulting to "
              "inviscid_facial_flux_rusanov.", stacklevel=2)
@@ -67,4 +73,46 @@ def _euler_quad(dcoll, state, gas_model, boundaries, time, kind):
               gas_model.device_params(state.cv.nspecies, with_transport=False),
               dev.quad_ops(), _lib.ptr(q), _lib.ptr(qbq), kind, _lib.ptr(rhs),
               _lib.stream_handle())
+    return ConservedVars(state.dim, rhs)
+
+
+def _euler_esdg(dcoll, state, gas_model, boundaries, time, numerical_flux_func,
+                entropy_conserving_flux_func):
+    from mirgecom_amd._opcommon import quad_boundary_states
+    from mirgecom_amd.device import get_device_discretization
+    from mirgecom_amd.inviscid import (
+        entropy_conserving_flux_chandrashekar, entropy_stable_inviscid_facial_flux,
+        entropy_stable_inviscid_facial_flux_rusanov)
+    if getattr(gas_model.eos, "is_mixture", False):
+        raise NotImplementedError("the entropy-stable operator is built for the "
+                                  "ideal gas")
+    if entropy_conserving_flux_func not in (
+            None, entropy_conserving_flux_chandrashekar):
+        raise NotImplementedError("entropy_conserving_flux_chandrashekar is the "
+                                  "built-in entropy-conserving flux")
+    if numerical_flux_func is None:
+        warn("No inviscid_numerical_flux_func was given for ESDG. Automatically "
+             "setting facial flux to entropy-stable Rusanov "
+             "(entropy_stable_inviscid_facial_flux_rusanov).", stacklevel=3)
+        numerical_flux_func = entropy_stable_inviscid_facial_flux_rusanov
+    if numerical_flux_func is entropy_stable_inviscid_facial_flux_rusanov:
+        dissipation = 1
+    elif numerical_flux_func is entropy_stable_inviscid_facial_flux:
+        dissipation = 0
+    else:
+        raise NotImplementedError("ESDG takes entropy_stable_inviscid_facial_flux"
+                                  "_rusanov or entropy_stable_inviscid_facial_flux")
+    q = state.cv.q
+    if not q.is_cuda or q.dtype != torch.float64:
+        raise TypeError("the state must be a float64 tensor on the GPU")
+    dev = get_device_discretization(dcoll, q.device)
+    q = q.contiguous()
+    qbq = quad_boundary_states(dcoll, dev, q, state.dim, gas_model, boundaries,
+                               time)
+    ev = torch.empty_like(q)
+    rhs = torch.empty_like(q)
+    _lib.call("mg_euler_rhs_esdg", dev.mesh,
+              gas_model.device_params(state.cv.nspecies, with_transport=False),
+              dev.quad_ops(), dev.esdg_ops(), _lib.ptr(q), _lib.ptr(qbq),
+              dissipation, _lib.ptr(ev), _lib.ptr(rhs), _lib.stream_handle())
     return ConservedVars(state.dim, rhs)

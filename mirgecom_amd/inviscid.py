@@ -72,6 +72,72 @@ def inviscid_facial_flux_hll(state_pair, gas_model, normal):
         s_i.cv.q, s_e.cv.q, s_minus, s_plus))
 
 
+def _ln_mean(x, y, epsilon=1e-4):
+    """mirgecom/inviscid.py:470-476"""
+    f2 = (x * (x - 2 * y) + y * y) / (x * (x + 2 * y) + y * y)
+    series = (x + y) / (2 + f2 * 2 / 3 + f2 * f2 * 2 / 5 + f2 * f2 * f2 * 2 / 7)
+    safe = torch.where(f2 < epsilon, torch.full_like(x, 2.0), y / x)
+    return torch.where(f2 < epsilon, series, (y - x) / torch.log(safe))
+
+
+def entropy_conserving_flux_chandrashekar(gas_model, state_ll, state_rr):
+    """mirgecom/inviscid.py:451-518: the two-point flux ``(F, dim, ...)``
+    (Chandrashekar 2013, eqs. 4.12-4.14)."""
+    dim = state_ll.dim
+    g_ll = gas_model.eos.gamma(state_ll.cv, state_ll.temperature)
+    g_rr = gas_model.eos.gamma(state_rr.cv, state_rr.temperature)
+    rho_l, rho_r = state_ll.mass_density, state_rr.mass_density
+    u_l, u_r = state_ll.velocity, state_rr.velocity
+    beta_l = 0.5 * rho_l / state_ll.pressure
+    beta_r = 0.5 * rho_r / state_rr.pressure
+    rho_avg = 0.5 * (rho_l + rho_r)
+    rho_mean = _ln_mean(rho_l, rho_r)
+    beta_mean = _ln_mean(beta_l, beta_r)
+    beta_avg = 0.5 * (beta_l + beta_r)
+    u_avg = 0.5 * (u_l + u_r)
+    p_mean = 0.5 * rho_avg / beta_avg
+    vsq_avg = 0.5 * (u_l * u_l).sum(dim=0) + 0.5 * (u_r * u_r).sum(dim=0)
+    mass = rho_mean * u_avg
+    mom = torch.stack([mass[i] * u_avg + torch.stack(
+        [p_mean if j == i else 0 * p_mean for j in range(dim)])
+        for i in range(dim)])
+    gamma = 0.5 * (g_ll + g_rr)
+    energy = (mass * 0.5 * (1 / (gamma - 1) / beta_mean - vsq_avg)
+              + torch.einsum("ij...,j...->i...", mom, u_avg))
+    rows = [mass, energy] + [mom[i] for i in range(dim)]
+    y_mean = 0.5 * (state_ll.species_mass_fractions
+                    + state_rr.species_mass_fractions)
+    rows += [rho_mean * y_mean[k] * u_avg for k in range(state_ll.nspecies)]
+    return ConservedVars(dim, torch.stack(rows))
+
+
+def entropy_stable_inviscid_facial_flux(state_pair, gas_model, normal,
+                                        entropy_conserving_flux_func=None,
+                                        alpha=None):
+    """mirgecom/inviscid.py:601-662: ``F_S(q-, q+) . n``, penalised by
+    ``alpha/2 (q+ - q-)`` when ``alpha`` is given."""
+    if entropy_conserving_flux_func is None:
+        entropy_conserving_flux_func = entropy_conserving_flux_chandrashekar
+    flux = entropy_conserving_flux_func(gas_model, state_pair.int,
+                                        state_pair.ext).q
+    if alpha is not None:
+        jump = state_pair.ext.cv.q - state_pair.int.cv.q
+        flux = flux - 0.5 * alpha * jump[:, None] * normal[None]
+    return ConservedVars(state_pair.int.dim,
+                         torch.einsum("fi...,i...->f...", flux, normal))
+
+
+def entropy_stable_inviscid_facial_flux_rusanov(state_pair, gas_model, normal,
+                                                entropy_conserving_flux_func=None,
+                                                **kwargs):
+    """mirgecom/inviscid.py:665-711: the entropy-stable flux with the
+    Rusanov penalty ``alpha = max(|v| + c)``."""
+    alpha = torch.maximum(state_pair.int.wavespeed, state_pair.ext.wavespeed)
+    return entropy_stable_inviscid_facial_flux(
+        state_pair, gas_model, normal,
+        entropy_conserving_flux_func=entropy_conserving_flux_func, alpha=alpha)
+
+
 _FLUX_KIND = {
     inviscid_facial_flux_rusanov: _lib.MG_FLUX_RUSANOV,
     inviscid_facial_flux_hll: _lib.MG_FLUX_HLL,

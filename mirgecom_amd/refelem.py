@@ -593,3 +593,48 @@ def get_quadrature_operators(dim: int, order: int,
         face_interp=face_interp, face_xref=face_xref, lift=re.inv_mass @ emb)
 
 # }}}
+
+
+# {{{ hybridized SBP operators (entropy-stable DG)
+
+@dataclass(frozen=True)
+class HybridOperators:
+    """Reference-element pieces of the hybridized summation-by-parts
+    operators that grudge's flux differencing uses (Chan, J. Comput. Phys.
+    362, 2018), on the quadrature of ``get_quadrature_operators``:
+
+    * ``vh (nh, Np)``: nodal values -> the ``nh = nq + Nf*nfq`` hybridized
+      points (volume points, then each face's points)
+    * ``proj (Np, nq)``: ``P_q = M^-1 V_q^T W`` (L2 projection)
+    * ``skew (dim, nq, nq)``: ``Q_r - Q_r^T`` with ``Q_r = W V_q D_r P_q``
+    * ``extrap (Nf*nfq, nq)``: ``E = V_f P_q``
+    * ``apply (Np, nh)``: ``M^-1 V_h^T``
+    """
+
+    vh: np.ndarray
+    proj: np.ndarray
+    skew: np.ndarray
+    extrap: np.ndarray
+    apply: np.ndarray
+
+    @property
+    def nh(self):
+        return self.vh.shape[0]
+
+
+@lru_cache(maxsize=None)
+def get_hybrid_operators(dim: int, order: int, degree: int) -> HybridOperators:
+    re = get_reference_element(dim, order)
+    qo = get_quadrature_operators(dim, order, degree)
+    vinv = np.linalg.inv(re.vdm)
+    vq = qo.interp
+    vf = np.concatenate([simplex_basis(dim, order, qo.face_xref[f]) @ vinv
+                         for f in range(dim + 1)])
+    proj = re.inv_mass @ vq.T * qo.weights[None, :]
+    qr = np.stack([(qo.weights[:, None] * vq) @ re.diff[r] @ proj
+                   for r in range(dim)])
+    vh = np.concatenate([vq, vf])
+    return HybridOperators(vh=vh, proj=proj, skew=qr - np.transpose(qr, (0, 2, 1)),
+                           extrap=vf @ proj, apply=re.inv_mass @ vh.T)
+
+# }}}
