@@ -204,7 +204,8 @@ def measure_reacting(args, n):
 
 def measure_overintegration(args, dcoll, gm, cv):
     """Euler RHS of the same state on the nodal and on the quadrature
-    (degree 2p) discretization, FP64 (overintegration's cost)."""
+    (degree 2p) discretization, and the entropy-stable flux-differencing
+    operator on that quadrature, FP64."""
     import torch
 
     from mirgecom_amd import _lib
@@ -212,10 +213,16 @@ def measure_overintegration(args, dcoll, gm, cv):
     from mirgecom_amd.euler import euler_operator
     from mirgecom_amd.gas_model import make_fluid_state
     from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+    from mirgecom_amd.inviscid import entropy_stable_inviscid_facial_flux_rusanov
     st = make_fluid_state(cv, _inviscid_model(gm))
     out = {}
-    for tag in (DISCR_TAG_BASE, DISCR_TAG_QUAD):
+    for tag in (DISCR_TAG_BASE, DISCR_TAG_QUAD, "esdg"):
         def go():
+            if tag == "esdg":
+                return euler_operator(
+                    dcoll, st, st.gas_model, {}, use_esdg=True,
+                    inviscid_numerical_flux_func=entropy_stable_inviscid_facial_flux_rusanov,
+                    quadrature_tag=DISCR_TAG_QUAD)
             return euler_operator(dcoll, st, st.gas_model, {},
                                   inviscid_numerical_flux_func=inviscid_facial_flux_rusanov,
                                   quadrature_tag=tag)
@@ -235,7 +242,10 @@ def measure_overintegration(args, dcoll, gm, cv):
                        f"quadrature of degree {qo.degree} ({qo.nq} volume, "
                        f"{qo.nfq} points per face)",
            "nodal_ms": out[DISCR_TAG_BASE].get("mg_euler_rhs"),
-           "quad_ms": out[DISCR_TAG_QUAD].get("mg_euler_rhs_quad")}
+           "quad_ms": out[DISCR_TAG_QUAD].get("mg_euler_rhs_quad"),
+           "esdg_ms": out["esdg"].get("mg_euler_rhs_esdg"),
+           "esdg_workload": "entropy-stable flux differencing on the same "
+                            "quadrature (entropy-stable Rusanov faces)"}
     print(f"# overintegration: {json.dumps(res)}", file=sys.stderr, flush=True)
     return res
 

@@ -1438,9 +1438,12 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   double* s_v = reinterpret_cast<double*>(s_prim + epb * nh);  // [F][epb][NP]
   double* s_q = s_v + NFLD * epb * NP;                      // [F][epb][NP]
   double* s_vp = s_q + NFLD * epb * NP;                     // [F][epb][NF*NFP]
-  double* s_qf = s_vp + NFLD * epb * NF * NFP;              // [F][epb][nft]
-  double* s_fs = s_qf + NFLD * epb * nft;                   // [F][epb][nft]
-  double* s_acc = s_fs + NFLD * epb * nft;                  // [F][epb][nh]
+  // phase 4 splits volume rows into C pieces of nq entries
+  const int C = (nh + nq - 1) / nq;
+  const int nacc = C * nq + nft;                            // row sums per field
+  double* s_fs = s_vp + NFLD * epb * NF * NFP;              // [F][epb][nft]
+  double* s_qf = s_fs + NFLD * epb * nft;                   // [F][epb][nft], phases 2-3
+  double* s_acc = s_qf;                                     // [F][epb][nacc], phases 4-5
   const int64_t e0 = int64_t(blockIdx.x) * epb;
   const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
   const int64_t N = a.nnodes;
@@ -1546,21 +1549,38 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
     for (int f = 0; f < NFLD; ++f) s_fs[(f * epb + le) * nft + m] = sc * fs[f];
   }
   __syncthreads();
-  // 4: rows of (Q_h - Q_h^T) o F_S, plus the face flux on face rows
-  for (int t = threadIdx.x; t < ne * nh; t += blockDim.x) {
-    const int le = t / nh;
-    const int h = t - le * nh;
+  // 4: rows of (Q_h - Q_h^T) o F_S, plus the face flux on face rows.  A
+  // volume row has nh entries, a face row nq (the face-face block is zero):
+  // volume rows are cut into C = ceil(nh / nq) pieces of nq entries so that
+  // every work item sums nq pairs; the pieces are added in phase 5.
+  const int nitems = nq * C + nft;
+  for (int t = threadIdx.x; t < ne * nitems; t += blockDim.x) {
+    const int le = t / nitems;
+    const int it = t - le * nitems;
+    int h, c, j0, j1;
+    if (it < nq * C) {
+      c = it / nq;
+      h = it - c * nq;
+      j0 = c * nq;
+      j1 = j0 + nq < nh ? j0 + nq : nh;
+    } else {
+      c = 0;
+      h = nq + (it - nq * C);
+      j0 = 0;
+      j1 = nq;
+    }
     const int64_t e = e0 + le;
     const Prim ph = s_prim[le * nh + h];
     const Prim* pj = s_prim + le * nh;
-    double rx[DIM * DIM];
-#pragma unroll
-    for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
     double acc[NFLD];
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
     if (h < nq) {
-      for (int j = 0; j < nq; ++j) {
+      double rx[DIM * DIM];
+#pragma unroll
+      for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
+      const int jv = j1 < nq ? j1 : nq;
+      for (int j = j0; j < jv; ++j) {
         double sr[DIM], sd[DIM];
 #pragma unroll
         for (int r = 0; r < DIM; ++r) sr[r] = ea.skew[(r * nq + j) * nq + h];
@@ -1573,13 +1593,14 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
         }
         ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
       }
-      for (int m = 0; m < nft; ++m) {
+      for (int j = (j0 > nq ? j0 : nq); j < j1; ++j) {
+        const int m = j - nq;
         const int fc = m / nfq;
         const double b = ea.ext[m * nq + h] * a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
         double sd[DIM];
 #pragma unroll
         for (int d = 0; d < DIM; ++d) sd[d] = b * a.normals[(e * NF + fc) * DIM + d];
-        ec_flux_dot<DIM, NSP>(ph, pj[nq + m], sd, gamma, acc);
+        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
       }
     } else {
       const int m = h - nq;
@@ -1598,8 +1619,9 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
 #pragma unroll
       for (int f = 0; f < NFLD; ++f) acc[f] += s_fs[(f * epb + le) * nft + m];
     }
+    const int slot = h < nq ? c * nq + h : C * nq + (h - nq);
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_acc[(f * epb + le) * nh + h] = acc[f];
+    for (int f = 0; f < NFLD; ++f) s_acc[(f * epb + le) * nacc + slot] = acc[f];
   }
   __syncthreads();
   // 5: dQ/dt = -M^-1 V_h^T acc
@@ -1609,10 +1631,21 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
     double out[NFLD];
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) out[f] = 0.0;
-    for (int h = 0; h < nh; ++h) {
+    for (int h = 0; h < nq; ++h) {
       const double w = ea.apply[h * NP + i];
 #pragma unroll
-      for (int f = 0; f < NFLD; ++f) out[f] += w * s_acc[(f * epb + le) * nh + h];
+      for (int f = 0; f < NFLD; ++f) {
+        const double* sa = s_acc + (f * epb + le) * nacc + h;
+        double v = 0.0;
+        for (int cc = 0; cc < C; ++cc) v += sa[cc * nq];
+        out[f] += w * v;
+      }
+    }
+    for (int m = 0; m < nft; ++m) {
+      const double w = ea.apply[(nq + m) * NP + i];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+        out[f] += w * s_acc[(f * epb + le) * nacc + C * nq + m];
     }
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) a.out[f * N + e0 * NP + t] = -out[f];
@@ -2712,10 +2745,11 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
     hipLaunchKernelGGL(kp, dim3(unsigned((a.K + epb_p - 1) / epb_p)),                   \
                        dim3(mgdg::kBlock), per_p * epb_p, st, a, ea);                  \
     if ((s = check_launch("esdg_project_kernel"))) return s;                            \
+    const size_t nchunk = (size_t(esdg->nh) + quad->nq - 1) / quad->nq;               \
+    const size_t nacc = std::max(nchunk * quad->nq + nft, nft);                         \
     const size_t per_e = sizeof(mgdg::EcPrim<D, NS>) * esdg->nh +                       \
         sizeof(double) * nfld *                                                         \
-            (2 * size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp + 2 * nft +        \
-             size_t(esdg->nh));                                                         \
+            (2 * size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp + nft + nacc);     \
     if (per_e > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
     int epb = int(std::max<size_t>(1, std::min<size_t>(16, (64 * 1024) / per_e)));     \
     ea.epb = epb;                                                                       \

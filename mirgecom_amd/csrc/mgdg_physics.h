@@ -461,14 +461,23 @@ __device__ __forceinline__ void ec_prim(const double* q, double gamma,
   for (int k = 0; k < NSP; ++k) a.y[k] = q[2 + DIM + k] * irho;
 }
 
+// 1/x from the hardware reciprocal estimate and two Newton steps (a few
+// ulp; the two-point flux divides six times per pair of points).
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
+}
+
 // Logarithmic mean (mirgecom/inviscid.py:470-476) from the values and their
 // logarithms.
 __device__ __forceinline__ double ln_mean(double x, double lx, double y, double ly) {
-  const double f2 = (x * (x - 2.0 * y) + y * y) / (x * (x + 2.0 * y) + y * y);
+  const double f2 = (x * (x - 2.0 * y) + y * y) * rcp_nr(x * (x + 2.0 * y) + y * y);
   if (f2 < 1e-4)
-    return (x + y) / (2.0 + f2 * 2.0 / 3.0 + f2 * f2 * 2.0 / 5.0 +
-                      f2 * f2 * f2 * 2.0 / 7.0);
-  return (y - x) / (ly - lx);
+    return (x + y) * rcp_nr(2.0 + f2 * 2.0 / 3.0 + f2 * f2 * 2.0 / 5.0 +
+                            f2 * f2 * f2 * 2.0 / 7.0);
+  return (y - x) * rcp_nr(ly - lx);
 }
 
 // sum_d s_d F_S,d(a, b) of Chandrashekar's entropy-conserving, kinetic-
@@ -482,7 +491,7 @@ __device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
   const double beta_mean = ln_mean(a.beta, a.lbeta, b.beta, b.lbeta);
   const double rho_avg = 0.5 * (a.rho + b.rho);
   const double beta_avg = 0.5 * (a.beta + b.beta);
-  const double p_mean = 0.5 * rho_avg / beta_avg;
+  const double p_mean = 0.5 * rho_avg * rcp_nr(beta_avg);
   double un = 0.0, ua2 = 0.0, ua[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
@@ -492,7 +501,7 @@ __device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
   }
   const double mflux = rho_mean * un;
   acc[0] += mflux;
-  acc[1] += mflux * (0.5 * (1.0 / ((gamma - 1.0) * beta_mean) - (a.kin + b.kin)) + ua2) +
+  acc[1] += mflux * (0.5 * (rcp_nr((gamma - 1.0) * beta_mean) - (a.kin + b.kin)) + ua2) +
             p_mean * un;
 #pragma unroll
   for (int i = 0; i < DIM; ++i) acc[2 + i] += mflux * ua[i] + s[i] * p_mean;
