@@ -188,3 +188,27 @@ def test_quadrature_entry_point_validation():
     big = _lib.MgQuad(nq=20000, nfq=3, interp=1, weak_diff=1, face_interp=1,
                       lift=1)
     assert lib.mg_euler_rhs_quad(m, gas, big, one, None, 0, one, None) == 2
+
+
+def test_esdg_entry_point_validation():
+    lib = _lib.load()
+    m = _lib.MgMesh(dim=2, order=2, np=6, nfp=3, nfaces=3, nelements=4,
+                    nbnd_nodes=0, rx=1, normals=1, fscale=1, vmap_p=1,
+                    weak_diff=1, lift=1, face_mask=1, workspace=1)
+    gas = _lib.MgGas(eos_kind=0, gamma=1.4, gas_const=1.0)
+    one = ctypes.c_void_p(1)
+    qd = _lib.MgQuad(nq=9, nfq=3, interp=1, weak_diff=1, face_interp=1, lift=1)
+    ok = dict(proj=1, vh=1, skew=1, ext=1, ext_col=1, apply=1, face_weights=1)
+    # nh must be nq + nfaces * nfq
+    bad = _lib.MgEsdg(nh=17, pad0=0, **ok)
+    assert lib.mg_euler_rhs_esdg(m, gas, qd, bad, one, None, 1, one, one,
+                                 None) == 1
+    assert b"nh" in lib.mg_last_error()
+    good = _lib.MgEsdg(nh=18, pad0=0, **ok)
+    # the work buffer for the projected entropy variables is required
+    assert lib.mg_euler_rhs_esdg(m, gas, qd, good, one, None, 1, None, one,
+                                 None) == 1
+    assert b"ev_work" in lib.mg_last_error()
+    mix = _lib.MgGas(eos_kind=1, nspecies=7)
+    assert lib.mg_euler_rhs_esdg(m, mix, qd, good, one, None, 1, one, one,
+                                 None) == 2
