@@ -249,14 +249,18 @@ typedef struct mg_quad {
  * interpolated to the volume points, where the inviscid flux is formed, and
  * both face traces to the face points, where the numerical flux (flux_kind)
  * is formed; weak divergence and lift integrate them with the quadrature
- * weights.  Ideal gas (passive species allowed).  q_bnd_quad [F][nbnd_faces
- * * nfq] holds the boundary exterior states at the boundary faces' points
- * (face by face, in the order of the boundary nodes); NULL without
- * boundaries. */
+ * weights.  Ideal gas (passive species allowed) or the uiuc_7sp mixture:
+ * for the mixture the temperature at every point is the niter-step Newton
+ * inversion seeded with the interpolated nodal temperature [N]
+ * (project_fluid_state, mirgecom/gas_model.py:494-588).  q_bnd_quad
+ * [F][nbnd_faces*nfq] (and t_bnd_quad [nbnd_faces*nfq] for mixtures) hold the
+ * boundary exterior states at the boundary faces' points (face by face, in
+ * the order of the boundary nodes); NULL without boundaries. */
 mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
                             const mg_quad* quad, const double* q,
-                            const double* q_bnd_quad, int32_t flux_kind, double* rhs,
-                            void* stream);
+                            const double* temperature, const double* q_bnd_quad,
+                            const double* t_bnd_quad, int32_t niter, int32_t flux_kind,
+                            double* rhs, void* stream);
 
 /* Hybridized summation-by-parts operators of the entropy-stable path (on
  * the quadrature of an mg_quad; nh = nq + nfaces*nfq hybridized points:

@@ -107,7 +107,8 @@ SIGNATURES = {
                                  _P, _P, _P, _I32, _P, _P]),
     "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
     "mg_euler_rhs_quad": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
-                                 ctypes.POINTER(MgQuad), _P, _P, _I32, _P, _P]),
+                                 ctypes.POINTER(MgQuad), _P, _P, _P, _P, _I32,
+                                 _I32, _P, _P]),
     "mg_euler_rhs_esdg": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  ctypes.POINTER(MgQuad), ctypes.POINTER(MgEsdg),
                                  _P, _P, _I32, _P, _P, _P]),

@@ -182,12 +182,15 @@ def local_timestep(dcoll, state, viscous):
     return out
 
 
-def quad_boundary_states(dcoll, dev, q, dim, gas_model, boundaries, time):
+def quad_boundary_states(dcoll, dev, q, dim, gas_model, boundaries, time,
+                         temperature=None):
     """Exterior states ``(F, Nbf*nfq)`` at the boundary faces' quadrature
     points (mirgecom/euler.py:260-301: the interior state projected to the
     quadrature points of each boundary, ``project_fluid_state``, and the
-    boundary's exterior state evaluated there), or ``None`` without
-    boundary faces."""
+    boundary's exterior state evaluated there) and, for mixtures (nodal
+    ``temperature`` given), the exterior temperatures; ``(None, None)``
+    without boundary faces.  Mixture interior states get their temperature
+    by Newton from the interpolated nodal temperature."""
     boundaries = normalize_boundaries(boundaries)
     nb = dcoll.nbnd_nodes
     if nb == 0:
@@ -195,9 +198,16 @@ def quad_boundary_states(dcoll, dev, q, dim, gas_model, boundaries, time):
             unknown = [b.tag for b in boundaries if b.tag not in (BTAG_ALL,)]
             if unknown:
                 raise ValueError(f"boundary tags {unknown} are not on the mesh")
-        return None
+        return None, None
     nf = q.shape[0]
     interior = dev.to_bnd_quad(q.reshape(nf, -1)[:, dev.bnd_vol_index])
+    t_int = None
+    tbq = None
+    if temperature is not None:
+        seed = dev.to_bnd_quad(temperature.reshape(-1)[dev.bnd_vol_index])
+        t_int = gas_model.eos.temperature(ConservedVars(dim, interior),
+                                          temperature_seed=seed)
+        tbq = torch.empty_like(t_int)
     qbq = torch.empty_like(interior)
     covered = torch.zeros(interior.shape[1], dtype=torch.int32, device=q.device)
     for btag, bdry in boundaries.items():
@@ -206,11 +216,14 @@ def quad_boundary_states(dcoll, dev, q, dim, gas_model, boundaries, time):
             continue
         covered[idx] += 1
         cv_m = ConservedVars(dim, interior[:, idx])
-        state_m = FluidState(cv_m, _LazyDV(cv_m, gas_model.eos, None), gas_model)
+        t_m = t_int[idx] if t_int is not None else None
+        state_m = FluidState(cv_m, _LazyDV(cv_m, gas_model.eos, t_m), gas_model)
         dd_bdry = DOFDesc(BoundaryDomainTag(btag.tag), DISCR_TAG_QUAD)
         ext = bdry.exterior_state(dcoll, dd_bdry, gas_model, state_m, time=time)
         qbq[:, idx] = ext.cv.q.to(q.dtype)
+        if tbq is not None:
+            tbq[idx] = ext.temperature
     if int(covered.min()) != 1 or int(covered.max()) != 1:
         raise ValueError("every boundary face needs exactly one boundary "
                          "condition")
-    return qbq
+    return qbq, tbq

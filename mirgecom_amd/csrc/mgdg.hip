@@ -1200,7 +1200,9 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 // phase 4; operator rows vary per lane and are read through the caches.
 
 struct QuadArgs {
-  int nq, nfq, epb;
+  int nq, nfq, epb, niter;
+  const double* temp;     // [N] nodal temperature (mixtures: Newton seeds)
+  const double* tbq;      // [nbq] exterior temperatures at boundary face points
   // operators transposed so that consecutive lanes (consecutive points in
   // phases 2-3, consecutive nodes in phase 4) read consecutive addresses
   const double* interp;   // [NP][nq]
@@ -1211,7 +1213,24 @@ struct QuadArgs {
   int64_t nbq;
 };
 
-template <int DIM, int P, int NSP>
+// Temperature of a mixture state at a quadrature point: the Newton
+// inversion of its internal energy from an interpolated seed (the
+// reference's project_fluid_state -> make_fluid_state, gas_model.py:494-588).
+template <int DIM, int NSP>
+__device__ __forceinline__ double point_temperature(const double* q, double seed,
+                                                    int niter) {
+  const double irho = 1.0 / q[0];
+  double mm = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) mm += q[2 + d] * q[2 + d];
+  const double e = (q[1] - 0.5 * mm * irho) * irho;
+  double y[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) y[k] = q[2 + DIM + k] * irho;
+  return mix_temperature(e, seed, y, niter);
+}
+
+template <int DIM, int P, int NSP, int MIX>
 __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa) {
   using E = Elem<DIM, P>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
@@ -1223,6 +1242,8 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
   double* s_f = s_g + NFLD * DIM * epb * nq;
   double* s_p = s_f + NFLD * epb * nfq_all;   // [F][epb][NF*NFP] exterior
                                               // face-node values
+  double* s_t = s_p + NFLD * epb * NF * NFP;  // [epb][NP] temperature (MIX)
+  double* s_tp = s_t + epb * NP;              // [epb][NF*NFP] exterior (MIX)
   const int64_t e0 = int64_t(blockIdx.x) * epb;
   const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
   const int64_t N = a.nnodes;
@@ -1240,6 +1261,11 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
     if (nb < 0) continue;
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) s_p[(f * epb + le) * NF * NFP + fn] = a.q[f * N + nb];
+    if constexpr (MIX != 0) s_tp[le * NF * NFP + fn] = qa.temp[nb];
+  }
+  if constexpr (MIX != 0) {
+    for (int t = threadIdx.x; t < ne * NP; t += blockDim.x)
+      s_t[t] = qa.temp[e0 * NP + t];
   }
   __syncthreads();
   // 2: inviscid flux at the volume points, in reference directions
@@ -1250,13 +1276,17 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
     Node<DIM, NSP> s;
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) s.q[f] = 0.0;
+    double tseed = 0.0;
 #pragma unroll
     for (int i = 0; i < NP; ++i) {
       const double wi = w[i * nq];
 #pragma unroll
       for (int f = 0; f < NFLD; ++f) s.q[f] += wi * s_q[(f * epb + le) * NP + i];
+      if constexpr (MIX != 0) tseed += wi * s_t[le * NP + i];
     }
-    node_state<DIM, NSP, 0, false>(s, a.gas, 0.0);
+    double tq = 0.0;
+    if constexpr (MIX != 0) tq = point_temperature<DIM, NSP>(s.q, tseed, qa.niter);
+    node_state<DIM, NSP, MIX, false>(s, a.gas, tq);
     double F[NFLD][DIM];
     inviscid_flux<DIM, NSP>(s, F);
     const double* rx = a.rx + (e0 + le) * DIM * DIM;
@@ -1282,6 +1312,7 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
     const int32_t* vp = a.vmap_p + (e * NF + fc) * NFP;
     const int* fm = a.fmask + fc * NFP;
     Node<DIM, NSP> sm, sp;
+    double seed_m = 0.0, seed_p = 0.0, tm = 0.0, tp = 0.0;
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) sm.q[f] = 0.0;
 #pragma unroll
@@ -1290,6 +1321,7 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
       const int ij = fm[j];
 #pragma unroll
       for (int f = 0; f < NFLD; ++f) sm.q[f] += wj * s_q[(f * epb + le) * NP + ij];
+      if constexpr (MIX != 0) seed_m += wj * s_t[le * NP + ij];
     }
     const int32_t v0 = vp[0];
     if (v0 >= 0) {
@@ -1301,14 +1333,18 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
 #pragma unroll
         for (int f = 0; f < NFLD; ++f)
           sp.q[f] += wj * s_p[(f * epb + le) * NF * NFP + fc * NFP + j];
+        if constexpr (MIX != 0) seed_p += wj * s_tp[le * NF * NFP + fc * NFP + j];
       }
+      if constexpr (MIX != 0) tp = point_temperature<DIM, NSP>(sp.q, seed_p, qa.niter);
     } else {
       const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + m;
 #pragma unroll
       for (int f = 0; f < NFLD; ++f) sp.q[f] = qa.qbq[f * qa.nbq + bq];
+      if constexpr (MIX != 0) tp = qa.tbq[bq];
     }
-    node_state<DIM, NSP, 0, false>(sm, a.gas, 0.0);
-    node_state<DIM, NSP, 0, false>(sp, a.gas, 0.0);
+    if constexpr (MIX != 0) tm = point_temperature<DIM, NSP>(sm.q, seed_m, qa.niter);
+    node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
+    node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
     double n[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
@@ -2628,15 +2664,18 @@ mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double
 
 mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
                             const mg_quad* quad, const double* q,
-                            const double* q_bnd_quad, int32_t flux_kind, double* rhs,
-                            void* stream) {
+                            const double* temperature, const double* q_bnd_quad,
+                            const double* t_bnd_quad, int32_t niter, int32_t flux_kind,
+                            double* rhs, void* stream) {
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, false))) return s;
-  if (gas->eos_kind != 0)
-    return fail(MG_ERR_UNSUPPORTED, "the quadrature path is built for the ideal gas");
+  const bool mix = gas->eos_kind == 1;
+  if (mix && (s = ensure_mech())) return s;
   if (gas->nspecies != 0 && gas->nspecies != 7)
     return fail(MG_ERR_UNSUPPORTED, "nspecies must be 0 or 7");
+  if (mix && (!temperature || niter < 0 || (mesh->nbnd_nodes > 0 && !t_bnd_quad)))
+    return fail(MG_ERR_ARG, "mixture needs temperature (and t_bnd_quad), niter >= 0");
   if (!quad || !quad->interp || !quad->weak_diff || !quad->face_interp || !quad->lift ||
       quad->nq < 1 || quad->nfq < 1)
     return fail(MG_ERR_ARG, "mg_quad incomplete");
@@ -2647,9 +2686,10 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
   if (flux_kind < 0 || flux_kind > 2) return fail(MG_ERR_ARG, "flux_kind");
   const int nfld = 2 + mesh->dim + gas->nspecies;
   const size_t per_elem =
-      sizeof(double) * size_t(nfld) *
-      (size_t(mesh->np) + size_t(mesh->dim) * quad->nq +
-       size_t(mesh->nfaces) * (quad->nfq + mesh->nfp));
+      sizeof(double) * (size_t(nfld) *
+                            (size_t(mesh->np) + size_t(mesh->dim) * quad->nq +
+                             size_t(mesh->nfaces) * (quad->nfq + mesh->nfp)) +
+                        size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp);
   // up to 80 KiB per workgroup: two workgroups per CU, which is what the
   // kernel's registers allow (at least one element; at most 128 KiB)
   constexpr size_t kBudget = 128 * 1024;
@@ -2671,20 +2711,25 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
   qa.lift = quad->lift;
   qa.qbq = q_bnd_quad;
   qa.nbq = mesh->nbnd_nodes / mesh->nfp * quad->nfq;
+  qa.temp = temperature;
+  qa.tbq = t_bnd_quad;
+  qa.niter = niter;
   const size_t lds = per_elem * epb;
   const unsigned nblk = unsigned((a.K + epb - 1) / epb);
   hipStream_t st = static_cast<hipStream_t>(stream);
-#define MG_QCASE(D, P, NS)                                                        \
-  if (mesh->dim == D && mesh->order == P && gas->nspecies == NS) {                \
-    auto kern = mgdg::euler_quad_kernel<D, P, NS>;                                \
+#define MG_QCASE(D, P, NS, MX)                                                    \
+  if (mesh->dim == D && mesh->order == P && gas->nspecies == NS && int(mix) == MX) { \
+    auto kern = mgdg::euler_quad_kernel<D, P, NS, MX>;                            \
     if ((s = set_lds(kern, kBudget))) return s;                                   \
     hipLaunchKernelGGL(kern, dim3(nblk), dim3(mgdg::kBlock), lds, st, a, qa);     \
     return check_launch("euler_quad_kernel");                                     \
   }
-  MG_QCASE(2, 1, 0) MG_QCASE(2, 2, 0) MG_QCASE(2, 3, 0) MG_QCASE(2, 4, 0)
-  MG_QCASE(3, 1, 0) MG_QCASE(3, 2, 0) MG_QCASE(3, 3, 0) MG_QCASE(3, 4, 0)
-  MG_QCASE(2, 1, 7) MG_QCASE(2, 2, 7) MG_QCASE(2, 3, 7) MG_QCASE(2, 4, 7)
-  MG_QCASE(3, 1, 7) MG_QCASE(3, 2, 7) MG_QCASE(3, 3, 7) MG_QCASE(3, 4, 7)
+  MG_QCASE(2, 1, 0, 0) MG_QCASE(2, 2, 0, 0) MG_QCASE(2, 3, 0, 0) MG_QCASE(2, 4, 0, 0)
+  MG_QCASE(3, 1, 0, 0) MG_QCASE(3, 2, 0, 0) MG_QCASE(3, 3, 0, 0) MG_QCASE(3, 4, 0, 0)
+  MG_QCASE(2, 1, 7, 0) MG_QCASE(2, 2, 7, 0) MG_QCASE(2, 3, 7, 0) MG_QCASE(2, 4, 7, 0)
+  MG_QCASE(3, 1, 7, 0) MG_QCASE(3, 2, 7, 0) MG_QCASE(3, 3, 7, 0) MG_QCASE(3, 4, 7, 0)
+  MG_QCASE(2, 1, 7, 1) MG_QCASE(2, 2, 7, 1) MG_QCASE(2, 3, 7, 1) MG_QCASE(2, 4, 7, 1)
+  MG_QCASE(3, 1, 7, 1) MG_QCASE(3, 2, 7, 1) MG_QCASE(3, 3, 7, 1) MG_QCASE(3, 4, 7, 1)
 #undef MG_QCASE
   return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }

@@ -104,8 +104,10 @@ def _lagrange_grad(el, x):
 
 def euler_operator_quad(disc, box, cv, gas_model, boundaries, time=0.0,
                         numerical_flux_func=ph.inviscid_facial_flux_rusanov,
-                        degree=None):
-    """Overintegrated Euler RHS ``(F, K, Np)`` of the conserved state ``cv``."""
+                        degree=None, temperature=None):
+    """Overintegrated Euler RHS ``(F, K, Np)`` of the conserved state ``cv``;
+    for a mixture, ``temperature (K, Np)`` is interpolated to every point as
+    the seed of the Newton inversion there (gas_model.py:494-588)."""
     el = disc.el
     dim, nf, nfp = disc.dim, disc.nf, el.nfp
     degree = 2 * el.order if degree is None else degree
@@ -114,8 +116,11 @@ def euler_operator_quad(disc, box, cv, gas_model, boundaries, time=0.0,
     xq, wq = stroud_rule(dim, degree)
     phi = _lagrange(el, xq)
     dphi = _lagrange_grad(el, xq)
+    mix = temperature is not None
     uq = np.einsum("qn,fkn->fkq", phi, cv)
-    sq = ph.make_fluid_state(uq.reshape(nvar, -1), gas_model, dim)
+    tq = np.einsum("qn,kn->kq", phi, temperature).reshape(-1) if mix else None
+    sq = ph.make_fluid_state(uq.reshape(nvar, -1), gas_model, dim,
+                             temperature_seed=tq)
     fq = ph.inviscid_flux(sq).reshape(nvar, dim, disc.K, -1)   # (F, d, K, q)
     vol = np.einsum("q,aqi,kad,fdkq->fki", wq, dphi, disc.drdx, fq)
     vol = vol * disc.J[None, :, None]
@@ -126,6 +131,11 @@ def euler_operator_quad(disc, box, cv, gas_model, boundaries, time=0.0,
     own_nodes = disc.trace(cv)
     um = np.zeros((nvar, disc.K, nf, nfq))
     up = np.zeros((nvar, disc.K, nf, nfq))
+    tm = np.zeros((disc.K, nf, nfq))
+    tp = np.zeros((disc.K, nf, nfq))
+    if mix:
+        t_own = disc.trace(temperature[None])[0]                   # (K, Nf, Nfp)
+        t_ext = np.nan_to_num(disc.exterior(temperature[None])[0], nan=1.0)
     phif = []
     xref = []
     for f, fv in enumerate(FACES[dim]):
@@ -138,13 +148,19 @@ def euler_operator_quad(disc, box, cv, gas_model, boundaries, time=0.0,
         um[:, :, f] = np.einsum("mj,fkj->fkm", trace_basis, own_nodes[:, :, f])
         up[:, :, f] = np.einsum("mj,fkj->fkm", trace_basis,
                                 np.nan_to_num(ext_nodes[:, :, f]))
+        if mix:
+            tm[:, f] = np.einsum("mj,kj->km", trace_basis, t_own[:, f])
+            tp[:, f] = np.einsum("mj,kj->km", trace_basis, t_ext[:, f])
     normal = np.repeat(disc.normal[:, :, None, :], nfq, axis=2)  # (K, Nf, q, d)
     flat = lambda a: a.reshape(a.shape[0], -1)                 # noqa: E731
     nrm = np.moveaxis(normal.reshape(-1, dim), 1, 0)
-    sm = ph.make_fluid_state(flat(um), gas_model, dim)
+    sm = ph.make_fluid_state(flat(um), gas_model, dim,
+                             temperature_seed=tm.reshape(-1) if mix else None)
     fstar = np.zeros((nvar, disc.K * nf * nfq))
     inter = np.nonzero(~disc.is_bnd[..., 0].reshape(-1).repeat(nfq))[0]
-    sp_int = ph.make_fluid_state(flat(up)[:, inter], gas_model, dim)
+    sp_int = ph.make_fluid_state(flat(up)[:, inter], gas_model, dim,
+                                 temperature_seed=(tp.reshape(-1)[inter]
+                                                   if mix else None))
     fstar[:, inter] = numerical_flux_func(_sub_state(sm, inter), sp_int,
                                           gas_model, nrm[:, inter])
     # boundary faces: conditions evaluated at the faces' physical points

@@ -173,21 +173,25 @@ def test_quadrature_entry_point_validation():
     # incomplete mg_quad
     qd = _lib.MgQuad(nq=9, nfq=3, interp=1, weak_diff=None, face_interp=1,
                      lift=1)
-    assert lib.mg_euler_rhs_quad(m, gas, qd, one, None, 0, one, None) == 1
+    assert lib.mg_euler_rhs_quad(m, gas, qd, one, None, None, None, 0, 0, one,
+                                 None) == 1
     assert b"mg_quad" in lib.mg_last_error()
     qd = _lib.MgQuad(nq=9, nfq=3, interp=1, weak_diff=1, face_interp=1, lift=1)
-    # mixtures are not built for the quadrature path
+    # a mixture needs its nodal temperature (and the mechanism)
     mix = _lib.MgGas(eos_kind=1, nspecies=7)
-    assert lib.mg_euler_rhs_quad(m, mix, qd, one, None, 0, one, None) == 2
+    assert lib.mg_euler_rhs_quad(m, mix, qd, one, None, None, None, 5, 0, one,
+                                 None) == 1
     # boundary exterior states required when the mesh has boundary nodes
     m.nbnd_nodes = 6
-    assert lib.mg_euler_rhs_quad(m, gas, qd, one, None, 0, one, None) == 1
+    assert lib.mg_euler_rhs_quad(m, gas, qd, one, None, None, None, 0, 0, one,
+                                 None) == 1
     assert b"boundary" in lib.mg_last_error()
     # an LDS footprint above 128 KiB per element is refused
     m.nbnd_nodes = 0
     big = _lib.MgQuad(nq=20000, nfq=3, interp=1, weak_diff=1, face_interp=1,
                       lift=1)
-    assert lib.mg_euler_rhs_quad(m, gas, big, one, None, 0, one, None) == 2
+    assert lib.mg_euler_rhs_quad(m, gas, big, one, None, None, None, 0, 0, one,
+                                 None) == 2
 
 
 def test_esdg_entry_point_validation():

@@ -192,3 +192,38 @@ def test_boundaries_3d(esdg):
     err = rel_err(_host(got.q), ref)
     print("err", err)
     assert err.max() < tol
+
+
+@pytest.mark.parametrize("kind", ["periodic", "dummy"])
+def test_mixture(kind):
+    """uiuc_7sp mixture: Newton temperatures at every volume and face point
+    from the interpolated nodal temperature (project_fluid_state)."""
+    from mirgecom_amd.boundary import DummyBoundary
+    from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
+    from mirgecom_amd.eos import PyrometheusMixture
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+    from oracle import operators as oop, physics as ph, thermochem as tc
+    from oracle.overintegration import euler_operator_quad
+    from tests.cases import mixture_state
+    periodic = kind == "periodic"
+    pb = Problem(3, 2, (3, 3, 3), 0.0, 2 * np.pi, (periodic,) * 3)
+    q, t = mixture_state(pb.x, seed=31)
+    gm = GasModel(eos=PyrometheusMixture())
+    ogm = ph.GasModel(tc.PyrometheusMixture())
+    gb = {} if periodic else {"all": DummyBoundary()}
+    ob = {} if periodic else {"all": oop.DummyBoundary()}
+    print(f"overintegration mixture {kind}", flush=True)
+    st = make_fluid_state(ConservedVars(3, _dev(q)), gm,
+                          temperature_seed=_dev(t + 2.0))
+    with pytest.warns(UserWarning):
+        got = euler_operator(pb.dcoll, st, gm, gb, quadrature_tag=DISCR_TAG_QUAD,
+                             inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
+    t_nodal = _host(st.temperature)
+    ref = euler_operator_quad(pb.odisc, pb.box, q, ogm, ob,
+                              temperature=t_nodal)
+    err = rel_err(_host(got.q), ref)
+    print("err", err)
+    assert err.max() < 1e-11
