@@ -271,7 +271,12 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
  *   ext[m][h]      = E[m][h], E = V_f P_q          [nfaces*nfq][nq]
  *   ext_col[h][m]  = E[m][h]                       [nq][nfaces*nfq]
  *   apply_t[h][i]  = (M^-1 V_h^T)[i][h]            [nh][np]
- *   face_weights   = the face rule's weights       [nfq]                  */
+ *   face_weights   = the face rule's weights       [nfq]
+ * and the same operators along the cyclic diagonals (optional, used when
+ * nq = 64 and nfaces*nfq = 192, i.e. 3-D p = 3 at degree 6, where each
+ * two-point flux is then evaluated once for both of its rows):
+ *   skew_diag[r][s][h] = (Q_r - Q_r^T)[h][(h + s) mod nq]   [dim][nq][nq]
+ *   ext_diag[s][m]     = E[m][(m + s) mod nq]               [nq][nfaces*nfq] */
 typedef struct mg_esdg {
   int32_t nh, pad0;
   const double* proj;
@@ -281,6 +286,8 @@ typedef struct mg_esdg {
   const double* ext_col;
   const double* apply;
   const double* face_weights;
+  const double* skew_diag;
+  const double* ext_diag;
 } mg_esdg;
 
 /* Entropy-stable Euler RHS by flux differencing (euler_operator with

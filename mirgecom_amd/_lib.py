@@ -64,7 +64,8 @@ class MgEsdg(ctypes.Structure):
         ("proj", ctypes.c_void_p), ("vh", ctypes.c_void_p),
         ("skew", ctypes.c_void_p), ("ext", ctypes.c_void_p),
         ("ext_col", ctypes.c_void_p), ("apply", ctypes.c_void_p),
-        ("face_weights", ctypes.c_void_p),
+        ("face_weights", ctypes.c_void_p), ("skew_diag", ctypes.c_void_p),
+        ("ext_diag", ctypes.c_void_p),
     ]
 
 

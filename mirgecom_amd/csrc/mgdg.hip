@@ -208,6 +208,7 @@ __device__ __forceinline__ const float* mfma_ops(const double* ws) {
 }
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
 
 template <typename R, int DIM, int P>
 __device__ __forceinline__ const R* packed_ops(const double* ws) {
@@ -1193,14 +1194,17 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 // {{{ Euler RHS on the quadrature discretization (overintegration)
 //
 // A workgroup owns epb elements (host-chosen from the LDS budget).  LDS:
-// s_q [F][epb][NP] the state, s_g [F*DIM][epb][nq] the inviscid flux in
-// reference directions (rx . F) at the volume points, s_f [F][epb][NF*nfq]
-// the numerical flux times the face Jacobian ratio at the face points.
-// Lanes own quadrature points in phases 2-3 and (field, node) outputs in
-// phase 4; operator rows vary per lane and are read through the caches.
+// s_q [F][epb][NP] the state (then the result), s_g [F*epb][gs] the inviscid
+// flux in reference directions (rx . F) at the volume points (r-major rows
+// of DIM*nq), s_f [F*epb][fs] minus the numerical flux times the face
+// Jacobian ratio at the face points; the row strides gs, fs are = 4 mod 32
+// doubles so that the matrix-core operand reads of 16 rows spread over the
+// LDS banks.  Lanes own quadrature points in phases 2-3; phase 4 is one
+// (NP x K) x (K x F*epb) product on the matrix cores.
 
 struct QuadArgs {
   int nq, nfq, epb, niter;
+  int gs, fs;             // LDS row strides of s_g and s_f (doubles)
   const double* temp;     // [N] nodal temperature (mixtures: Newton seeds)
   const double* tbq;      // [nbq] exterior temperatures at boundary face points
   // operators transposed so that consecutive lanes (consecutive points in
@@ -1230,6 +1234,52 @@ __device__ __forceinline__ double point_temperature(const double* q, double seed
   return mix_temperature(e, seed, y, niter);
 }
 
+// acc0 + acc1 += A[i][0:kn) B[0:kn)[j] for one 16x16 tile on the matrix
+// cores: a = &A[i][0] with k stride NP (global, read through the caches),
+// b = &B[0][j] with k contiguous (LDS); kk = lane / 16 mod 4.  Reads are
+// clamped into [0, kn) and B is zero beyond kn, so the k padding adds
+// nothing.  Two register sets of 4 * kQuadSet k each alternate, so that the
+// loads of one are in flight while the other feeds the matrix cores.
+constexpr int kQuadSet = 4;
+
+template <int NP>
+__device__ __forceinline__ void quad_mfma_load(const double* a, const double* b, int kn,
+                                               int k0, int kk, double (&av)[kQuadSet],
+                                               double (&bv)[kQuadSet]) {
+#pragma unroll
+  for (int u = 0; u < kQuadSet; ++u) {
+    const int k = k0 + 4 * u + kk;
+    const bool ok = k < kn;
+    av[u] = a[(ok ? k : 0) * NP];
+    const double y = b[ok ? k : 0];
+    bv[u] = ok ? y : 0.0;
+  }
+}
+
+__device__ __forceinline__ void quad_mfma_step(const double (&av)[kQuadSet],
+                                               const double (&bv)[kQuadSet],
+                                               doublex4& acc0, doublex4& acc1) {
+#pragma unroll
+  for (int u = 0; u < kQuadSet; u += 2) {
+    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc0, 0, 0, 0);
+    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u + 1], bv[u + 1], acc1, 0, 0, 0);
+  }
+}
+
+template <int NP>
+__device__ __forceinline__ void quad_mfma_panel(const double* a, const double* b, int kn,
+                                                int kk, doublex4& acc0, doublex4& acc1) {
+  constexpr int KS = 4 * kQuadSet;
+  double ax[kQuadSet], bx[kQuadSet], ay[kQuadSet], by[kQuadSet];
+  quad_mfma_load<NP>(a, b, kn, 0, kk, ax, bx);
+  for (int k0 = 0; k0 < kn; k0 += 2 * KS) {
+    quad_mfma_load<NP>(a, b, kn, k0 + KS, kk, ay, by);
+    quad_mfma_step(ax, bx, acc0, acc1);
+    quad_mfma_load<NP>(a, b, kn, k0 + 2 * KS, kk, ax, bx);
+    quad_mfma_step(ay, by, acc0, acc1);
+  }
+}
+
 template <int DIM, int P, int NSP, int MIX>
 __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa) {
   using E = Elem<DIM, P>;
@@ -1237,10 +1287,11 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
   constexpr int NFLD = 2 + DIM + NSP;
   extern __shared__ double lds[];
   const int nq = qa.nq, nfq = qa.nfq, epb = qa.epb, nfq_all = NF * qa.nfq;
+  const int gs = qa.gs, fs = qa.fs;
   double* s_q = lds;
   double* s_g = s_q + NFLD * epb * NP;
-  double* s_f = s_g + NFLD * DIM * epb * nq;
-  double* s_p = s_f + NFLD * epb * nfq_all;   // [F][epb][NF*NFP] exterior
+  double* s_f = s_g + NFLD * epb * gs;
+  double* s_p = s_f + NFLD * epb * fs;        // [F][epb][NF*NFP] exterior
                                               // face-node values
   double* s_t = s_p + NFLD * epb * NF * NFP;  // [epb][NP] temperature (MIX)
   double* s_tp = s_t + epb * NP;              // [epb][NF*NFP] exterior (MIX)
@@ -1297,7 +1348,7 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
         double g = 0.0;
 #pragma unroll
         for (int d = 0; d < DIM; ++d) g += rx[r * DIM + d] * F[f][d];
-        s_g[((f * DIM + r) * epb + le) * nq + m] = g;
+        s_g[(f * epb + le) * gs + r * nq + m] = g;
       }
   }
   // 3: numerical flux at the face points (both traces interpolated from the
@@ -1348,40 +1399,52 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
     double n[DIM];
 #pragma unroll
     for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
-    double fs[NFLD];
-    inviscid_numflux<DIM, NSP>(sm, sp, n, a.flux_kind, fs);
+    double fstar[NFLD];
+    inviscid_numflux<DIM, NSP>(sm, sp, n, a.flux_kind, fstar);
     const double sc = a.fscale[e * NF + fc];
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_f[(f * epb + le) * nfq_all + rem] = sc * fs[f];
+    for (int f = 0; f < NFLD; ++f) s_f[(f * epb + le) * fs + rem] = -sc * fstar[f];
   }
   __syncthreads();
-  // 4: dQ/dt = sum_r Wq_r (rx . F)_r - Lq (fscale F*), all fields of one
-  // node per lane (each operator entry read once for the NFLD fields)
-  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
-    const int le = t / NP;
-    const int i = t - le * NP;
-    double acc[NFLD];
+  // 4: dQ/dt = [Wq_0 .. Wq_DIM-1 Lq] [G; -fscale F*]: rows are the NP nodes,
+  // columns j = f * epb + le the (field, element) pairs, the contraction
+  // runs over the DIM*nq volume points, then the NF*nfq face points.  Each
+  // wave forms 16x16 output tiles with v_mfma_f64_16x16x4f64 (A[i][k] in
+  // lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i % 4),
+  // register i / 4).  Every read stays in bounds: points past the rule enter
+  // with B = 0; rows past NP (read from row 0) and columns of absent
+  // elements are formed but not stored.
+  {
+    constexpr int RT = (NP + 15) / 16;
+    const int ncol = NFLD * epb;
+    const int ntile = RT * ((ncol + 15) / 16);
+    const int li = threadIdx.x & 15, kk = (threadIdx.x >> 4) & 3;
+    for (int tile = threadIdx.x >> 6; tile < ntile; tile += blockDim.x >> 6) {
+      const int rt = tile % RT, ct = tile / RT;
+      const int row = rt * 16 + li;
+      const int j = ct * 16 + li;
+      const int lej = j % epb;
+      const bool rok = row < NP, cok = j < ncol && lej < ne;
+      doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+      const int jc = j < ncol ? j : 0;
+      const int r0 = rok ? row : 0;
+      // volume points: k = r * nq + m; face points: k = face * nfq + m
+      quad_mfma_panel<NP>(qa.wdiff + r0, s_g + jc * gs, DIM * nq, kk, acc0, acc1);
+      quad_mfma_panel<NP>(qa.lift + r0, s_f + jc * fs, nfq_all, kk, acc0, acc1);
+      if (cok) {
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
-#pragma unroll
-    for (int rr = 0; rr < DIM; ++rr) {
-      const double* wd = qa.wdiff + rr * nq * NP + i;
-      const double* g = s_g + (rr * epb + le) * nq;
-      for (int m = 0; m < nq; ++m) {
-        const double w = wd[m * NP];
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) acc[f] += w * g[f * DIM * epb * nq + m];
+        for (int g = 0; g < 4; ++g) {
+          const int i = rt * 16 + 4 * g + kk;
+          if (i < NP) s_q[j * NP + i] = acc0[g] + acc1[g];
+        }
       }
     }
-    const double* lf = qa.lift + i;
-    const double* fl = s_f + le * nfq_all;
-    for (int m = 0; m < nfq_all; ++m) {
-      const double w = lf[m * NP];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) acc[f] -= w * fl[f * epb * nfq_all + m];
-    }
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) a.out[f * N + e0 * NP + t] = acc[f];
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int f = t / (ne * NP);
+    const int r = t - f * ne * NP;
+    a.out[f * N + e0 * NP + r] = s_q[f * epb * NP + r];
   }
 }
 
@@ -1414,6 +1477,11 @@ struct EsdgArgs {
   int64_t nbq;
   double* ev;             // [F][N] projected entropy variables
   int dissipation;
+  // nq = 64, NF*nfq = 192, 256 threads: the paired phase 4 (each two-point
+  // flux once, for both rows) with the operators along cyclic diagonals
+  int paired;
+  const double* skew_diag;  // [DIM][nq(s)][nq(h)]: (Q_r - Q_r^T)[h][(h+s) % nq]
+  const double* ext_diag;   // [nq(s)][NF*nfq(m)]: E[m][(m+s) % nq]
 };
 
 template <int DIM, int P, int NSP>
@@ -1459,6 +1527,108 @@ __global__ __launch_bounds__(kBlock) void esdg_project_kernel(KArgs a, EsdgArgs 
     }
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) ea.ev[f * N + e0 * NP + t] = v[f];
+  }
+}
+
+// Phase 4 of esdg_kernel when nq = 64 and NF*nfq = 192 (nh = 256, four
+// waves): each two-point flux is evaluated once.  (Q_h - Q_h^T) is skew and
+// F_S symmetric, so the pair (h, j) adds S[h][j] F_S to row h and the
+// negative to row j.  Wave 0 owns the 64 volume rows and meets the pairs
+// (h, (h+s) mod 64), s = 1..31, in step s; the contributions to the partner
+// rows travel in a register vector that moves one lane down per step, so
+// that lane h always holds row (h+s) mod 64 (s = 32 pairs each lane with the
+// lane that owns the same pair: both add their own row).  Wave 1 + b owns
+// face rows 64b..64b+63 and meets the volume points (l+s) mod 64 in step s,
+// s = 0..63 (the face-face block is zero); its moving vector collects that
+// block's part of the volume rows and is back in place after 64 steps.
+// Results: volume rows in the four nq-long pieces of s_acc (piece 0 from
+// wave 0, piece 1 + b from wave 1 + b; phase 5 adds them), face rows plus
+// the face flux in the face slots.
+template <int DIM, int NSP>
+__device__ __forceinline__ void esdg_pairs_64(const KArgs& a, const EsdgArgs& ea,
+                                              const EcPrim<DIM, NSP>* s_prim,
+                                              const double* s_fs, double* s_acc,
+                                              int64_t e0, int ne, int nacc, double gamma) {
+  constexpr int NF = DIM + 1;
+  constexpr int NFLD = 2 + DIM + NSP;
+  using Prim = EcPrim<DIM, NSP>;
+  constexpr int NQ = 64, NFT = 192, NH = NQ + NFT;
+  const int epb = ea.epb, nfq = ea.nfq;
+  const int l = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int down = (l + 1) & 63;
+  for (int le = 0; le < ne; ++le) {
+    const int64_t e = e0 + le;
+    const Prim* pj = s_prim + le * NH;
+    double acc[NFLD], mv[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) acc[f] = mv[f] = 0.0;
+    if (wave == 0) {
+      const Prim ph = pj[l];
+      double rx[DIM * DIM];
+#pragma unroll
+      for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
+      for (int s = 1; s <= 32; ++s) {
+        double sd[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sd[d] = 0.0;
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) {
+          const double sr = ea.skew_diag[(r * NQ + s) * NQ + l];
+#pragma unroll
+          for (int d = 0; d < DIM; ++d) sd[d] += rx[r * DIM + d] * sr;
+        }
+        double fv[NFLD];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) fv[f] = 0.0;
+        ec_flux_dot<DIM, NSP>(ph, pj[(l + s) & 63], sd, gamma, fv);
+        if (s < 32) {
+#pragma unroll
+          for (int f = 0; f < NFLD; ++f) {
+            acc[f] += fv[f];
+            mv[f] = __shfl(mv[f] - fv[f], down, 64);
+          }
+        } else {
+#pragma unroll
+          for (int f = 0; f < NFLD; ++f) acc[f] += fv[f];
+        }
+      }
+      // lane l holds row (l + 32) mod 64
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) {
+        acc[f] += __shfl(mv[f], (l + 32) & 63, 64);
+        s_acc[(f * epb + le) * nacc + l] = acc[f];
+      }
+    } else {
+      const int m = (wave - 1) * NQ + l;
+      const Prim pm = pj[NQ + m];
+      const int fc = m / nfq;
+      const double bsc = a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
+      double nrm[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) nrm[d] = bsc * a.normals[(e * NF + fc) * DIM + d];
+      for (int s = 0; s < NQ; ++s) {
+        const double b = ea.ext_diag[s * NFT + m];
+        double sd[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sd[d] = b * nrm[d];
+        double fv[NFLD];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) fv[f] = 0.0;
+        ec_flux_dot<DIM, NSP>(pm, pj[(l + s) & 63], sd, gamma, fv);
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) {
+          acc[f] -= fv[f];
+          mv[f] = __shfl(mv[f] + fv[f], down, 64);
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) {
+        double* row = s_acc + (f * epb + le) * nacc;
+        row[wave * NQ + l] = mv[f];
+        row[4 * NQ + m] = acc[f] + s_fs[(f * epb + le) * NFT + m];
+      }
+    }
   }
 }
 
@@ -1589,6 +1759,9 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   // volume row has nh entries, a face row nq (the face-face block is zero):
   // volume rows are cut into C = ceil(nh / nq) pieces of nq entries so that
   // every work item sums nq pairs; the pieces are added in phase 5.
+  if (ea.paired) {
+    esdg_pairs_64<DIM, NSP>(a, ea, s_prim, s_fs, s_acc, e0, ne, nacc, gamma);
+  } else {
   const int nitems = nq * C + nft;
   for (int t = threadIdx.x; t < ne * nitems; t += blockDim.x) {
     const int le = t / nitems;
@@ -1658,6 +1831,7 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
     const int slot = h < nq ? c * nq + h : C * nq + (h - nq);
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) s_acc[(f * epb + le) * nacc + slot] = acc[f];
+  }
   }
   __syncthreads();
   // 5: dQ/dt = -M^-1 V_h^T acc
@@ -2685,10 +2859,13 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
   if (mesh->nbnd_nodes % mesh->nfp) return fail(MG_ERR_ARG, "nbnd_nodes % nfp");
   if (flux_kind < 0 || flux_kind > 2) return fail(MG_ERR_ARG, "flux_kind");
   const int nfld = 2 + mesh->dim + gas->nspecies;
+  // LDS row strides = 4 mod 32 doubles (bank spread of the matrix-core reads)
+  auto bank_stride = [](int n) { return n + ((4 - n % 32) + 32) % 32; };
+  const int gs = bank_stride(mesh->dim * quad->nq);
+  const int fs = bank_stride(mesh->nfaces * quad->nfq);
   const size_t per_elem =
-      sizeof(double) * (size_t(nfld) *
-                            (size_t(mesh->np) + size_t(mesh->dim) * quad->nq +
-                             size_t(mesh->nfaces) * (quad->nfq + mesh->nfp)) +
+      sizeof(double) * (size_t(nfld) * (size_t(mesh->np) + size_t(gs) + size_t(fs) +
+                                        size_t(mesh->nfaces) * mesh->nfp) +
                         size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp);
   // up to 80 KiB per workgroup: two workgroups per CU, which is what the
   // kernel's registers allow (at least one element; at most 128 KiB)
@@ -2705,6 +2882,8 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
   qa.nq = quad->nq;
   qa.nfq = quad->nfq;
   qa.epb = epb;
+  qa.gs = gs;
+  qa.fs = fs;
   qa.interp = quad->interp;
   qa.wdiff = quad->weak_diff;
   qa.finterp = quad->face_interp;
@@ -2776,6 +2955,12 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
   ea.nbq = mesh->nbnd_nodes / mesh->nfp * quad->nfq;
   ea.ev = ev_work;
   ea.dissipation = dissipation ? 1 : 0;
+  ea.skew_diag = esdg->skew_diag;
+  ea.ext_diag = esdg->ext_diag;
+  ea.paired = (mesh->dim == 3 && quad->nq == 64 && mesh->nfaces * quad->nfq == 192 &&
+               esdg->skew_diag && esdg->ext_diag && mgdg::kBlock == 256)
+                  ? 1
+                  : 0;
   hipStream_t st = static_cast<hipStream_t>(stream);
   constexpr size_t kBudget = 128 * 1024;
   const size_t nft = size_t(mesh->nfaces) * quad->nfq;

@@ -462,7 +462,7 @@ __device__ __forceinline__ void ec_prim(const double* q, double gamma,
 }
 
 // 1/x from the hardware reciprocal estimate and two Newton steps (a few
-// ulp; the two-point flux divides six times per pair of points).
+// ulp; the two-point flux takes three reciprocals per pair of points).
 __device__ __forceinline__ double rcp_nr(double x) {
   double r = __builtin_amdgcn_rcp(x);
   r = fma(r, fma(-x, r, 1.0), r);
@@ -470,25 +470,40 @@ __device__ __forceinline__ double rcp_nr(double x) {
   return r;
 }
 
-// Logarithmic mean (mirgecom/inviscid.py:470-476) from the values and their
-// logarithms.
-__device__ __forceinline__ double ln_mean(double x, double lx, double y, double ly) {
-  const double f2 = (x * (x - 2.0 * y) + y * y) * rcp_nr(x * (x + 2.0 * y) + y * y);
-  if (f2 < 1e-4)
-    return (x + y) * rcp_nr(2.0 + f2 * 2.0 / 3.0 + f2 * f2 * 2.0 / 5.0 +
-                            f2 * f2 * f2 * 2.0 / 7.0);
-  return (y - x) * rcp_nr(ly - lx);
+// Logarithmic mean (mirgecom/inviscid.py:470-476) of x, y from the values
+// and their logarithms, as the quotient num / den: (y - x) / (ln y - ln x),
+// or, when f2 = ((x - y) / (x + y))^2 < 1e-4, the series
+// (x + y) / (2 + 2/3 f2 + 2/5 f2^2 + 2/7 f2^3), written with u = (x - y)^2,
+// w = (x + y)^2 as (x + y) w^3 / (2 w^3 + 2/3 u w^2 + 2/5 u^2 w + 2/7 u^3).
+// Both forms are selected before any division, so a wave whose lanes take
+// different branches pays for one reciprocal, and the caller takes either
+// the mean or its inverse from the same parts.
+__device__ __forceinline__ void ln_mean_parts(double x, double lx, double y, double ly,
+                                              double& num, double& den) {
+  const double sm = x + y, df = x - y;
+  const double u = df * df, w = sm * sm;
+  const bool small = u < 1e-4 * w;
+  const double ser = fma(fma(fma(2.0, w, (2.0 / 3.0) * u), w, 0.4 * u * u), w,
+                         (2.0 / 7.0) * u * u * u);
+  num = small ? sm * (w * w * w) : y - x;
+  den = small ? ser : ly - lx;
 }
 
 // sum_d s_d F_S,d(a, b) of Chandrashekar's entropy-conserving, kinetic-
-// energy-preserving flux (mirgecom/inviscid.py:451-518), added to acc.
+// energy-preserving flux (mirgecom/inviscid.py:451-518), added to acc.  The
+// energy flux needs 1 / ((gamma - 1) beta_mean): it is formed as
+// den / num of beta's logarithmic mean.
 template <int DIM, int NSP>
 __device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
                                             const EcPrim<DIM, NSP>& b,
                                             const double* s, double gamma,
                                             double* acc) {
-  const double rho_mean = ln_mean(a.rho, a.lrho, b.rho, b.lrho);
-  const double beta_mean = ln_mean(a.beta, a.lbeta, b.beta, b.lbeta);
+  double rn, rd, bn, bd;
+  ln_mean_parts(a.rho, a.lrho, b.rho, b.lrho, rn, rd);
+  ln_mean_parts(a.beta, a.lbeta, b.beta, b.lbeta, bn, bd);
+  const double rho_mean = rn * rcp_nr(rd);
+  const double inv_beta_mean = bd * rcp_nr(bn);
+  const double inv_gm1 = 1.0 / (gamma - 1.0);
   const double rho_avg = 0.5 * (a.rho + b.rho);
   const double beta_avg = 0.5 * (a.beta + b.beta);
   const double p_mean = 0.5 * rho_avg * rcp_nr(beta_avg);
@@ -501,7 +516,7 @@ __device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
   }
   const double mflux = rho_mean * un;
   acc[0] += mflux;
-  acc[1] += mflux * (0.5 * (rcp_nr((gamma - 1.0) * beta_mean) - (a.kin + b.kin)) + ua2) +
+  acc[1] += mflux * (0.5 * (inv_beta_mean * inv_gm1 - (a.kin + b.kin)) + ua2) +
             p_mean * un;
 #pragma unroll
   for (int i = 0; i < DIM; ++i) acc[2 + i] += mflux * ua[i] + s[i] * p_mean;

@@ -124,7 +124,8 @@ class DeviceDiscretization:
                 "skew": t(np.transpose(ho.skew, (0, 2, 1))),
                 "ext": t(ho.extrap), "ext_col": t(ho.extrap.T),
                 "apply": t(ho.apply.T),
-                "face_weights": t(host.quadrature.face_weights)}
+                "face_weights": t(host.quadrature.face_weights),
+                "skew_diag": t(ho.skew_diag), "ext_diag": t(ho.extrap_diag)}
             c["esdg"] = _lib.MgEsdg(
                 nh=ho.nh, pad0=0, **{k: v.data_ptr() for k, v in arr.items()})
         return c["esdg"]

@@ -609,6 +609,10 @@ class HybridOperators:
     * ``skew (dim, nq, nq)``: ``Q_r - Q_r^T`` with ``Q_r = W V_q D_r P_q``
     * ``extrap (Nf*nfq, nq)``: ``E = V_f P_q``
     * ``apply (Np, nh)``: ``M^-1 V_h^T``
+
+    ``skew_diag`` and ``extrap_diag`` hold ``skew`` and ``extrap`` along
+    their cyclic diagonals (the order in which the device kernel pairs the
+    points).
     """
 
     vh: np.ndarray
@@ -620,6 +624,20 @@ class HybridOperators:
     @property
     def nh(self):
         return self.vh.shape[0]
+
+    @property
+    def skew_diag(self):
+        """``(dim, nq, nq)``: ``[r][s][h] = skew[r][h][(h + s) mod nq]``."""
+        nq = self.skew.shape[1]
+        h = np.arange(nq)[None, :]
+        return self.skew[:, h, (h + np.arange(nq)[:, None]) % nq]
+
+    @property
+    def extrap_diag(self):
+        """``(nq, Nf*nfq)``: ``[s][m] = extrap[m][(m + s) mod nq]``."""
+        nft, nq = self.extrap.shape
+        m = np.arange(nft)[None, :]
+        return self.extrap[m, (m + np.arange(nq)[:, None]) % nq]
 
 
 @lru_cache(maxsize=None)
