@@ -1834,31 +1834,46 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   }
   }
   __syncthreads();
-  // 5: dQ/dt = -M^-1 V_h^T acc
-  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
-    const int le = t / NP;
-    const int i = t - le * NP;
-    double out[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) out[f] = 0.0;
-    for (int h = 0; h < nq; ++h) {
-      const double w = ea.apply[h * NP + i];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) {
-        const double* sa = s_acc + (f * epb + le) * nacc + h;
-        double v = 0.0;
-        for (int cc = 0; cc < C; ++cc) v += sa[cc * nq];
-        out[f] += w * v;
-      }
+  // 5: dQ/dt = -M^-1 V_h^T acc: the volume rows' pieces summed in place,
+  // then one (field, element, node) output per thread
+  if (C > 1) {
+    for (int t = threadIdx.x; t < NFLD * ne * nq; t += blockDim.x) {
+      const int fl = t / nq;
+      const int h = t - fl * nq;
+      const int f = fl / ne, le = fl - f * ne;
+      double* sa = s_acc + (f * epb + le) * nacc + h;
+      double v = sa[0];
+      for (int cc = 1; cc < C; ++cc) v += sa[cc * nq];
+      sa[0] = v;
     }
-    for (int m = 0; m < nft; ++m) {
-      const double w = ea.apply[(nq + m) * NP + i];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f)
-        out[f] += w * s_acc[(f * epb + le) * nacc + C * nq + m];
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int fl = t / NP;
+    const int i = t - fl * NP;
+    const int f = fl / ne, le = fl - f * ne;
+    const double* sa = s_acc + (f * epb + le) * nacc;
+    const double* w = ea.apply + i;
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
+    int h = 0;
+    for (; h + 4 <= nq; h += 4) {
+      o0 += w[h * NP] * sa[h];
+      o1 += w[(h + 1) * NP] * sa[h + 1];
+      o2 += w[(h + 2) * NP] * sa[h + 2];
+      o3 += w[(h + 3) * NP] * sa[h + 3];
     }
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) a.out[f * N + e0 * NP + t] = -out[f];
+    for (; h < nq; ++h) o0 += w[h * NP] * sa[h];
+    const double* sf = sa + C * nq;
+    const double* wf = w + nq * NP;
+    int m = 0;
+    for (; m + 4 <= nft; m += 4) {
+      o0 += wf[m * NP] * sf[m];
+      o1 += wf[(m + 1) * NP] * sf[m + 1];
+      o2 += wf[(m + 2) * NP] * sf[m + 2];
+      o3 += wf[(m + 3) * NP] * sf[m + 3];
+    }
+    for (; m < nft; ++m) o0 += wf[m * NP] * sf[m];
+    a.out[f * N + (e0 + le) * NP + i] = -((o0 + o1) + (o2 + o3));
   }
 }
 
