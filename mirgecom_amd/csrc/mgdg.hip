@@ -1579,9 +1579,7 @@ __device__ __forceinline__ void esdg_pairs_64(const KArgs& a, const EsdgArgs& ea
           for (int d = 0; d < DIM; ++d) sd[d] += rx[r * DIM + d] * sr;
         }
         double fv[NFLD];
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) fv[f] = 0.0;
-        ec_flux_dot<DIM, NSP>(ph, pj[(l + s) & 63], sd, gamma, fv);
+        ec_flux_dot<DIM, NSP, true>(ph, pj[(l + s) & 63], sd, gamma, fv);
         if (s < 32) {
 #pragma unroll
           for (int f = 0; f < NFLD; ++f) {
@@ -1613,9 +1611,7 @@ __device__ __forceinline__ void esdg_pairs_64(const KArgs& a, const EsdgArgs& ea
 #pragma unroll
         for (int d = 0; d < DIM; ++d) sd[d] = b * nrm[d];
         double fv[NFLD];
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) fv[f] = 0.0;
-        ec_flux_dot<DIM, NSP>(pm, pj[(l + s) & 63], sd, gamma, fv);
+        ec_flux_dot<DIM, NSP, true>(pm, pj[(l + s) & 63], sd, gamma, fv);
 #pragma unroll
         for (int f = 0; f < NFLD; ++f) {
           acc[f] -= fv[f];

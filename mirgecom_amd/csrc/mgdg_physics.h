@@ -435,9 +435,11 @@ __device__ __forceinline__ void entropy_to_cons(const double* ev, double gamma,
 // of rho and beta = rho/(2p) taken once per point.
 template <int DIM, int NSP>
 struct EcPrim {
-  double rho, lrho, beta, lbeta, kin, p, c;
-  double u[DIM];
-  double y[NSP > 0 ? NSP : 1];
+  double rho, lrho, beta, lbeta;
+  double kh;                     // |u|^2 / 4
+  double p, c;
+  double uh[DIM];                // u / 2
+  double yh[NSP > 0 ? NSP : 1];  // Y / 2
 };
 
 template <int DIM, int NSP>
@@ -448,17 +450,18 @@ __device__ __forceinline__ void ec_prim(const double* q, double gamma,
   double u2 = 0.0;
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    a.u[d] = q[2 + d] * irho;
-    u2 += a.u[d] * a.u[d];
+    const double u = q[2 + d] * irho;
+    a.uh[d] = 0.5 * u;
+    u2 += u * u;
   }
-  a.kin = 0.5 * u2;
+  a.kh = 0.25 * u2;
   a.p = (gamma - 1.0) * (q[1] - 0.5 * q[0] * u2);
   a.beta = 0.5 * q[0] / a.p;
   a.lrho = log(a.rho);
   a.lbeta = log(a.beta);
   a.c = sqrt(u2) + sqrt(gamma * a.p * irho);   // wavespeed |u| + c
 #pragma unroll
-  for (int k = 0; k < NSP; ++k) a.y[k] = q[2 + DIM + k] * irho;
+  for (int k = 0; k < NSP; ++k) a.yh[k] = 0.5 * q[2 + DIM + k] * irho;
 }
 
 // 1/x from the hardware reciprocal estimate and two Newton steps (a few
@@ -490,10 +493,11 @@ __device__ __forceinline__ void ln_mean_parts(double x, double lx, double y, dou
 }
 
 // sum_d s_d F_S,d(a, b) of Chandrashekar's entropy-conserving, kinetic-
-// energy-preserving flux (mirgecom/inviscid.py:451-518), added to acc.  The
-// energy flux needs 1 / ((gamma - 1) beta_mean): it is formed as
-// den / num of beta's logarithmic mean.
-template <int DIM, int NSP>
+// energy-preserving flux (mirgecom/inviscid.py:451-518), added to acc (SET:
+// stored into acc).  The energy flux needs 1 / ((gamma - 1) beta_mean): it
+// is formed as den / num of beta's logarithmic mean; p_mean =
+// rho_avg / (2 beta_avg) = (rho_a + rho_b) / (2 (beta_a + beta_b)).
+template <int DIM, int NSP, bool SET = false>
 __device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
                                             const EcPrim<DIM, NSP>& b,
                                             const double* s, double gamma,
@@ -503,25 +507,30 @@ __device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
   ln_mean_parts(a.beta, a.lbeta, b.beta, b.lbeta, bn, bd);
   const double rho_mean = rn * rcp_nr(rd);
   const double inv_beta_mean = bd * rcp_nr(bn);
-  const double inv_gm1 = 1.0 / (gamma - 1.0);
-  const double rho_avg = 0.5 * (a.rho + b.rho);
-  const double beta_avg = 0.5 * (a.beta + b.beta);
-  const double p_mean = 0.5 * rho_avg * rcp_nr(beta_avg);
+  const double half_inv_gm1 = 0.5 / (gamma - 1.0);
+  const double p_mean = 0.5 * (a.rho + b.rho) * rcp_nr(a.beta + b.beta);
   double un = 0.0, ua2 = 0.0, ua[DIM];
 #pragma unroll
   for (int d = 0; d < DIM; ++d) {
-    ua[d] = 0.5 * (a.u[d] + b.u[d]);
+    ua[d] = a.uh[d] + b.uh[d];
     un += s[d] * ua[d];
     ua2 += ua[d] * ua[d];
   }
   const double mflux = rho_mean * un;
-  acc[0] += mflux;
-  acc[1] += mflux * (0.5 * (inv_beta_mean * inv_gm1 - (a.kin + b.kin)) + ua2) +
-            p_mean * un;
+  double v[2 + DIM + NSP];
+  v[0] = mflux;
+  v[1] = mflux * (inv_beta_mean * half_inv_gm1 - (a.kh + b.kh) + ua2) + p_mean * un;
 #pragma unroll
-  for (int i = 0; i < DIM; ++i) acc[2 + i] += mflux * ua[i] + s[i] * p_mean;
+  for (int i = 0; i < DIM; ++i) v[2 + i] = mflux * ua[i] + s[i] * p_mean;
 #pragma unroll
-  for (int k = 0; k < NSP; ++k) acc[2 + DIM + k] += mflux * 0.5 * (a.y[k] + b.y[k]);
+  for (int k = 0; k < NSP; ++k) v[2 + DIM + k] = mflux * (a.yh[k] + b.yh[k]);
+#pragma unroll
+  for (int f = 0; f < 2 + DIM + NSP; ++f) {
+    if constexpr (SET)
+      acc[f] = v[f];
+    else
+      acc[f] += v[f];
+  }
 }
 
 // }}}
