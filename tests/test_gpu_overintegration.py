@@ -146,10 +146,12 @@ def _analytic(x, lib):
     return lib.stack(rows)
 
 
+@pytest.mark.parametrize("order", [2, 3])
 @pytest.mark.parametrize("esdg", [False, True])
-def test_boundaries_3d(esdg):
+def test_boundaries_3d(esdg, order):
     """Prescribed analytic state on every face of a 3-D box: boundary
-    quadrature points, their ordering and normals in 3-D."""
+    quadrature points, their ordering and normals in 3-D (p = 3 with ESDG
+    runs the paired pair loop)."""
     from mirgecom_amd.boundary import PrescribedFluidBoundary
     from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
     from mirgecom_amd.eos import IdealSingleGas
@@ -161,7 +163,7 @@ def test_boundaries_3d(esdg):
     from oracle import operators as oop, physics as ph
     from oracle.esdg import esdg_euler_operator
     from oracle.overintegration import euler_operator_quad
-    pb = Problem(3, 2, (3, 2, 2), 0.0, (1.0, 0.8, 0.6), (False,) * 3)
+    pb = Problem(3, order, (3, 2, 2), 0.0, (1.0, 0.8, 0.6), (False,) * 3)
     gm = GasModel(eos=IdealSingleGas(gamma=1.4, gas_const=1.0))
     ogm = ph.GasModel(ph.IdealSingleGas(gamma=1.4, gas_const=1.0))
     q = _analytic(pb.x, np)
@@ -177,7 +179,7 @@ def test_boundaries_3d(esdg):
     st = make_fluid_state(ConservedVars(3, _dev(q)), gm)
     gb = {"all": PrescribedFluidBoundary(boundary_state_func=gstate)}
     ob = {"all": oop.PrescribedFluidBoundary(ostate)}
-    print(f"3-D boundaries esdg={esdg}", flush=True)
+    print(f"3-D boundaries esdg={esdg} p={order}", flush=True)
     if esdg:
         got = euler_operator(pb.dcoll, st, gm, gb, use_esdg=True,
                              quadrature_tag=DISCR_TAG_QUAD,
