@@ -180,18 +180,37 @@ def test_boundaries_3d(esdg, order):
     gb = {"all": PrescribedFluidBoundary(boundary_state_func=gstate)}
     ob = {"all": oop.PrescribedFluidBoundary(ostate)}
     print(f"3-D boundaries esdg={esdg} p={order}", flush=True)
+    # The RHS of this near-uniform state peaks at 0.06-0.18 against O(1)
+    # fluxes, so round-off shows 10-20x larger relative to it than in the
+    # periodic cases, and it grows with p (the oracle's monomial basis):
+    # measured 2e-12 / 1.2e-11 (quadrature) and 2.2e-11 / 1.35e-10 (ESDG)
+    # at p = 2 / 3, the same for the paired and the row-by-row ESDG loops.
     if esdg:
-        got = euler_operator(pb.dcoll, st, gm, gb, use_esdg=True,
-                             quadrature_tag=DISCR_TAG_QUAD,
-                             inviscid_numerical_flux_func=entropy_stable_inviscid_facial_flux_rusanov)
+        def run():
+            return _host(euler_operator(
+                pb.dcoll, st, gm, gb, use_esdg=True, quadrature_tag=DISCR_TAG_QUAD,
+                inviscid_numerical_flux_func=entropy_stable_inviscid_facial_flux_rusanov).q)
+        got = run()
         ref = esdg_euler_operator(pb.odisc, pb.box, q, 1.4, ob)
-        tol = 5e-11   # entropy projection (log/exp/pow) and 28k-term row sums
+        tol = 5e-11 if order == 2 else 3e-10
+        if order == 3:
+            # the paired pair loop against the row-by-row loop on the device
+            from mirgecom_amd.device import get_device_discretization
+            ops = get_device_discretization(pb.dcoll, st.cv.q.device).esdg_ops()
+            diag = ops.skew_diag, ops.ext_diag
+            ops.skew_diag = ops.ext_diag = None
+            try:
+                rows = run()
+            finally:
+                ops.skew_diag, ops.ext_diag = diag
+            assert rel_err(got, rows).max() < 1e-11
     else:
-        got = euler_operator(pb.dcoll, st, gm, gb, quadrature_tag=DISCR_TAG_QUAD,
-                             inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
+        got = _host(euler_operator(
+            pb.dcoll, st, gm, gb, quadrature_tag=DISCR_TAG_QUAD,
+            inviscid_numerical_flux_func=inviscid_facial_flux_rusanov).q)
         ref = euler_operator_quad(pb.odisc, pb.box, q, ogm, ob)
-        tol = 1e-11
-    err = rel_err(_host(got.q), ref)
+        tol = 1e-11 if order == 2 else 3e-11
+    err = rel_err(got, ref)
     print("err", err)
     assert err.max() < tol
 
