@@ -333,7 +333,7 @@ mg_status mg_transport_vars(const mg_gas* gas, int32_t dim, int64_t n,
 /* Message of the last failed call on this thread. */
 const char* mg_last_error(void);
 
-/* Library ABI version. */
+/* Library ABI version (4: mg_esdg carries skew_diag and ext_diag). */
 int32_t mg_abi_version(void);
 
 #ifdef __cplusplus

@@ -2520,7 +2520,7 @@ extern "C" {
 
 const char* mg_last_error(void) { return g_err.c_str(); }
 
-int32_t mg_abi_version(void) { return 3; }
+int32_t mg_abi_version(void) { return 4; }
 
 /* Upload the uiuc_7sp mechanism constants (host array of 7*7*2 + 7 + 7 + 7
  * + 2 + 9 doubles, packed as in mgdg::MechDev). */
