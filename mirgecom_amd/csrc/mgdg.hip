@@ -1197,7 +1197,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
 // s_q [F][epb][NP] the state (then the result), s_g [F*epb][gs] the inviscid
 // flux in reference directions (rx . F) at the volume points (r-major rows
 // of DIM*nq), s_f [F*epb][fs] minus the numerical flux times the face
-// Jacobian ratio at the face points; the row strides gs, fs are = 4 mod 32
+// Jacobian ratio at the face points; the row strides gs, fs are 4 mod 32
 // doubles so that the matrix-core operand reads of 16 rows spread over the
 // LDS banks.  Lanes own quadrature points in phases 2-3; phase 4 is one
 // (NP x K) x (K x F*epb) product on the matrix cores.
@@ -1543,23 +1543,25 @@ __global__ __launch_bounds__(kBlock) void esdg_project_kernel(KArgs a, EsdgArgs 
 // block's part of the volume rows and is back in place after 64 steps.
 // Results: volume rows in the four nq-long pieces of s_acc (piece 0 from
 // wave 0, piece 1 + b from wave 1 + b; phase 5 adds them), face rows plus
-// the face flux in the face slots.
+// the face flux in the face slots.  One element per workgroup: thread
+// 64 + m has computed face point m's flux (fkeep) in phase 3.
 template <int DIM, int NSP>
 __device__ __forceinline__ void esdg_pairs_64(const KArgs& a, const EsdgArgs& ea,
-                                              const EcPrim<DIM, NSP>* s_prim,
-                                              const double* s_fs, double* s_acc,
-                                              int64_t e0, int ne, int nacc, double gamma) {
+                                              const EcPair<DIM, NSP>* s_prim,
+                                              const double (&fkeep)[2 + DIM + NSP],
+                                              double* s_acc, int64_t e, int nacc,
+                                              double gamma) {
   constexpr int NF = DIM + 1;
   constexpr int NFLD = 2 + DIM + NSP;
-  using Prim = EcPrim<DIM, NSP>;
-  constexpr int NQ = 64, NFT = 192, NH = NQ + NFT;
-  const int epb = ea.epb, nfq = ea.nfq;
+  using Prim = EcPair<DIM, NSP>;
+  constexpr int NQ = 64, NFT = 192;
+  const int nfq = ea.nfq;
   const int l = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int down = (l + 1) & 63;
-  for (int le = 0; le < ne; ++le) {
-    const int64_t e = e0 + le;
-    const Prim* pj = s_prim + le * NH;
+  {
+    const int le = 0, epb = 1;
+    const Prim* pj = s_prim;
     double acc[NFLD], mv[NFLD];
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) acc[f] = mv[f] = 0.0;
@@ -1622,10 +1624,84 @@ __device__ __forceinline__ void esdg_pairs_64(const KArgs& a, const EsdgArgs& ea
       for (int f = 0; f < NFLD; ++f) {
         double* row = s_acc + (f * epb + le) * nacc;
         row[wave * NQ + l] = mv[f];
-        row[4 * NQ + m] = acc[f] + s_fs[(f * epb + le) * NFT + m];
+        row[4 * NQ + m] = acc[f] + fkeep[f];
       }
     }
   }
+}
+
+// Entropy-stable face flux times fscale * w at face point m of element
+// e0 + le (phase 3 of esdg_kernel): the own trace state from s_qf, the
+// neighbour's from its projected entropy variables at the shared face nodes
+// (s_vp), on boundary faces the re-projected interior state and the
+// boundary condition's exterior state.
+template <int DIM, int P, int NSP>
+__device__ __forceinline__ void esdg_face_flux(const KArgs& a, const EsdgArgs& ea,
+                                               const double* s_qf, const double* s_vp,
+                                               const double* s_q, int64_t e0, int le, int m,
+                                               double gamma, double* out) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + NSP;
+  using Prim = EcPrim<DIM, NSP>;
+  const int nfq = ea.nfq, epb = ea.epb, nft = NF * ea.nfq;
+  const int fc = m / nfq;
+  const int mm = m - fc * nfq;
+  const int64_t e = e0 + le;
+  const double* w = ea.finterp + fc * NFP * nfq + mm;
+  const int32_t v0 = a.vmap_p[(e * NF + fc) * NFP];
+  double qm[NFLD], qp[NFLD];
+  if (v0 >= 0) {
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) qm[f] = s_qf[(f * epb + le) * nft + m];
+    double vp[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) vp[f] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) {
+      const double wj = w[j * nfq];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+        vp[f] += wj * s_vp[(f * epb + le) * NF * NFP + fc * NFP + j];
+    }
+    entropy_to_cons<DIM, NSP>(vp, gamma, qp);
+  } else {
+    // boundary: the interpolated interior state (through the entropy
+    // variables and back, as the reference's projection does) and the
+    // boundary condition's exterior state
+    double ui[NFLD], vi[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) ui[f] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) {
+      const double wj = w[j * nfq];
+      const int ij = a.fmask[fc * NFP + j];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) ui[f] += wj * s_q[(f * epb + le) * NP + ij];
+    }
+    cons_to_entropy<DIM, NSP>(ui, gamma, vi);
+    entropy_to_cons<DIM, NSP>(vi, gamma, qm);
+    const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + mm;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) qp[f] = ea.qbq[f * ea.nbq + bq];
+  }
+  Prim pm, pp;
+  ec_prim<DIM, NSP>(qm, gamma, pm);
+  ec_prim<DIM, NSP>(qp, gamma, pp);
+  double n[DIM], fs[NFLD];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
+#pragma unroll
+  for (int f = 0; f < NFLD; ++f) fs[f] = 0.0;
+  ec_flux_dot<DIM, NSP>(pm, pp, n, gamma, fs);
+  if (ea.dissipation) {
+    const double alpha = fmax(pm.c, pp.c);
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) fs[f] -= 0.5 * alpha * (qp[f] - qm[f]);
+  }
+  const double sc = a.fscale[e * NF + fc] * ea.wf[mm];
+#pragma unroll
+  for (int f = 0; f < NFLD; ++f) out[f] = sc * fs[f];
 }
 
 template <int DIM, int P, int NSP>
@@ -1633,18 +1709,19 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   using E = Elem<DIM, P>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
   constexpr int NFLD = 2 + DIM + NSP;
-  using Prim = EcPrim<DIM, NSP>;
+  using Pair = EcPair<DIM, NSP>;
   extern __shared__ double lds[];
   const int nq = ea.nq, nfq = ea.nfq, nh = ea.nh, epb = ea.epb, nft = NF * ea.nfq;
-  Prim* s_prim = reinterpret_cast<Prim*>(lds);             // [epb][nh]
+  Pair* s_prim = reinterpret_cast<Pair*>(lds);             // [epb][nh]
   double* s_v = reinterpret_cast<double*>(s_prim + epb * nh);  // [F][epb][NP]
   double* s_q = s_v + NFLD * epb * NP;                      // [F][epb][NP]
   double* s_vp = s_q + NFLD * epb * NP;                     // [F][epb][NF*NFP]
   // phase 4 splits volume rows into C pieces of nq entries
   const int C = (nh + nq - 1) / nq;
   const int nacc = C * nq + nft;                            // row sums per field
-  double* s_fs = s_vp + NFLD * epb * NF * NFP;              // [F][epb][nft]
-  double* s_qf = s_fs + NFLD * epb * nft;                   // [F][epb][nft], phases 2-3
+  // face fluxes [F][epb][nft] (not in the paired layout: kept in registers)
+  double* s_fs = s_vp + NFLD * epb * NF * NFP;
+  double* s_qf = ea.paired ? s_fs : s_fs + NFLD * epb * nft;  // [F][epb][nft], 2-3
   double* s_acc = s_qf;                                     // [F][epb][nacc], phases 4-5
   const int64_t e0 = int64_t(blockIdx.x) * epb;
   const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
@@ -1681,7 +1758,9 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
       for (int f = 0; f < NFLD; ++f) v[f] += w * s_v[(f * epb + le) * NP + i];
     }
     entropy_to_cons<DIM, NSP>(v, gamma, u);
-    ec_prim<DIM, NSP>(u, gamma, s_prim[le * nh + h]);
+    EcPrim<DIM, NSP> pr;
+    ec_prim<DIM, NSP>(u, gamma, pr);
+    s_prim[le * nh + h] = pr;
     if (h >= nq) {
 #pragma unroll
       for (int f = 0; f < NFLD; ++f) s_qf[(f * epb + le) * nft + (h - nq)] = u[f];
@@ -1689,66 +1768,24 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   }
   __syncthreads();
   // 3: entropy-stable face flux at the face points, times fscale * w
-  for (int t = threadIdx.x; t < ne * nft; t += blockDim.x) {
-    const int le = t / nft;
-    const int m = t - le * nft;
-    const int fc = m / nfq;
-    const int mm = m - fc * nfq;
-    const int64_t e = e0 + le;
-    const double* w = ea.finterp + fc * NFP * nfq + mm;
-    const int32_t v0 = a.vmap_p[(e * NF + fc) * NFP];
-    double qm[NFLD], qp[NFLD];
-    if (v0 >= 0) {
+  double fkeep[NFLD];
 #pragma unroll
-      for (int f = 0; f < NFLD; ++f) qm[f] = s_qf[(f * epb + le) * nft + m];
-      double vp[NFLD];
+  for (int f = 0; f < NFLD; ++f) fkeep[f] = 0.0;
+  if (ea.paired) {
+    // one element per workgroup: thread 64 + m keeps face point m's flux
+    // for its face row in phase 4
+    const int m = int(threadIdx.x) - 64;
+    if (m >= 0 && m < nft && ne == 1)
+      esdg_face_flux<DIM, P, NSP>(a, ea, s_qf, s_vp, s_q, e0, 0, m, gamma, fkeep);
+  } else {
+    for (int t = threadIdx.x; t < ne * nft; t += blockDim.x) {
+      const int le = t / nft;
+      const int m = t - le * nft;
+      double fs[NFLD];
+      esdg_face_flux<DIM, P, NSP>(a, ea, s_qf, s_vp, s_q, e0, le, m, gamma, fs);
 #pragma unroll
-      for (int f = 0; f < NFLD; ++f) vp[f] = 0.0;
-#pragma unroll
-      for (int j = 0; j < NFP; ++j) {
-        const double wj = w[j * nfq];
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f)
-          vp[f] += wj * s_vp[(f * epb + le) * NF * NFP + fc * NFP + j];
-      }
-      entropy_to_cons<DIM, NSP>(vp, gamma, qp);
-    } else {
-      // boundary: the interpolated interior state (through the entropy
-      // variables and back, as the reference's projection does) and the
-      // boundary condition's exterior state
-      double ui[NFLD], vi[NFLD];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) ui[f] = 0.0;
-#pragma unroll
-      for (int j = 0; j < NFP; ++j) {
-        const double wj = w[j * nfq];
-        const int ij = a.fmask[fc * NFP + j];
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) ui[f] += wj * s_q[(f * epb + le) * NP + ij];
-      }
-      cons_to_entropy<DIM, NSP>(ui, gamma, vi);
-      entropy_to_cons<DIM, NSP>(vi, gamma, qm);
-      const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + mm;
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) qp[f] = ea.qbq[f * ea.nbq + bq];
+      for (int f = 0; f < NFLD; ++f) s_fs[(f * epb + le) * nft + m] = fs[f];
     }
-    Prim pm, pp;
-    ec_prim<DIM, NSP>(qm, gamma, pm);
-    ec_prim<DIM, NSP>(qp, gamma, pp);
-    double n[DIM], fs[NFLD];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) fs[f] = 0.0;
-    ec_flux_dot<DIM, NSP>(pm, pp, n, gamma, fs);
-    if (ea.dissipation) {
-      const double alpha = fmax(pm.c, pp.c);
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) fs[f] -= 0.5 * alpha * (qp[f] - qm[f]);
-    }
-    const double sc = a.fscale[e * NF + fc] * ea.wf[mm];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_fs[(f * epb + le) * nft + m] = sc * fs[f];
   }
   __syncthreads();
   // 4: rows of (Q_h - Q_h^T) o F_S, plus the face flux on face rows.  A
@@ -1756,7 +1793,7 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   // volume rows are cut into C = ceil(nh / nq) pieces of nq entries so that
   // every work item sums nq pairs; the pieces are added in phase 5.
   if (ea.paired) {
-    esdg_pairs_64<DIM, NSP>(a, ea, s_prim, s_fs, s_acc, e0, ne, nacc, gamma);
+    esdg_pairs_64<DIM, NSP>(a, ea, s_prim, fkeep, s_acc, e0, nacc, gamma);
   } else {
   const int nitems = nq * C + nft;
   for (int t = threadIdx.x; t < ne * nitems; t += blockDim.x) {
@@ -1775,8 +1812,8 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
       j1 = nq;
     }
     const int64_t e = e0 + le;
-    const Prim ph = s_prim[le * nh + h];
-    const Prim* pj = s_prim + le * nh;
+    const Pair ph = s_prim[le * nh + h];
+    const Pair* pj = s_prim + le * nh;
     double acc[NFLD];
 #pragma unroll
     for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
@@ -2988,11 +3025,13 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
     if ((s = check_launch("esdg_project_kernel"))) return s;                            \
     const size_t nchunk = (size_t(esdg->nh) + quad->nq - 1) / quad->nq;               \
     const size_t nacc = std::max(nchunk * quad->nq + nft, nft);                         \
-    const size_t per_e = sizeof(mgdg::EcPrim<D, NS>) * esdg->nh +                       \
+    const size_t per_e = sizeof(mgdg::EcPair<D, NS>) * esdg->nh +                       \
         sizeof(double) * nfld *                                                         \
-            (2 * size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp + nft + nacc);     \
+            (2 * size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp +                  \
+             (ea.paired ? 0 : nft) + nacc);                                             \
     if (per_e > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
-    int epb = int(std::max<size_t>(1, std::min<size_t>(16, (64 * 1024) / per_e)));     \
+    int epb = ea.paired ? 1                                                             \
+                        : int(std::max<size_t>(1, std::min<size_t>(16, (64 * 1024) / per_e))); \
     ea.epb = epb;                                                                       \
     auto kern = mgdg::esdg_kernel<D, P, NS>;                                            \
     if ((s = set_lds(kern, kBudget))) return s;                                         \

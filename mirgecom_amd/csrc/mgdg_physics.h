@@ -433,13 +433,30 @@ __device__ __forceinline__ void entropy_to_cons(const double* ev, double gamma,
 
 // Per-point quantities of the entropy-conserving flux, with the logarithms
 // of rho and beta = rho/(2p) taken once per point.
+template <int NSP>
+struct EcSpecies {
+  double yh[NSP];  // Y / 2
+};
+// Without species one padding double makes a 3-D point 9 doubles: an odd
+// stride in 8-byte words, so that lanes reading consecutive points from LDS
+// spread over the banks (8 doubles put 64 lanes on 4 bank groups).
+template <>
+struct EcSpecies<0> {
+  double pad_;
+};
+
+// What the two-point flux reads of a point (kept in LDS per hybridized point).
 template <int DIM, int NSP>
-struct EcPrim {
+struct EcPair : EcSpecies<NSP> {
   double rho, lrho, beta, lbeta;
-  double kh;                     // |u|^2 / 4
+  double kh;       // |u|^2 / 4
+  double uh[DIM];  // u / 2
+};
+
+// ... with the pressure and the wavespeed |u| + c the face flux needs.
+template <int DIM, int NSP>
+struct EcPrim : EcPair<DIM, NSP> {
   double p, c;
-  double uh[DIM];                // u / 2
-  double yh[NSP > 0 ? NSP : 1];  // Y / 2
 };
 
 template <int DIM, int NSP>
@@ -460,8 +477,8 @@ __device__ __forceinline__ void ec_prim(const double* q, double gamma,
   a.lrho = log(a.rho);
   a.lbeta = log(a.beta);
   a.c = sqrt(u2) + sqrt(gamma * a.p * irho);   // wavespeed |u| + c
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) a.yh[k] = 0.5 * q[2 + DIM + k] * irho;
+  if constexpr (NSP > 0)
+    for (int k = 0; k < NSP; ++k) a.yh[k] = 0.5 * q[2 + DIM + k] * irho;
 }
 
 // 1/x from the hardware reciprocal estimate and two Newton steps (a few
@@ -498,8 +515,8 @@ __device__ __forceinline__ void ln_mean_parts(double x, double lx, double y, dou
 // is formed as den / num of beta's logarithmic mean; p_mean =
 // rho_avg / (2 beta_avg) = (rho_a + rho_b) / (2 (beta_a + beta_b)).
 template <int DIM, int NSP, bool SET = false>
-__device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
-                                            const EcPrim<DIM, NSP>& b,
+__device__ __forceinline__ void ec_flux_dot(const EcPair<DIM, NSP>& a,
+                                            const EcPair<DIM, NSP>& b,
                                             const double* s, double gamma,
                                             double* acc) {
   double rn, rd, bn, bd;
@@ -522,8 +539,8 @@ __device__ __forceinline__ void ec_flux_dot(const EcPrim<DIM, NSP>& a,
   v[1] = mflux * (inv_beta_mean * half_inv_gm1 - (a.kh + b.kh) + ua2) + p_mean * un;
 #pragma unroll
   for (int i = 0; i < DIM; ++i) v[2 + i] = mflux * ua[i] + s[i] * p_mean;
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) v[2 + DIM + k] = mflux * (a.yh[k] + b.yh[k]);
+  if constexpr (NSP > 0)
+    for (int k = 0; k < NSP; ++k) v[2 + DIM + k] = mflux * (a.yh[k] + b.yh[k]);
 #pragma unroll
   for (int f = 0; f < 2 + DIM + NSP; ++f) {
     if constexpr (SET)
