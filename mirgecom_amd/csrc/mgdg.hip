@@ -1,2561 +1,66 @@
-// MI355X (gfx950) kernels of the DG right-hand side and their C ABI.
-//
-// Element-batched kernels: a workgroup owns EPB consecutive elements and runs
-// three phases over them with LDS as the exchange buffer:
-//   1. volume nodes   (thread per node): load the state (and gradients),
-//      evaluate the pointwise physics and the contravariant fluxes
-//      G_a = sum_i (dr_a/dx_i) F_i  ->  LDS
-//   2. face nodes     (thread per face node): coalesced loads of the own
-//      trace and indexed loads of the neighbour trace (interior_trace_pairs),
-//      numerical flux times the face/volume Jacobian ratio  ->  LDS
-//   3. volume nodes   (thread per node): the element-local contractions
-//      sum_a Dw_a G_a - LIFT f*   (weak divergence, face mass and inverse
-//      mass fused, mirgecom/operators.py:66-95), written once to HBM.
-// The reference matrices (Dw_a = M^-1 D_a^T M, LIFT = M^-1 E) live in LDS,
-// stored transposed so that consecutive nodes read consecutive banks.
-#include <hip/hip_runtime.h>
-#include <algorithm>
+// MI355X (gfx950) DG right-hand-side library: the C ABI (include/mgdg.h)
+// and the pointwise entry points.  The element-batched kernels live in
+// mgdg_kernels.h; their instantiations are spread over mg_disp_*.hip and the
+// quadrature / entropy-stable entry points over mg_quad.hip, so that the
+// files compile in parallel.
+#include <mutex>
+#include <vector>
 
-#include <cstdio>
-#include <cstring>
-#include <string>
+#include "mgdg_host.h"
 
-#include "../../include/mgdg.h"
-#include "mgdg_physics.h"
+// {{{ state shared by all files of the library
 
-namespace mgdg {
+namespace mgdg_shared {
 
-__constant__ MechDev c_mech;
-__constant__ PolyDev c_poly;
-__constant__ TranDev c_tran;
-
-constexpr int kBlock = 256;
-constexpr int kLdsBudget = 64 * 1024;
-
-template <int DIM, int P>
-struct Elem {
-  static constexpr int NP = binom(P + DIM, DIM);
-  static constexpr int NFP = binom(P + DIM - 1, DIM - 1);
-  static constexpr int NF = DIM + 1;
-  static constexpr int NFN = NF * NFP;
-  static constexpr int MAT = DIM * NP * NP + NFN * NP;  // doubles of Dw + LIFT
-};
-
-struct KArgs {
-  const double* q;
-  const double* temp;
-  const double* grad;
-  const double* qb;
-  const double* tb;
-  double* out;
-  const double* rx;
-  const double* normals;
-  const double* fscale;
-  const int32_t* vmap_p;
-  const double* dw;
-  const double* lift;
-  const int32_t* fmask;
-  const double* ops;   // packed operators (mg_mesh_prepare)
-  const double* fnet;  // nodal net flux [F*DIM][N] (kFlux)
-  const double* fnb;   // exterior net normal flux [F][nbnd] (kFlux)
-  const float* fnet32; // nodal net flux, FP32 (mixed precision)
-  float* fnet32_out;   // FP32 nodal net flux output (mixed precision)
-  double* grad_out;    // optional gradient output of the fused flux kernel
-  double* fnb_out;     // exterior net normal flux output of the flux kernel
-  // Mengaldo boundary data per boundary node (NULL wflag: none)
-  const double* wq;     // CV_bc [F][nbnd]
-  const double* wt;     // T_bc [nbnd]
-  const double* wts;    // temperature of state_bc [nbnd] (mixtures)
-  const int32_t* wflag; // MG_BC_* bits [nbnd]
-  const int32_t* wvol;  // interior volume node of each boundary node [nbnd]
-  const double* wn;     // unit normal [DIM][nbnd]
-  double* wwork;        // boundary gradients / viscous fluxes [(F+1)*DIM][nbnd]
-  int64_t K;
-  int64_t nbnd;
-  int64_t nnodes;      // K * NP
-  GasDev gas;
-  int flux_kind;
-  int inviscid_on;
-};
-
-template <int DIM, int P>
-__device__ __forceinline__ void load_matrices(double* s_dw, double* s_lift,
-                                              int* s_fmask, const KArgs& a) {
-  using E = Elem<DIM, P>;
-  // s_dw[(r*NP + j)*NP + i] = Dw_r[i][j];  s_lift[m*NP + i] = LIFT[i][m]
-  for (int t = threadIdx.x; t < DIM * E::NP * E::NP; t += blockDim.x) {
-    const int r = t / (E::NP * E::NP);
-    const int rem = t - r * E::NP * E::NP;
-    const int i = rem / E::NP;
-    const int j = rem - i * E::NP;
-    s_dw[(r * E::NP + j) * E::NP + i] = a.dw[t];
-  }
-  for (int t = threadIdx.x; t < E::NP * E::NFN; t += blockDim.x) {
-    const int i = t / E::NFN;
-    const int m = t - i * E::NFN;
-    s_lift[m * E::NP + i] = a.lift[t];
-  }
-  for (int t = threadIdx.x; t < E::NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
+std::string& err() {
+  thread_local std::string s;
+  return s;
 }
 
-template <int DIM, int NSP>
-__device__ __forceinline__ void load_q(Node<DIM, NSP>& s, const double* q,
-                                       int64_t stride, int64_t g) {
-#pragma unroll
-  for (int f = 0; f < Node<DIM, NSP>::NFLD; ++f) s.q[f] = q[f * stride + g];
+bool& mech_loaded() {
+  static bool f = false;
+  return f;
 }
 
-template <int DIM, int NSP>
-__device__ __forceinline__ void load_grad(double (*gq)[DIM], double* gt,
-                                          const double* grad, int64_t stride,
-                                          int64_t g) {
-  constexpr int NFLD = 2 + DIM + NSP;
-#pragma unroll
-  for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) gq[f][d] = grad[(f * DIM + d) * stride + g];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) gt[d] = grad[(NFLD * DIM + d) * stride + g];
+bool& tran_loaded() {
+  static bool f = false;
+  return f;
 }
-
-// {{{ packed element operators
-//
-// The contraction phase assigns one (element, field) column per lane and
-// splits the element's output rows over the kWaves waves of a workgroup.
-// Every lane of a wave then needs the same operator entry at the same time,
-// so the operators are read with wave-uniform addresses (scalar loads into
-// SGPRs) from a packed copy in which each wave's rows are contiguous:
-//   dvol [w][j][r][rr]     Dw_r[i][j]       (divergence and gradient volume)
-//   lift [w][m][rr]        LIFT[i][m]
-// with i = w*RPW + rr (rows >= NP are zero).  The workspace holds the FP64
-// copy followed by an FP32 copy for the mixed-precision kernels.
-
-constexpr int kWaves = kBlock / 64;
-
-template <int DIM, int P>
-struct Ops {
-  using E = Elem<DIM, P>;
-  static constexpr int RPW = (E::NP + kWaves - 1) / kWaves;
-  static constexpr int DVOL = kWaves * E::NP * DIM * RPW;
-  static constexpr int LIFT = kWaves * E::NFN * RPW;
-  static constexpr int TOTAL = DVOL + LIFT;
-  // MFMA A-fragments (FP32, v_mfma_f32_16x16x4f32) of the same operators for
-  // the mixed-precision kernels: 16-row tiles of Dw_r and of each face's
-  // LIFT block, K in steps of 4, 64 floats per fragment in lane order
-  // (lane l holds A[l % 16][l / 16]); rows >= NP and k past the end are 0.
-  static constexpr int MT = (E::NP + 15) / 16;
-  static constexpr int KSV = (E::NP + 3) / 4;
-  static constexpr int KSF = (E::NFP + 3) / 4;
-  static constexpr int MFV = DIM * MT * KSV * 64;
-  static constexpr int MFL = E::NF * MT * KSF * 64;
-  static constexpr int MF = MFV + MFL;
-  // bytes: TOTAL doubles + TOTAL floats + MF floats
-  static constexpr int BYTES = TOTAL * 12 + MF * 4;
-};
-
-template <int DIM, int P>
-__global__ __launch_bounds__(kBlock) void pack_ops_kernel(const double* dw,
-                                                          const double* lift,
-                                                          double* ws) {
-  using E = Elem<DIM, P>;
-  using O = Ops<DIM, P>;
-  constexpr int NP = E::NP, NFN = E::NFN, RPW = O::RPW;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  double v = 0.0;
-  if (t < O::DVOL) {
-    const int rr = t % RPW;
-    const int r = (t / RPW) % DIM;
-    const int j = (t / (RPW * DIM)) % NP;
-    const int w = t / (RPW * DIM * NP);
-    const int i = w * RPW + rr;
-    v = i < NP ? dw[(r * NP + i) * NP + j] : 0.0;
-  } else if (t < O::TOTAL) {
-    const int u = t - O::DVOL;
-    const int rr = u % RPW;
-    const int m = (u / RPW) % NFN;
-    const int w = u / (RPW * NFN);
-    const int i = w * RPW + rr;
-    v = i < NP ? lift[i * NFN + m] : 0.0;
-  }
-  if (t < O::TOTAL) {
-    ws[t] = v;
-    reinterpret_cast<float*>(ws + O::TOTAL)[t] = float(v);
-  } else if (t < O::TOTAL + O::MF) {
-    constexpr int NFP = E::NFP, MT = O::MT, KSV = O::KSV, KSF = O::KSF;
-    int u = t - O::TOTAL;
-    const int lane = u % 64;
-    const int i0 = lane % 16, k0 = lane / 16;
-    double a = 0.0;
-    if (u < O::MFV) {
-      const int frag = u / 64;
-      const int st = frag % KSV, mt = (frag / KSV) % MT, r = frag / (KSV * MT);
-      const int i = mt * 16 + i0, j = st * 4 + k0;
-      if (i < NP && j < NP) a = dw[(r * NP + i) * NP + j];
-    } else {
-      const int frag = (u - O::MFV) / 64;
-      const int st = frag % KSF, mt = (frag / KSF) % MT, f = frag / (KSF * MT);
-      const int i = mt * 16 + i0, m = st * 4 + k0;
-      if (i < NP && m < NFP) a = lift[i * NFN + f * NFP + m];
-    }
-    (reinterpret_cast<float*>(ws + O::TOTAL) + O::TOTAL)[u] = float(a);
-  }
-}
-
-// MFMA A-fragments of the packed operators (see Ops).
-template <int DIM, int P>
-__device__ __forceinline__ const float* mfma_ops(const double* ws) {
-  return reinterpret_cast<const float*>(ws + Ops<DIM, P>::TOTAL) + Ops<DIM, P>::TOTAL;
-}
-
-typedef float floatx4 __attribute__((ext_vector_type(4)));
-typedef double doublex4 __attribute__((ext_vector_type(4)));
-
-template <typename R, int DIM, int P>
-__device__ __forceinline__ const R* packed_ops(const double* ws) {
-  if constexpr (sizeof(R) == 8) {
-    return reinterpret_cast<const R*>(ws);
-  } else {
-    return reinterpret_cast<const R*>(ws + Ops<DIM, P>::TOTAL);
-  }
-}
-
-// Coalesced write-out of a block's staged results: stage[plane][le*NP + i]
-// -> out[plane*N + (e0 + le)*NP + i].
-template <int NP, typename S, typename T>
-__device__ __forceinline__ void write_staged(const S* stage, int nplanes,
-                                             int epb, int pstride, int64_t e0,
-                                             int64_t K, T* __restrict__ out) {
-  const int64_t N = K * NP;
-  const int64_t nvalid = (K - e0 < epb ? K - e0 : epb) * NP;
-  const int per = epb * NP;
-  for (int t = threadIdx.x; t < nplanes * per; t += blockDim.x) {
-    const int plane = t / per;
-    const int idx = t - plane * per;
-    if (idx < nvalid) out[plane * N + e0 * NP + idx] = T(stage[plane * pstride + idx]);
-  }
-}
-
-// XCD-aware block order.  Workgroups are dispatched round-robin over the 8
-// XCDs (blockIdx.x % 8 picks the XCD), each with its own L2.  Consecutive
-// element blocks are mesh neighbours (elements are ordered cube by cube), so
-// the logical block index gives every XCD one contiguous range of the mesh:
-// the neighbour gathers of a block then mostly hit the L2 that loaded the
-// neighbouring blocks.  A bijection on [0, gridDim.x).
-constexpr int kXcds = 8;
-
-__device__ __forceinline__ int64_t xcd_block() {
-  const int64_t nb = gridDim.x, b = blockIdx.x;
-  const int64_t q = nb / kXcds, r = nb % kXcds;
-  const int64_t x = b % kXcds, i = b / kXcds;
-  return x * q + (x < r ? x : r) + i;
-}
-
-// LDS row strides are padded to an odd number of elements: rows written by
-// consecutive lanes then fall into different banks (an even stride of 60-64
-// doubles maps whole waves onto a few banks).
-constexpr int odd_stride(int n) { return n | 1; }
-
-// Reference pressure of the mixed-precision path: the pressure at the first
-// node of the mesh.  Subtracting a constant p_ref from the momentum flux
-// diagonal does not change the DG divergence (the weak divergence of a
-// constant flux cancels its lift exactly), but it removes the large
-// low-Mach background from the values that are rounded to FP32.
-template <int DIM, int NSP, int MIX>
-__device__ __forceinline__ double reference_pressure(const KArgs& a) {
-  Node<DIM, NSP> s;
-  load_q<DIM, NSP>(s, a.q, a.nnodes, 0);
-  node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[0] : 0.0);
-  return s.p;
-}
-
-// Mengaldo boundary nodes (boundary.py:391-793): gradient values CV_bc and
-// T_bc, and the viscous flux F_V(state_bc, grad_cv_bc, grad_t_bc) . n with
-// the boundary conditions on the gradients applied to the interior ones.
-constexpr int kBcMengaldo = 1, kBcSlip = 2, kBcImpermeable = 4, kBcAdiabatic = 8;
-
-__device__ __forceinline__ int wall_flags(const KArgs& a, int64_t b) {
-  return a.wflag ? a.wflag[b] : 0;
-}
-
-// Effective exterior viscous normal flux 2 F_V(bc).n - F_V(q-, grad-).n: the
-// face flux averages both sides, which then gives F_V(bc).n.  gq/gt hold the
-// interior gradients on entry and the boundary-condition gradients on exit.
-template <int DIM, int NSP, int MIX>
-__device__ void wall_viscous_flux_n(const KArgs& a, int64_t b, int64_t gm, int fl,
-                                    const double* n, double (*gq)[DIM], double* gt,
-                                    double* fv) {
-  constexpr int NFLD = 2 + DIM + NSP;
-  Node<DIM, NSP> s;
-  load_q<DIM, NSP>(s, a.q, a.nnodes, gm);
-  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[gm] : 0.0);
-  double fm[NFLD];
-  viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fm);
-  if (fl & kBcSlip) {
-    // grad v_bc = G - (G n) n^T + n (n^T G n) n^T (boundary.py:159-173 in
-    // frame-free form); grad(rho v)_bc = rho_bc grad v_bc + v_bc grad rho-
-    const double rho_bc = a.wq[b];
-    double gv[DIM][DIM], gn[DIM], ngn = 0.0;
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-      gn[i] = 0.0;
-#pragma unroll
-      for (int j = 0; j < DIM; ++j) {
-        gv[i][j] = (gq[2 + i][j] - s.v[i] * gq[0][j]) * s.inv_rho;
-        gn[i] += gv[i][j] * n[j];
-      }
-      ngn += n[i] * gn[i];
-    }
-#pragma unroll
-    for (int i = 0; i < DIM; ++i) {
-      const double vbc = a.wq[(2 + i) * a.nbnd + b] / rho_bc;
-#pragma unroll
-      for (int j = 0; j < DIM; ++j)
-        gq[2 + i][j] = rho_bc * (gv[i][j] - gn[i] * n[j] + n[i] * ngn * n[j]) +
-                       vbc * gq[0][j];
-    }
-  }
-  if (fl & kBcImpermeable) {
-    // grad Y_bc = grad Y - (grad Y . n) n (boundary.py:196-213)
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) {
-      double gy[DIM], gyn = 0.0;
-#pragma unroll
-      for (int j = 0; j < DIM; ++j) {
-        gy[j] = (gq[2 + DIM + k][j] - s.y[k] * gq[0][j]) * s.inv_rho;
-        gyn += gy[j] * n[j];
-      }
-#pragma unroll
-      for (int j = 0; j < DIM; ++j)
-        gq[2 + DIM + k][j] = s.q[0] * (gy[j] - gyn * n[j]) + s.y[k] * gq[0][j];
-    }
-  }
-  if (fl & kBcAdiabatic) {
-    double tn = 0.0;
-#pragma unroll
-    for (int j = 0; j < DIM; ++j) tn += gt[j] * n[j];
-#pragma unroll
-    for (int j = 0; j < DIM; ++j) gt[j] -= tn * n[j];
-  }
-  load_q<DIM, NSP>(s, a.wq, a.nbnd, b);
-  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.wts[b] : 0.0);
-  viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
-#pragma unroll
-  for (int k = 0; k < NFLD; ++k) fv[k] = 2.0 * fv[k] - fm[k];
-}
-
-// }}}
-
-// {{{ divergence (Euler / Navier-Stokes RHS)
-//
-// Net flux F = F_I - F_V per node, by mode:
-//   kEuler  F_I(q) evaluated here
-//   kGrad   F_I(q) - F_V(q, grad q, grad T) from given gradients
-//   kFlux   read from the nodal-flux array written by grad_kernel<FLUX>
-// Face flux from the net normal fluxes Fn = F.n of both sides:
-//   Rusanov / central  (Fn- + Fn+ + lam (q- - q+)) / 2
-//   HLL                HLL(F_I- . n, F_I+ . n) - (F_V- . n + F_V+ . n) / 2
-// R is the type of the LDS buffers and of the contraction (double, or float
-// for the mixed-precision path, which only exists for kFlux).  With float,
-// each element's fluxes are stored relative to the element's first node
-// (the weak divergence of that constant cancels its lift), so that FP32
-// holds small variations; face fluxes are formed in FP64 and shifted alike.
-
-constexpr int kEuler = 0, kGrad = 1, kFlux = 2;
-
-template <int DIM, int P, int NSP, int MIX, int MODE, typename R>
-struct RhsCfg {
-  using E = Elem<DIM, P>;
-  static constexpr int NFLD = 2 + DIM + NSP;
-  static constexpr int EPB = 64 / NFLD;           // columns fit one wave
-  static constexpr int COLS = EPB * NFLD;
-  static constexpr int NODEF = NFLD * DIM;        // net flux
-  static constexpr int NODEP = odd_stride(NODEF);  // LDS node stride
-  static constexpr int FROW = odd_stride(COLS);    // LDS face-row stride
-  static constexpr int SP = odd_stride(EPB * E::NP);  // staging plane stride
-  static constexpr int SF = EPB * E::NP * NODEP;
-  static constexpr int SFACE = E::NFN * FROW;
-  static constexpr int SGEO = EPB * DIM * DIM;
-  static constexpr int SREF = sizeof(R) == 4 ? EPB * NODEF : 0;
-  static constexpr int SWS = EPB * E::NP;          // wavespeed |v| + c per node
-  static constexpr int STAGE = NFLD * SP;
-  static_assert(STAGE <= SF, "staging aliases the flux buffer");
-  static constexpr size_t LDS = sizeof(R) * (SF + SFACE) +
-                                sizeof(double) * (SGEO + SREF + SWS) +
-                                sizeof(int) * E::NFN;
-  static_assert(EPB >= 1, "too many fields");
-  static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
-};
-
-// HLL selects the HLL face flux (compiled separately: its two full face
-// states cost registers that the Rusanov / central path does not need).
-template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
-__global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
-                                                     const double* __restrict__ ws,
-                                                     double* __restrict__ out) {
-  using E = Elem<DIM, P>;
-  using O = Ops<DIM, P>;
-  using C = RhsCfg<DIM, P, NSP, MIX, MODE, R>;
-  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
-  constexpr int NFLD = C::NFLD, EPB = C::EPB, COLS = C::COLS;
-  constexpr int RPW = O::RPW, NODEF = C::NODEF, NODEP = C::NODEP;
-  constexpr int FROW = C::FROW, SP = C::SP;
-  constexpr bool VISC = MODE == kGrad;
-  constexpr bool MIXED = sizeof(R) == 4;
-  static_assert(!MIXED || MODE == kFlux, "mixed precision is the fused NS path");
-  extern __shared__ double smem[];
-  double* s_geo = smem;                                   // [EPB][DIM][DIM]
-  double* s_ref = s_geo + C::SGEO;                        // [EPB][NODEF] (mixed)
-  double* s_ws = s_ref + C::SREF;                         // [EPB][NP]
-  R* s_f = reinterpret_cast<R*>(s_ws + C::SWS);           // [EPB][NP][NFLD][DIM]
-  R* s_face = s_f + C::SF;                                // [NFN][FROW]
-  int* s_fmask = reinterpret_cast<int*>(s_face + C::SFACE);
-  const R* ops = packed_ops<R, DIM, P>(ws);
-
-  const int64_t N = a.K * NP;
-  const int64_t e0 = xcd_block() * EPB;
-  const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
-  const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
-  const bool inv = MODE == kEuler || a.inviscid_on;
-  const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
-  for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
-  for (int t = threadIdx.x; t < EPB * DIM * DIM; t += blockDim.x)
-    s_geo[t] = t < nel * DIM * DIM ? a.rx[e0 * DIM * DIM + t] : 0.0;
-  if constexpr (MIXED) {
-    for (int t = threadIdx.x; t < EPB * NODEF; t += blockDim.x) {
-      const int le = t / NODEF;
-      const int k = t - le * NODEF;
-      s_ref[t] = le < nel ? double(a.fnet32[k * N + (e0 + le) * NP]) : 0.0;
-    }
-  }
-
-  // phase 1: net flux at the volume nodes (coalesced; fixed trip count)
-#pragma unroll
-  for (int it = 0; it < (EPB * NP + kBlock - 1) / kBlock; ++it) {
-    const int t = int(threadIdx.x) + it * kBlock;
-    if (t >= EPB * NP) continue;
-    const int le = t / NP;
-    const int i = t - le * NP;
-    const int64_t e = e0 + le;
-    if (e >= a.K) continue;
-    const int64_t g = e * NP + i;
-    R* fdst = s_f + t * NODEP;
-    if constexpr (MODE == kFlux) {
-      Node<DIM, NSP> s;
-      load_q<DIM, NSP>(s, a.q, N, g);
-      node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[g] : 0.0);
-      s_ws[t] = s.speed + s.c;
-      if constexpr (MIXED) {
-        const int64_t g0 = e * NP;
-#pragma unroll
-        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet32[k * N + g] - a.fnet32[k * N + g0];
-      } else {
-#pragma unroll
-        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet[k * N + g];
-      }
-    } else {
-      Node<DIM, NSP> s;
-      load_q<DIM, NSP>(s, a.q, N, g);
-      node_state<DIM, NSP, MIX, VISC>(s, a.gas, MIX ? a.temp[g] : 0.0);
-      s_ws[t] = s.speed + s.c;
-      double F[NFLD][DIM];
-      if (inv) {
-        inviscid_flux<DIM, NSP>(s, F);
-      } else {
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) F[f][d] = 0.0;
-      }
-      if constexpr (VISC) {
-        double gq[NFLD][DIM], gt[DIM], FV[NFLD][DIM];
-        load_grad<DIM, NSP>(gq, gt, a.grad, N, g);
-        viscous_flux<DIM, NSP, MIX>(s, gq, gt, FV);
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) F[f][d] -= FV[f][d];
-      }
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) fdst[f * DIM + d] = F[f][d];
-    }
-  }
-  __syncthreads();
-
-  // phase 2: numerical face fluxes (FP64).  The own side and neighbours
-  // inside this workgroup come from LDS; other neighbours are gathered here.
-  constexpr bool hll = HLL;
-  // With many fields (the 12-field mixture) the Rusanov / central face flux
-  // is formed field by field, without per-field arrays that would hold ~70
-  // doubles per thread and cut occupancy to one wave per SIMD; with five
-  // fields the arrays are cheap and let all loads issue up front.
-  constexpr bool STREAM = !HLL && NFLD > 6;
-  // fixed trip count, unrolled: the gathers of a thread's face nodes can
-  // issue together
-#pragma unroll
-  for (int it = 0; it < (EPB * NFN + kBlock - 1) / kBlock; ++it) {
-    const int t = int(threadIdx.x) + it * kBlock;
-    if (t >= EPB * NFN) continue;
-    const int le = t / NFN;
-    const int fn = t - le * NFN;
-    const int64_t e = e0 + le;
-    if (e >= a.K) continue;
-    const int f = fn / NFP;
-    const int im = s_fmask[fn];
-    const int32_t nb = a.vmap_p[e * NFN + fn];
-    double n[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + f) * DIM + d];
-    const double fs = a.fscale[e * NF + f];
-    const int lm = le * NP + im;
-    if constexpr (!STREAM) {
-      double fnm[NFLD], fnp[NFLD], qm[NFLD], qp[NFLD], shift[NFLD];
-  #pragma unroll
-      for (int k = 0; k < NFLD; ++k) {
-        double c = 0.0, sh = 0.0;
-  #pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
-          c += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
-          sh += r0 * n[d];
-        }
-        fnm[k] = c;
-        shift[k] = sh;
-      }
-      const int64_t gm = e * NP + im;
-      const double tm = MIX ? a.temp[gm] : 0.0;
-      const double wsm = s_ws[lm];
-  #pragma unroll
-      for (int k = 0; k < NFLD; ++k) qm[k] = a.q[k * N + gm];
-      double tp, wsp;
-      if (nb >= g_lo && nb < g_hi) {
-        const int lp = int(nb - g_lo);
-        const int ple = lp / NP;
-        tp = MIX ? a.temp[nb] : 0.0;
-        wsp = s_ws[lp];
-  #pragma unroll
-        for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
-  #pragma unroll
-        for (int k = 0; k < NFLD; ++k) {
-          double c = 0.0;
-  #pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            c += (double(s_f[lp * NODEP + k * DIM + d]) +
-                  (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
-          fnp[k] = c;
-        }
-      } else {
-        const bool bnd = nb < 0;
-        const int64_t b = bnd ? int64_t(-nb - 1) : 0;
-        Node<DIM, NSP> sp;
-        if (!bnd) {
-          load_q<DIM, NSP>(sp, a.q, N, nb);
-          tp = MIX ? a.temp[nb] : 0.0;
-        } else {
-          load_q<DIM, NSP>(sp, a.qb, a.nbnd, b);
-          tp = MIX ? a.tb[b] : 0.0;
-        }
-        node_state<DIM, NSP, MIX, VISC>(sp, a.gas, tp);
-        wsp = sp.speed + sp.c;
-  #pragma unroll
-        for (int k = 0; k < NFLD; ++k) qp[k] = sp.q[k];
-        if constexpr (MODE == kFlux) {
-          if (!bnd) {
-  #pragma unroll
-            for (int k = 0; k < NFLD; ++k) {
-              double c = 0.0;
-  #pragma unroll
-              for (int d = 0; d < DIM; ++d)
-                c += (MIXED ? double(a.fnet32[(k * DIM + d) * N + nb])
-                            : a.fnet[(k * DIM + d) * N + nb]) * n[d];
-              fnp[k] = c;
-            }
-          } else {
-  #pragma unroll
-            for (int k = 0; k < NFLD; ++k) fnp[k] = a.fnb[k * a.nbnd + b];
-          }
-        } else {
-          if (inv) {
-            inviscid_flux_n<DIM, NSP>(sp, n, fnp);
-          } else {
-  #pragma unroll
-            for (int k = 0; k < NFLD; ++k) fnp[k] = 0.0;
-          }
-          if constexpr (VISC) {
-            double gq[NFLD][DIM], gt[DIM], fv[NFLD];
-            load_grad<DIM, NSP>(gq, gt, a.grad, N, bnd ? e * NP + im : int64_t(nb));
-            if (bnd && (wall_flags(a, b) & kBcMengaldo)) {
-              // precomputed by wall_flux_kernel
-  #pragma unroll
-              for (int k = 0; k < NFLD; ++k) fv[k] = a.wwork[k * a.nbnd + b];
-            } else {
-              viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
-            }
-  #pragma unroll
-            for (int k = 0; k < NFLD; ++k) fnp[k] -= fv[k];
-          }
-        }
-      }
-      double fl[NFLD];
-      if constexpr (hll) {
-        Node<DIM, NSP> sm, sp;
-  #pragma unroll
-        for (int k = 0; k < NFLD; ++k) {
-          sm.q[k] = qm[k];
-          sp.q[k] = qp[k];
-        }
-        node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
-        node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
-        double fim[NFLD], fip[NFLD];
-        inviscid_flux_n<DIM, NSP>(sm, n, fim);
-        inviscid_flux_n<DIM, NSP>(sp, n, fip);
-        inviscid_numflux<DIM, NSP>(sm, sp, n, 1, fl);
-        // stored net fluxes exclude p_ref I: F_V.n = F_I.n - (Fn + p_ref n)
-  #pragma unroll
-        for (int k = 0; k < NFLD; ++k) {
-          const double pn = (k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0;
-          fl[k] -= ((fim[k] - fnm[k] - pn) + (fip[k] - fnp[k] - pn)) / 2 + pn;
-        }
-      } else {
-        const double lam = (inv && a.flux_kind == 0) ? fmax(wsm, wsp) : 0.0;
-  #pragma unroll
-        for (int k = 0; k < NFLD; ++k) fl[k] = (fnm[k] + fnp[k] + lam * (qm[k] - qp[k])) / 2;
-      }
-  #pragma unroll
-      for (int k = 0; k < NFLD; ++k)
-        s_face[fn * FROW + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
-    } else {
-      // Rusanov / central, field by field
-      const int64_t gm = e * NP + im;
-      const double wsm = s_ws[lm];
-      const bool local = nb >= g_lo && nb < g_hi;
-      const bool bnd = nb < 0;
-      const int64_t b = bnd ? int64_t(-nb - 1) : 0;
-      const double* qsrc = bnd ? a.qb : a.q;
-      const int64_t qstr = bnd ? a.nbnd : N;
-      const int64_t qix = bnd ? b : int64_t(nb);
-      const int lp = local ? int(nb - g_lo) : 0;
-      const int ple = lp / NP;
-      double wsp;
-      Node<DIM, NSP> sp;
-      double fv[VISC ? NFLD : 1];
-      if (local) {
-        wsp = s_ws[lp];
-      } else {
-        load_q<DIM, NSP>(sp, qsrc, qstr, qix);
-        node_state<DIM, NSP, MIX, VISC>(sp, a.gas,
-                                        MIX ? (bnd ? a.tb[b] : a.temp[nb]) : 0.0);
-        wsp = sp.speed + sp.c;
-        if constexpr (VISC) {
-          double gq[NFLD][DIM], gt[DIM];
-          load_grad<DIM, NSP>(gq, gt, a.grad, N, bnd ? e * NP + im : int64_t(nb));
-          if (bnd && (wall_flags(a, b) & kBcMengaldo)) {
-            // precomputed by wall_flux_kernel
-#pragma unroll
-            for (int k = 0; k < NFLD; ++k) fv[k] = a.wwork[k * a.nbnd + b];
-          } else {
-            viscous_flux_n<DIM, NSP, MIX>(sp, gq, gt, n, fv);
-          }
-        }
-      }
-      double vnp = 0.0, mnp = 0.0;
-      if (!local) {
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          vnp += sp.v[d] * n[d];
-          mnp += sp.q[2 + d] * n[d];
-        }
-      }
-      const double lam = (inv && a.flux_kind == 0) ? fmax(wsm, wsp) : 0.0;
-#pragma unroll
-      for (int k = 0; k < NFLD; ++k) {
-        double fm = 0.0, sh = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
-          fm += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
-          sh += r0 * n[d];
-        }
-        const double qmk = a.q[k * N + gm];
-        const double qpk = local ? a.q[k * N + nb] : sp.q[k];
-        double fp = 0.0;
-        if (local) {
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            fp += (double(s_f[lp * NODEP + k * DIM + d]) +
-                   (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
-        } else if constexpr (MODE == kFlux) {
-          if (bnd) {
-            fp = a.fnb[k * a.nbnd + b];
-          } else {
-#pragma unroll
-            for (int d = 0; d < DIM; ++d)
-              fp += (MIXED ? double(a.fnet32[(k * DIM + d) * N + nb])
-                           : a.fnet[(k * DIM + d) * N + nb]) * n[d];
-          }
-        } else {
-          if (inv) {
-            // component k of inviscid_flux_n (mgdg_physics.h)
-            if (k == 0) fp = mnp;
-            else if (k == 1) fp = vnp * (sp.q[1] + sp.p);
-            else if (k < 2 + DIM) fp = sp.q[0] * sp.v[k - 2] * vnp + sp.p * n[k - 2];
-            else fp = vnp * sp.q[k];
-          }
-          if constexpr (VISC) fp -= fv[k];
-        }
-        const double flk = (fm + fp + lam * (qmk - qpk)) / 2;
-        s_face[fn * FROW + le * NFLD + k] = R(-(fs * (flk - sh)));
-      }
-    }
-  }
-  __syncthreads();
-
-  // phase 3: rhs[i] = sum_j sum_r Dw_r[i][j] G_r[j] + sum_m LIFT[i][m] B[m]
-  // per (element, field) column, G_r = sum_d rx[r][d] F_d; rows split over
-  // the waves, operator entries wave-uniform
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int c = threadIdx.x & 63;
-  const int cc = c < COLS ? c : 0;
-  const int cle = cc / NFLD;
-  const int cf = cc - cle * NFLD;
-  R rxl[DIM][DIM];
-#pragma unroll
-  for (int r = 0; r < DIM; ++r)
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) rxl[r][d] = R(s_geo[cle * DIM * DIM + r * DIM + d]);
-  const R* DV = ops + w * (NP * DIM * RPW);
-  const R* LF = ops + O::DVOL + w * (NFN * RPW);
-  R acc[RPW];
-#pragma unroll
-  for (int rr = 0; rr < RPW; ++rr) acc[rr] = R(0);
-#pragma unroll 2
-  for (int j = 0; j < NP; ++j) {
-    const R* fj = s_f + (cle * NP + j) * NODEP + cf * DIM;
-    R fd[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) fd[d] = fj[d];
-#pragma unroll
-    for (int r = 0; r < DIM; ++r) {
-      R gr = R(0);
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) gr += rxl[r][d] * fd[d];
-#pragma unroll
-      for (int rr = 0; rr < RPW; ++rr) acc[rr] += DV[(j * DIM + r) * RPW + rr] * gr;
-    }
-  }
-#pragma unroll 4
-  for (int m = 0; m < NFN; ++m) {
-    const R bv = s_face[m * FROW + cc];
-#pragma unroll
-    for (int rr = 0; rr < RPW; ++rr) acc[rr] += LF[m * RPW + rr] * bv;
-  }
-  __syncthreads();
-  // phase 4: stage in LDS (over s_f), write coalesced
-  R* stage = s_f;                            // [NFLD][SP]
-  if (c < COLS) {
-#pragma unroll
-    for (int rr = 0; rr < RPW; ++rr) {
-      const int i = w * RPW + rr;
-      if (i < NP) stage[cf * SP + cle * NP + i] = acc[rr];
-    }
-  }
-  __syncthreads();
-  write_staged<NP>(stage, NFLD, EPB, SP, e0, a.K, out);
-}
-
-// }}}
-
-// {{{ gradient (and the fused nodal net flux)
-//
-// grad_kernel computes the DG gradients of the conserved fields and of T
-// (central flux inside, exterior value on boundary faces).  With FLUX set it
-// then evaluates, at every node, the net flux F_I - F_V from the state and
-// the just-computed gradients (viscous stress, heat and species fluxes are
-// pointwise) and writes that instead of (or besides) the gradients; for
-// boundary face nodes it also writes (F_I - F_V)(q_ext, grad q-) . n, the
-// exterior side of the central viscous boundary flux.
-// With R = float (mixed precision) the traces are held relative to each
-// element's first node (the DG gradient of a constant is zero, and the
-// central face values are shifted by the same constant), the contraction
-// runs in FP32, the physics in FP64 and the nodal fluxes are stored in FP32
-// with p_ref removed from the momentum-flux diagonal.
-
-template <int DIM, int P, int NSP, int MIX, typename R>
-struct GradCfg {
-  using E = Elem<DIM, P>;
-  static constexpr int NS = 3 + DIM + NSP;  // conserved fields + T
-  static constexpr int EPB = 64 / NS;
-  static constexpr int COLS = 64;
-  static constexpr int ROW = odd_stride(COLS);         // LDS trace-row stride
-  static constexpr int SP = odd_stride(EPB * E::NP);   // gradient plane stride
-  static constexpr int KG = E::NP + E::NFN;
-  static constexpr int GEO = DIM * DIM + E::NF * DIM;  // rx and normals
-  static constexpr int STAGE = NS * DIM * SP;
-  static constexpr int SB = KG * ROW > STAGE ? KG * ROW : STAGE;
-  static constexpr int SREF = sizeof(R) == 4 ? EPB * NS : 0;
-  static constexpr size_t LDS = sizeof(R) * SB + sizeof(double) * (EPB * GEO + SREF) +
-                                sizeof(int) * E::NFN;
-  static_assert(EPB >= 1, "too many fields");
-  static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
-};
-
-template <int DIM, int NSP, int MIX>
-__device__ __forceinline__ double node_temperature(const double* qv,
-                                                   const GasDev& g, double tmix) {
-  if constexpr (MIX != 0) {
-    return tmix;
-  } else {
-    double mm = 0.0;
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) mm += qv[2 + d] * qv[2 + d];
-    const double irho = 1.0 / qv[0];
-    const double e_int = qv[1] - 0.5 * mm * irho;
-    return ((g.gamma - 1.0) / g.gas_const) * e_int * irho;
-  }
-}
-
-template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
-__global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
-                                                      const double* __restrict__ ws,
-                                                      double* __restrict__ out) {
-  using E = Elem<DIM, P>;
-  using O = Ops<DIM, P>;
-  using C = GradCfg<DIM, P, NSP, MIX, R>;
-  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
-  constexpr int NFLD = 2 + DIM + NSP, NS = C::NS, EPB = C::EPB, COLS = C::COLS;
-  constexpr int ROW = C::ROW, SP = C::SP;
-  constexpr int RPW = O::RPW, GEO = C::GEO;
-  constexpr bool MIXED = sizeof(R) == 4;
-  static_assert(!MIXED || FLUX, "mixed precision is the fused NS path");
-  extern __shared__ double smem[];
-  double* s_geo = smem;                          // [EPB][GEO]
-  double* s_ref = s_geo + EPB * GEO;             // [EPB][NS] (mixed)
-  R* s_b = reinterpret_cast<R*>(s_ref + C::SREF);  // [NP + NFN][ROW]
-  R* stage = s_b;                                // [NS][DIM][EPB*NP], after phase 3
-  int* s_fmask = reinterpret_cast<int*>(s_b + C::SB);
-  const R* ops = packed_ops<R, DIM, P>(ws);
-
-  const int64_t N = a.K * NP;
-  const int64_t e0 = xcd_block() * EPB;
-  const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
-  const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
-  for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
-  for (int t = threadIdx.x; t < EPB * GEO; t += blockDim.x) {
-    const int le = t / GEO;
-    const int u = t - le * GEO;
-    const int64_t e = e0 + le;
-    double v = 0.0;
-    if (e < a.K)
-      v = u < DIM * DIM ? a.rx[e * DIM * DIM + u] : a.normals[e * NF * DIM + u - DIM * DIM];
-    s_geo[t] = v;
-  }
-  if constexpr (MIXED) {
-    // per-element reference values: q and T at the element's first node
-    for (int t = threadIdx.x; t < EPB; t += blockDim.x) {
-      double qv[NFLD];
-      if (t < nel) {
-        const int64_t g0 = (e0 + t) * NP;
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) qv[f] = a.q[f * N + g0];
-        s_ref[t * NS + NFLD] =
-            node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g0] : 0.0);
-      } else {
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) qv[f] = 0.0;
-        s_ref[t * NS + NFLD] = 0.0;
-      }
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) s_ref[t * NS + f] = qv[f];
-    }
-    __syncthreads();
-  }
-
-  // phase 1: conserved fields and T at the volume nodes.  With one node per
-  // thread the node's state stays in registers for the flux epilogue.
-  constexpr bool KEEPQ = FLUX && EPB * NP <= kBlock;
-  double qkeep[KEEPQ ? NFLD : 1];
-  for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
-    const int le = t / NP;
-    const int i = t - le * NP;
-    const int64_t e = e0 + le;
-    if (e >= a.K) continue;
-    const int64_t g = e * NP + i;
-    double qv[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) qv[f] = a.q[f * N + g];
-    if constexpr (KEEPQ) {
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) qkeep[f] = qv[f];
-    }
-    const double tn = node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g] : 0.0);
-    const double* ref = s_ref + le * NS;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_b[i * ROW + le * NS + f] = R(qv[f] - (MIXED ? ref[f] : 0.0));
-    s_b[i * ROW + le * NS + NFLD] = R(tn - (MIXED ? ref[NFLD] : 0.0));
-  }
-  __syncthreads();
-
-  // phase 2: central (interior) / exterior-value (boundary) traces x fscale;
-  // neighbours inside this workgroup are read from LDS (fixed trip count,
-  // unrolled, so that a thread's gathers can issue together)
-#pragma unroll
-  for (int it = 0; it < (EPB * NFN + kBlock - 1) / kBlock; ++it) {
-    const int t = int(threadIdx.x) + it * kBlock;
-    if (t >= EPB * NFN) continue;
-    const int le = t / NFN;
-    const int fn = t - le * NFN;
-    const int64_t e = e0 + le;
-    if (e >= a.K) continue;
-    const int f = fn / NFP;
-    const int im = s_fmask[fn];
-    const int32_t nb = a.vmap_p[e * NFN + fn];
-    const double fs = a.fscale[e * NF + f];
-    R* dst = s_b + (NP + fn) * ROW + le * NS;
-    const R* own = s_b + im * ROW + le * NS;
-    const double* ref = s_ref + le * NS;
-    if (nb >= g_lo && nb < g_hi) {
-      const int lp = int(nb - g_lo);
-      const int ple = lp / NP;
-      const R* nbv = s_b + (lp - ple * NP) * ROW + ple * NS;
-      const double* pref = s_ref + ple * NS;
-#pragma unroll
-      for (int k = 0; k <= NFLD; ++k) {
-        const double dref = MIXED ? pref[k] - ref[k] : 0.0;
-        dst[k] = R(fs * ((double(nbv[k]) + dref + double(own[k])) / 2));
-      }
-    } else {
-      double qp[NFLD];
-      double tp;
-      const bool bnd = nb < 0;
-      if (!bnd) {
-#pragma unroll
-        for (int k = 0; k < NFLD; ++k) qp[k] = a.q[k * N + nb];
-        tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.temp[nb] : 0.0);
-      } else {
-        const int64_t b = -int64_t(nb) - 1;
-        if (wall_flags(a, b) & kBcMengaldo) {
-          // CV_bc n and T_bc n (boundary.py:713-793)
-#pragma unroll
-          for (int k = 0; k < NFLD; ++k) qp[k] = a.wq[k * a.nbnd + b];
-          tp = a.wt[b];
-        } else {
-#pragma unroll
-          for (int k = 0; k < NFLD; ++k) qp[k] = a.qb[k * a.nbnd + b];
-          tp = node_temperature<DIM, NSP, MIX>(qp, a.gas, MIX ? a.tb[b] : 0.0);
-        }
-      }
-#pragma unroll
-      for (int k = 0; k <= NFLD; ++k) {
-        const double up = (k < NFLD ? qp[k] : tp) - (MIXED ? ref[k] : 0.0);
-        dst[k] = R(bnd ? fs * up : fs * ((up + double(own[k])) / 2));
-      }
-    }
-  }
-  __syncthreads();
-
-  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int c = threadIdx.x & 63;
-  if constexpr (MIXED) {
-    // phase 3 on the matrix cores (v_mfma_f32_16x16x4f32): wave w owns the
-    // 16 columns [16w, 16w + 16); per 16-row tile of each output
-    // W_r = Dw_r u and L_f = LIFT_f (fs u*) the K dimension (nodes) runs in
-    // steps of 4.  Lane l supplies B[k = l / 16][col = l % 16] and holds
-    // D[4 (l / 16) + reg][l % 16] of each tile.
-    using O = Ops<DIM, P>;
-    constexpr int MT = O::MT, KSV = O::KSV, KSF = O::KSF;
-    const float* mf = mfma_ops<DIM, P>(ws);
-    const int col = w * 16 + (c & 15);
-    const int kq = c >> 4;
-    floatx4 aw[DIM][MT], al[NF][MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-#pragma unroll
-      for (int r = 0; r < DIM; ++r) aw[r][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-      for (int f = 0; f < NF; ++f) al[f][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-    }
-#pragma unroll
-    for (int st = 0; st < KSV; ++st) {
-      const int j = st * 4 + kq;
-      const float bv = float(s_b[(j < NP ? j : NP - 1) * ROW + col]);  // A is 0 past NP
-#pragma unroll
-      for (int r = 0; r < DIM; ++r)
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-          aw[r][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-              mf[((r * MT + mt) * KSV + st) * 64 + c], bv, aw[r][mt], 0, 0, 0);
-    }
-    // faces innermost: NF * MT independent accumulations per step
-#pragma unroll
-    for (int st = 0; st < KSF; ++st) {
-      const int m = st * 4 + kq;
-      const int mm = m < NFP ? m : NFP - 1;
-#pragma unroll
-      for (int f = 0; f < NF; ++f) {
-        const float bv = float(s_b[(NP + f * NFP + mm) * ROW + col]);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-          al[f][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-              mf[O::MFV + ((f * MT + mt) * KSF + st) * 64 + c], bv, al[f][mt], 0, 0, 0);
-      }
-    }
-    __syncthreads();
-    // phase 4: stage the gradients [NS][DIM][EPB*NP] (over the trace buffer)
-    if (col < EPB * NS) {
-      const int le = col / NS;
-      const int sc = col - le * NS;
-      const double* geo = s_geo + le * GEO;
-      // one direction at a time: only its 7 geometric factors are live
-#pragma unroll 1
-      for (int d = 0; d < DIM; ++d) {
-        float rxd[DIM], nfd[NF];
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) rxd[r] = float(geo[r * DIM + d]);
-#pragma unroll
-        for (int f = 0; f < NF; ++f) nfd[f] = float(geo[DIM * DIM + f * DIM + d]);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-          for (int reg = 0; reg < 4; ++reg) {
-            const int i = mt * 16 + 4 * kq + reg;
-            if (i >= NP) continue;
-            float vsum = 0.f, fsum = 0.f;
-#pragma unroll
-            for (int r = 0; r < DIM; ++r) vsum += rxd[r] * aw[r][mt][reg];
-#pragma unroll
-            for (int f = 0; f < NF; ++f) fsum += nfd[f] * al[f][mt][reg];
-            stage[(sc * DIM + d) * SP + le * NP + i] = R(fsum - vsum);
-          }
-      }
-    }
-  } else {
-    // phase 3: per (element, scalar) column: W_r = Dw_r u, L_f = LIFT_f (fs u*),
-    // grad_d = sum_f n_f[d] L_f - sum_r rx[r][d] W_r
-    const R* GV = ops + w * (NP * DIM * RPW);
-    const R* GL = ops + O::DVOL + w * (NFN * RPW);
-    R accw[RPW][DIM], accl[RPW][NF];
-  #pragma unroll
-    for (int rr = 0; rr < RPW; ++rr) {
-  #pragma unroll
-      for (int r = 0; r < DIM; ++r) accw[rr][r] = R(0);
-  #pragma unroll
-      for (int f = 0; f < NF; ++f) accl[rr][f] = R(0);
-    }
-  #pragma unroll 2
-    for (int j = 0; j < NP; ++j) {
-      const R bv = s_b[j * ROW + c];
-  #pragma unroll
-      for (int r = 0; r < DIM; ++r)
-  #pragma unroll
-        for (int rr = 0; rr < RPW; ++rr) accw[rr][r] += GV[(j * DIM + r) * RPW + rr] * bv;
-    }
-  #pragma unroll
-    for (int f = 0; f < NF; ++f) {
-  #pragma unroll 2
-      for (int m = 0; m < NFP; ++m) {
-        const R bv = s_b[(NP + f * NFP + m) * ROW + c];
-  #pragma unroll
-        for (int rr = 0; rr < RPW; ++rr) accl[rr][f] += GL[(f * NFP + m) * RPW + rr] * bv;
-      }
-    }
-    __syncthreads();
-    // phase 4: stage the gradients [NS][DIM][EPB*NP] (over the trace buffer)
-    if (c < EPB * NS) {
-      const int le = c / NS;
-      const int sc = c - le * NS;
-      const double* geo = s_geo + le * GEO;
-  #pragma unroll
-      for (int d = 0; d < DIM; ++d) {
-  #pragma unroll
-        for (int rr = 0; rr < RPW; ++rr) {
-          const int i = w * RPW + rr;
-          if (i >= NP) continue;
-          R vsum = R(0), fsum = R(0);
-  #pragma unroll
-          for (int r = 0; r < DIM; ++r) vsum += R(geo[r * DIM + d]) * accw[rr][r];
-  #pragma unroll
-          for (int f = 0; f < NF; ++f) fsum += R(geo[DIM * DIM + f * DIM + d]) * accl[rr][f];
-          stage[(sc * DIM + d) * SP + le * NP + i] = fsum - vsum;
-        }
-      }
-    }
-  }
-  __syncthreads();
-  if constexpr (!FLUX) {
-    write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, out);
-  } else {
-    if (a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, a.grad_out);
-    const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
-    // phase 5: net flux F_I - F_V at every node, written coalesced (fixed
-    // trip count)
-#pragma unroll
-    for (int it = 0; it < (EPB * NP + kBlock - 1) / kBlock; ++it) {
-      const int t = int(threadIdx.x) + it * kBlock;
-      if (t >= EPB * NP) continue;
-      const int le = t / NP;
-      const int i = t - le * NP;
-      const int64_t e = e0 + le;
-      if (e >= a.K) continue;
-      const int64_t g = e * NP + i;
-      Node<DIM, NSP> s;
-      if constexpr (KEEPQ) {
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) s.q[f] = qkeep[f];
-      } else {
-        load_q<DIM, NSP>(s, a.q, N, g);
-      }
-      node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
-      double gq[NFLD][DIM], gt[DIM], F[NFLD][DIM];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) gq[f][d] = double(stage[(f * DIM + d) * SP + t]);
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * SP + t]);
-      viscous_flux<DIM, NSP, MIX>(s, gq, gt, F);
-      double FI[NFLD][DIM];
-      if (a.inviscid_on) {
-        inviscid_flux<DIM, NSP>(s, FI);
-      } else {
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) FI[f][d] = 0.0;
-      }
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          const double v = FI[f][d] - F[f][d] - ((f == 2 + d) ? p_ref : 0.0);
-          if constexpr (MIXED) {
-            a.fnet32_out[(f * DIM + d) * N + g] = float(v);
-          } else {
-            out[(f * DIM + d) * N + g] = v;
-          }
-        }
-    }
-    // phase 6: exterior net normal flux at boundary face nodes: central
-    // viscous flux with the interior gradients (boundary.py:944-970), or the
-    // Mengaldo viscous flux F_V(bc) . n (boundary.py:630-710)
-    if (a.nbnd > 0) {
-      for (int t = threadIdx.x; t < EPB * NFN; t += blockDim.x) {
-        const int le = t / NFN;
-        const int fn = t - le * NFN;
-        const int64_t e = e0 + le;
-        if (e >= a.K) continue;
-        const int32_t nb = a.vmap_p[e * NFN + fn];
-        if (nb >= 0) continue;
-        const int64_t b = -int64_t(nb) - 1;
-        const int f = fn / NFP;
-        const int im = s_fmask[fn];
-        if (wall_flags(a, b) & kBcMengaldo) {
-          // the interior gradients go to wall_flux_kernel, which forms this
-          // node's exterior flux
-          const int ti = le * NP + im;
-#pragma unroll
-          for (int k = 0; k < NS * DIM; ++k)
-            a.wwork[k * a.nbnd + b] = double(stage[k * SP + ti]);
-          continue;
-        }
-        double n[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) n[d] = s_geo[le * GEO + DIM * DIM + f * DIM + d];
-        Node<DIM, NSP> s;
-        load_q<DIM, NSP>(s, a.qb, a.nbnd, b);
-        node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.tb[b] : 0.0);
-        double gq[NFLD][DIM], gt[DIM], fv[NFLD], fi[NFLD];
-        const int ti = le * NP + im;
-#pragma unroll
-        for (int k = 0; k < NFLD; ++k)
-#pragma unroll
-          for (int d = 0; d < DIM; ++d)
-            gq[k][d] = double(stage[(k * DIM + d) * SP + ti]);
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) gt[d] = double(stage[(NFLD * DIM + d) * SP + ti]);
-        if (a.inviscid_on) {
-          inviscid_flux_n<DIM, NSP>(s, n, fi);
-        } else {
-#pragma unroll
-          for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
-        }
-        viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
-#pragma unroll
-        for (int k = 0; k < NFLD; ++k)
-          a.fnb_out[k * a.nbnd + b] =
-              fi[k] - fv[k] - ((k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0);
-      }
-    }
-  }
-}
-
-// }}}
-
-// {{{ Euler RHS on the quadrature discretization (overintegration)
-//
-// A workgroup owns epb elements (host-chosen from the LDS budget).  LDS:
-// s_q [F][epb][NP] the state (then the result), s_g [F*epb][gs] the inviscid
-// flux in reference directions (rx . F) at the volume points (r-major rows
-// of DIM*nq), s_f [F*epb][fs] minus the numerical flux times the face
-// Jacobian ratio at the face points; the row strides gs, fs are 4 mod 32
-// doubles so that the matrix-core operand reads of 16 rows spread over the
-// LDS banks.  Lanes own quadrature points in phases 2-3; phase 4 is one
-// (NP x K) x (K x F*epb) product on the matrix cores.
-
-struct QuadArgs {
-  int nq, nfq, epb, niter;
-  int gs, fs;             // LDS row strides of s_g and s_f (doubles)
-  const double* temp;     // [N] nodal temperature (mixtures: Newton seeds)
-  const double* tbq;      // [nbq] exterior temperatures at boundary face points
-  // operators transposed so that consecutive lanes (consecutive points in
-  // phases 2-3, consecutive nodes in phase 4) read consecutive addresses
-  const double* interp;   // [NP][nq]
-  const double* wdiff;    // [DIM][nq][NP]
-  const double* finterp;  // [NF][NFP][nfq]
-  const double* lift;     // [NF*nfq][NP]
-  const double* qbq;      // [F][nbq] exterior states at boundary face points
-  int64_t nbq;
-};
-
-// Temperature of a mixture state at a quadrature point: the Newton
-// inversion of its internal energy from an interpolated seed (the
-// reference's project_fluid_state -> make_fluid_state, gas_model.py:494-588).
-template <int DIM, int NSP>
-__device__ __forceinline__ double point_temperature(const double* q, double seed,
-                                                    int niter) {
-  const double irho = 1.0 / q[0];
-  double mm = 0.0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) mm += q[2 + d] * q[2 + d];
-  const double e = (q[1] - 0.5 * mm * irho) * irho;
-  double y[MIX_NSP];
-#pragma unroll
-  for (int k = 0; k < MIX_NSP; ++k) y[k] = q[2 + DIM + k] * irho;
-  return mix_temperature(e, seed, y, niter);
-}
-
-// acc0 + acc1 += A[i][0:kn) B[0:kn)[j] for one 16x16 tile on the matrix
-// cores: a = &A[i][0] with k stride NP (global, read through the caches),
-// b = &B[0][j] with k contiguous (LDS); kk = lane / 16 mod 4.  Reads are
-// clamped into [0, kn) and B is zero beyond kn, so the k padding adds
-// nothing.  Two register sets of 4 * kQuadSet k each alternate, so that the
-// loads of one are in flight while the other feeds the matrix cores.
-constexpr int kQuadSet = 4;
-
-template <int NP>
-__device__ __forceinline__ void quad_mfma_load(const double* a, const double* b, int kn,
-                                               int k0, int kk, double (&av)[kQuadSet],
-                                               double (&bv)[kQuadSet]) {
-#pragma unroll
-  for (int u = 0; u < kQuadSet; ++u) {
-    const int k = k0 + 4 * u + kk;
-    const bool ok = k < kn;
-    av[u] = a[(ok ? k : 0) * NP];
-    const double y = b[ok ? k : 0];
-    bv[u] = ok ? y : 0.0;
-  }
-}
-
-__device__ __forceinline__ void quad_mfma_step(const double (&av)[kQuadSet],
-                                               const double (&bv)[kQuadSet],
-                                               doublex4& acc0, doublex4& acc1) {
-#pragma unroll
-  for (int u = 0; u < kQuadSet; u += 2) {
-    acc0 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u], acc0, 0, 0, 0);
-    acc1 = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u + 1], bv[u + 1], acc1, 0, 0, 0);
-  }
-}
-
-template <int NP>
-__device__ __forceinline__ void quad_mfma_panel(const double* a, const double* b, int kn,
-                                                int kk, doublex4& acc0, doublex4& acc1) {
-  constexpr int KS = 4 * kQuadSet;
-  double ax[kQuadSet], bx[kQuadSet], ay[kQuadSet], by[kQuadSet];
-  quad_mfma_load<NP>(a, b, kn, 0, kk, ax, bx);
-  for (int k0 = 0; k0 < kn; k0 += 2 * KS) {
-    quad_mfma_load<NP>(a, b, kn, k0 + KS, kk, ay, by);
-    quad_mfma_step(ax, bx, acc0, acc1);
-    quad_mfma_load<NP>(a, b, kn, k0 + 2 * KS, kk, ax, bx);
-    quad_mfma_step(ay, by, acc0, acc1);
-  }
-}
-
-template <int DIM, int P, int NSP, int MIX>
-__global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa) {
-  using E = Elem<DIM, P>;
-  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
-  constexpr int NFLD = 2 + DIM + NSP;
-  extern __shared__ double lds[];
-  const int nq = qa.nq, nfq = qa.nfq, epb = qa.epb, nfq_all = NF * qa.nfq;
-  const int gs = qa.gs, fs = qa.fs;
-  double* s_q = lds;
-  double* s_g = s_q + NFLD * epb * NP;
-  double* s_f = s_g + NFLD * epb * gs;
-  double* s_p = s_f + NFLD * epb * fs;        // [F][epb][NF*NFP] exterior
-                                              // face-node values
-  double* s_t = s_p + NFLD * epb * NF * NFP;  // [epb][NP] temperature (MIX)
-  double* s_tp = s_t + epb * NP;              // [epb][NF*NFP] exterior (MIX)
-  const int64_t e0 = int64_t(blockIdx.x) * epb;
-  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
-  const int64_t N = a.nnodes;
-  // 1: the elements' state (each field's rows are contiguous in HBM) and
-  // the neighbours' values at this element's face nodes (interior faces)
-  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
-    const int f = t / (ne * NP);
-    const int r = t - f * ne * NP;
-    s_q[f * epb * NP + r] = a.q[f * N + e0 * NP + r];
-  }
-  for (int t = threadIdx.x; t < ne * NF * NFP; t += blockDim.x) {
-    const int le = t / (NF * NFP);
-    const int fn = t - le * NF * NFP;
-    const int32_t nb = a.vmap_p[(e0 + le) * NF * NFP + fn];
-    if (nb < 0) continue;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_p[(f * epb + le) * NF * NFP + fn] = a.q[f * N + nb];
-    if constexpr (MIX != 0) s_tp[le * NF * NFP + fn] = qa.temp[nb];
-  }
-  if constexpr (MIX != 0) {
-    for (int t = threadIdx.x; t < ne * NP; t += blockDim.x)
-      s_t[t] = qa.temp[e0 * NP + t];
-  }
-  __syncthreads();
-  // 2: inviscid flux at the volume points, in reference directions
-  for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
-    const int le = t / nq;
-    const int m = t - le * nq;
-    const double* w = qa.interp + m;
-    Node<DIM, NSP> s;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s.q[f] = 0.0;
-    double tseed = 0.0;
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const double wi = w[i * nq];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) s.q[f] += wi * s_q[(f * epb + le) * NP + i];
-      if constexpr (MIX != 0) tseed += wi * s_t[le * NP + i];
-    }
-    double tq = 0.0;
-    if constexpr (MIX != 0) tq = point_temperature<DIM, NSP>(s.q, tseed, qa.niter);
-    node_state<DIM, NSP, MIX, false>(s, a.gas, tq);
-    double F[NFLD][DIM];
-    inviscid_flux<DIM, NSP>(s, F);
-    const double* rx = a.rx + (e0 + le) * DIM * DIM;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-      for (int r = 0; r < DIM; ++r) {
-        double g = 0.0;
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) g += rx[r * DIM + d] * F[f][d];
-        s_g[(f * epb + le) * gs + r * nq + m] = g;
-      }
-  }
-  // 3: numerical flux at the face points (both traces interpolated from the
-  // face nodes in this element's face-node order)
-  for (int t = threadIdx.x; t < ne * nfq_all; t += blockDim.x) {
-    const int le = t / nfq_all;
-    const int rem = t - le * nfq_all;
-    const int fc = rem / nfq;
-    const int m = rem - fc * nfq;
-    const int64_t e = e0 + le;
-    const double* w = qa.finterp + fc * NFP * nfq + m;
-    const int32_t* vp = a.vmap_p + (e * NF + fc) * NFP;
-    const int* fm = a.fmask + fc * NFP;
-    Node<DIM, NSP> sm, sp;
-    double seed_m = 0.0, seed_p = 0.0, tm = 0.0, tp = 0.0;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) sm.q[f] = 0.0;
-#pragma unroll
-    for (int j = 0; j < NFP; ++j) {
-      const double wj = w[j * nfq];
-      const int ij = fm[j];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) sm.q[f] += wj * s_q[(f * epb + le) * NP + ij];
-      if constexpr (MIX != 0) seed_m += wj * s_t[le * NP + ij];
-    }
-    const int32_t v0 = vp[0];
-    if (v0 >= 0) {
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) sp.q[f] = 0.0;
-#pragma unroll
-      for (int j = 0; j < NFP; ++j) {
-        const double wj = w[j * nfq];
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f)
-          sp.q[f] += wj * s_p[(f * epb + le) * NF * NFP + fc * NFP + j];
-        if constexpr (MIX != 0) seed_p += wj * s_tp[le * NF * NFP + fc * NFP + j];
-      }
-      if constexpr (MIX != 0) tp = point_temperature<DIM, NSP>(sp.q, seed_p, qa.niter);
-    } else {
-      const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + m;
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) sp.q[f] = qa.qbq[f * qa.nbq + bq];
-      if constexpr (MIX != 0) tp = qa.tbq[bq];
-    }
-    if constexpr (MIX != 0) tm = point_temperature<DIM, NSP>(sm.q, seed_m, qa.niter);
-    node_state<DIM, NSP, MIX, false>(sm, a.gas, tm);
-    node_state<DIM, NSP, MIX, false>(sp, a.gas, tp);
-    double n[DIM];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
-    double fstar[NFLD];
-    inviscid_numflux<DIM, NSP>(sm, sp, n, a.flux_kind, fstar);
-    const double sc = a.fscale[e * NF + fc];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_f[(f * epb + le) * fs + rem] = -sc * fstar[f];
-  }
-  __syncthreads();
-  // 4: dQ/dt = [Wq_0 .. Wq_DIM-1 Lq] [G; -fscale F*]: rows are the NP nodes,
-  // columns j = f * epb + le the (field, element) pairs, the contraction
-  // runs over the DIM*nq volume points, then the NF*nfq face points.  Each
-  // wave forms 16x16 output tiles with v_mfma_f64_16x16x4f64 (A[i][k] in
-  // lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16 (i % 4),
-  // register i / 4).  Every read stays in bounds: points past the rule enter
-  // with B = 0; rows past NP (read from row 0) and columns of absent
-  // elements are formed but not stored.
-  {
-    constexpr int RT = (NP + 15) / 16;
-    const int ncol = NFLD * epb;
-    const int ntile = RT * ((ncol + 15) / 16);
-    const int li = threadIdx.x & 15, kk = (threadIdx.x >> 4) & 3;
-    for (int tile = threadIdx.x >> 6; tile < ntile; tile += blockDim.x >> 6) {
-      const int rt = tile % RT, ct = tile / RT;
-      const int row = rt * 16 + li;
-      const int j = ct * 16 + li;
-      const int lej = j % epb;
-      const bool rok = row < NP, cok = j < ncol && lej < ne;
-      doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
-      const int jc = j < ncol ? j : 0;
-      const int r0 = rok ? row : 0;
-      // volume points: k = r * nq + m; face points: k = face * nfq + m
-      quad_mfma_panel<NP>(qa.wdiff + r0, s_g + jc * gs, DIM * nq, kk, acc0, acc1);
-      quad_mfma_panel<NP>(qa.lift + r0, s_f + jc * fs, nfq_all, kk, acc0, acc1);
-      if (cok) {
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-          const int i = rt * 16 + 4 * g + kk;
-          if (i < NP) s_q[j * NP + i] = acc0[g] + acc1[g];
-        }
-      }
-    }
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
-    const int f = t / (ne * NP);
-    const int r = t - f * ne * NP;
-    a.out[f * N + e0 * NP + r] = s_q[f * epb * NP + r];
-  }
-}
-
-// }}}
-
-// {{{ entropy-stable Euler RHS (flux differencing, ideal gas)
-//
-// esdg_project_kernel: projected entropy variables v~ = P_q v(V_q u) per
-// element (grudge's volume_quadrature_project), written to ev.
-// esdg_kernel: the state at the nh hybridized points from v~ (own element)
-// and the face fluxes from both sides' v~; each lane then owns one
-// hybridized point h of one element and sums over the points j its row of
-// (Q_h - Q_h^T) o F_S (volume rows: all nh points, face rows: the nq volume
-// points -- the face-face block vanishes); the sums are mapped back with
-// M^-1 V_h^T.  Geometric factors enter as rx and n * fscale (the Jacobian
-// cancels against the mass matrix on affine elements).
-
-struct EsdgArgs {
-  int nq, nfq, nh, epb;
-  const double* interp;   // [NP][nq]
-  const double* proj;     // [nq][NP]  P_q transposed
-  const double* vh;       // [NP][nh]  V_h transposed
-  const double* skew;     // [DIM][nq(j)][nq(h)]: (Q_r - Q_r^T)[h][j]
-  const double* ext;      // [NF*nfq][nq]  E
-  const double* ext_col;  // [nq][NF*nfq]  E transposed
-  const double* apply;    // [nh][NP]  (M^-1 V_h^T) transposed
-  const double* finterp;  // [NF][NFP][nfq]
-  const double* wf;       // [nfq]
-  const double* qbq;      // [F][nbq]
-  int64_t nbq;
-  double* ev;             // [F][N] projected entropy variables
-  int dissipation;
-  // nq = 64, NF*nfq = 192, 256 threads: the paired phase 4 (each two-point
-  // flux once, for both rows) with the operators along cyclic diagonals
-  int paired;
-  const double* skew_diag;  // [DIM][nq(s)][nq(h)]: (Q_r - Q_r^T)[h][(h+s) % nq]
-  const double* ext_diag;   // [nq(s)][NF*nfq(m)]: E[m][(m+s) % nq]
-};
-
-template <int DIM, int P, int NSP>
-__global__ __launch_bounds__(kBlock) void esdg_project_kernel(KArgs a, EsdgArgs ea) {
-  using E = Elem<DIM, P>;
-  constexpr int NP = E::NP;
-  constexpr int NFLD = 2 + DIM + NSP;
-  extern __shared__ double lds[];
-  const int nq = ea.nq, epb = ea.epb;
-  double* s_e = lds;  // [F][epb][nq]
-  const int64_t e0 = int64_t(blockIdx.x) * epb;
-  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
-  const int64_t N = a.nnodes;
-  const double gamma = a.gas.gamma;
-  for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
-    const int le = t / nq;
-    const int m = t - le * nq;
-    const int64_t base = (e0 + le) * NP;
-    double u[NFLD], v[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) u[f] = 0.0;
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const double w = ea.interp[i * nq + m];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) u[f] += w * a.q[f * N + base + i];
-    }
-    cons_to_entropy<DIM, NSP>(u, gamma, v);
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_e[(f * epb + le) * nq + m] = v[f];
-  }
-  __syncthreads();
-  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
-    const int le = t / NP;
-    const int i = t - le * NP;
-    double v[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
-    for (int m = 0; m < nq; ++m) {
-      const double w = ea.proj[m * NP + i];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) v[f] += w * s_e[(f * epb + le) * nq + m];
-    }
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) ea.ev[f * N + e0 * NP + t] = v[f];
-  }
-}
-
-// Phase 4 of esdg_kernel when nq = 64 and NF*nfq = 192 (nh = 256, four
-// waves): each two-point flux is evaluated once.  (Q_h - Q_h^T) is skew and
-// F_S symmetric, so the pair (h, j) adds S[h][j] F_S to row h and the
-// negative to row j.  Wave 0 owns the 64 volume rows and meets the pairs
-// (h, (h+s) mod 64), s = 1..31, in step s; the contributions to the partner
-// rows travel in a register vector that moves one lane down per step, so
-// that lane h always holds row (h+s) mod 64 (s = 32 pairs each lane with the
-// lane that owns the same pair: both add their own row).  Wave 1 + b owns
-// face rows 64b..64b+63 and meets the volume points (l+s) mod 64 in step s,
-// s = 0..63 (the face-face block is zero); its moving vector collects that
-// block's part of the volume rows and is back in place after 64 steps.
-// Results: volume rows in the four nq-long pieces of s_acc (piece 0 from
-// wave 0, piece 1 + b from wave 1 + b; phase 5 adds them), face rows plus
-// the face flux in the face slots.  One element per workgroup: thread
-// 64 + m has computed face point m's flux (fkeep) in phase 3.
-template <int DIM, int NSP>
-__device__ __forceinline__ void esdg_pairs_64(const KArgs& a, const EsdgArgs& ea,
-                                              const EcPair<DIM, NSP>* s_prim,
-                                              const double (&fkeep)[2 + DIM + NSP],
-                                              double* s_acc, int64_t e, int nacc,
-                                              double gamma) {
-  constexpr int NF = DIM + 1;
-  constexpr int NFLD = 2 + DIM + NSP;
-  using Prim = EcPair<DIM, NSP>;
-  constexpr int NQ = 64, NFT = 192;
-  const int nfq = ea.nfq;
-  const int l = threadIdx.x & 63;
-  const int wave = threadIdx.x >> 6;
-  const int down = (l + 1) & 63;
-  {
-    const int le = 0, epb = 1;
-    const Prim* pj = s_prim;
-    double acc[NFLD], mv[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) acc[f] = mv[f] = 0.0;
-    if (wave == 0) {
-      const Prim ph = pj[l];
-      double rx[DIM * DIM];
-#pragma unroll
-      for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
-      for (int s = 1; s <= 32; ++s) {
-        double sd[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) sd[d] = 0.0;
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) {
-          const double sr = ea.skew_diag[(r * NQ + s) * NQ + l];
-#pragma unroll
-          for (int d = 0; d < DIM; ++d) sd[d] += rx[r * DIM + d] * sr;
-        }
-        double fv[NFLD];
-        ec_flux_dot<DIM, NSP, true>(ph, pj[(l + s) & 63], sd, gamma, fv);
-        if (s < 32) {
-#pragma unroll
-          for (int f = 0; f < NFLD; ++f) {
-            acc[f] += fv[f];
-            mv[f] = __shfl(mv[f] - fv[f], down, 64);
-          }
-        } else {
-#pragma unroll
-          for (int f = 0; f < NFLD; ++f) acc[f] += fv[f];
-        }
-      }
-      // lane l holds row (l + 32) mod 64
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) {
-        acc[f] += __shfl(mv[f], (l + 32) & 63, 64);
-        s_acc[(f * epb + le) * nacc + l] = acc[f];
-      }
-    } else {
-      const int m = (wave - 1) * NQ + l;
-      const Prim pm = pj[NQ + m];
-      const int fc = m / nfq;
-      const double bsc = a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
-      double nrm[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) nrm[d] = bsc * a.normals[(e * NF + fc) * DIM + d];
-      for (int s = 0; s < NQ; ++s) {
-        const double b = ea.ext_diag[s * NFT + m];
-        double sd[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) sd[d] = b * nrm[d];
-        double fv[NFLD];
-        ec_flux_dot<DIM, NSP, true>(pm, pj[(l + s) & 63], sd, gamma, fv);
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) {
-          acc[f] -= fv[f];
-          mv[f] = __shfl(mv[f] + fv[f], down, 64);
-        }
-      }
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) {
-        double* row = s_acc + (f * epb + le) * nacc;
-        row[wave * NQ + l] = mv[f];
-        row[4 * NQ + m] = acc[f] + fkeep[f];
-      }
-    }
-  }
-}
-
-// Entropy-stable face flux times fscale * w at face point m of element
-// e0 + le (phase 3 of esdg_kernel): the own trace state from s_qf, the
-// neighbour's from its projected entropy variables at the shared face nodes
-// (s_vp), on boundary faces the re-projected interior state and the
-// boundary condition's exterior state.
-template <int DIM, int P, int NSP>
-__device__ __forceinline__ void esdg_face_flux(const KArgs& a, const EsdgArgs& ea,
-                                               const double* s_qf, const double* s_vp,
-                                               const double* s_q, int64_t e0, int le, int m,
-                                               double gamma, double* out) {
-  using E = Elem<DIM, P>;
-  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
-  constexpr int NFLD = 2 + DIM + NSP;
-  using Prim = EcPrim<DIM, NSP>;
-  const int nfq = ea.nfq, epb = ea.epb, nft = NF * ea.nfq;
-  const int fc = m / nfq;
-  const int mm = m - fc * nfq;
-  const int64_t e = e0 + le;
-  const double* w = ea.finterp + fc * NFP * nfq + mm;
-  const int32_t v0 = a.vmap_p[(e * NF + fc) * NFP];
-  double qm[NFLD], qp[NFLD];
-  if (v0 >= 0) {
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) qm[f] = s_qf[(f * epb + le) * nft + m];
-    double vp[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) vp[f] = 0.0;
-#pragma unroll
-    for (int j = 0; j < NFP; ++j) {
-      const double wj = w[j * nfq];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f)
-        vp[f] += wj * s_vp[(f * epb + le) * NF * NFP + fc * NFP + j];
-    }
-    entropy_to_cons<DIM, NSP>(vp, gamma, qp);
-  } else {
-    // boundary: the interpolated interior state (through the entropy
-    // variables and back, as the reference's projection does) and the
-    // boundary condition's exterior state
-    double ui[NFLD], vi[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) ui[f] = 0.0;
-#pragma unroll
-    for (int j = 0; j < NFP; ++j) {
-      const double wj = w[j * nfq];
-      const int ij = a.fmask[fc * NFP + j];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) ui[f] += wj * s_q[(f * epb + le) * NP + ij];
-    }
-    cons_to_entropy<DIM, NSP>(ui, gamma, vi);
-    entropy_to_cons<DIM, NSP>(vi, gamma, qm);
-    const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + mm;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) qp[f] = ea.qbq[f * ea.nbq + bq];
-  }
-  Prim pm, pp;
-  ec_prim<DIM, NSP>(qm, gamma, pm);
-  ec_prim<DIM, NSP>(qp, gamma, pp);
-  double n[DIM], fs[NFLD];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
-#pragma unroll
-  for (int f = 0; f < NFLD; ++f) fs[f] = 0.0;
-  ec_flux_dot<DIM, NSP>(pm, pp, n, gamma, fs);
-  if (ea.dissipation) {
-    const double alpha = fmax(pm.c, pp.c);
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) fs[f] -= 0.5 * alpha * (qp[f] - qm[f]);
-  }
-  const double sc = a.fscale[e * NF + fc] * ea.wf[mm];
-#pragma unroll
-  for (int f = 0; f < NFLD; ++f) out[f] = sc * fs[f];
-}
-
-template <int DIM, int P, int NSP>
-__global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
-  using E = Elem<DIM, P>;
-  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
-  constexpr int NFLD = 2 + DIM + NSP;
-  using Pair = EcPair<DIM, NSP>;
-  extern __shared__ double lds[];
-  const int nq = ea.nq, nfq = ea.nfq, nh = ea.nh, epb = ea.epb, nft = NF * ea.nfq;
-  Pair* s_prim = reinterpret_cast<Pair*>(lds);             // [epb][nh]
-  double* s_v = reinterpret_cast<double*>(s_prim + epb * nh);  // [F][epb][NP]
-  double* s_q = s_v + NFLD * epb * NP;                      // [F][epb][NP]
-  double* s_vp = s_q + NFLD * epb * NP;                     // [F][epb][NF*NFP]
-  // phase 4 splits volume rows into C pieces of nq entries
-  const int C = (nh + nq - 1) / nq;
-  const int nacc = C * nq + nft;                            // row sums per field
-  // face fluxes [F][epb][nft] (not in the paired layout: kept in registers)
-  double* s_fs = s_vp + NFLD * epb * NF * NFP;
-  double* s_qf = ea.paired ? s_fs : s_fs + NFLD * epb * nft;  // [F][epb][nft], 2-3
-  double* s_acc = s_qf;                                     // [F][epb][nacc], phases 4-5
-  const int64_t e0 = int64_t(blockIdx.x) * epb;
-  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
-  const int64_t N = a.nnodes;
-  const double gamma = a.gas.gamma;
-  // 1: own v~ and q, the neighbours' v~ at this element's face nodes
-  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
-    const int f = t / (ne * NP);
-    const int r = t - f * ne * NP;
-    s_v[f * epb * NP + r] = ea.ev[f * N + e0 * NP + r];
-    s_q[f * epb * NP + r] = a.q[f * N + e0 * NP + r];
-  }
-  for (int t = threadIdx.x; t < ne * NF * NFP; t += blockDim.x) {
-    const int le = t / (NF * NFP);
-    const int fn = t - le * NF * NFP;
-    const int32_t nb = a.vmap_p[(e0 + le) * NF * NFP + fn];
-    if (nb < 0) continue;
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f)
-      s_vp[(f * epb + le) * NF * NFP + fn] = ea.ev[f * N + nb];
-  }
-  __syncthreads();
-  // 2: the state at the hybridized points
-  for (int t = threadIdx.x; t < ne * nh; t += blockDim.x) {
-    const int le = t / nh;
-    const int h = t - le * nh;
-    double v[NFLD], u[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
-#pragma unroll
-    for (int i = 0; i < NP; ++i) {
-      const double w = ea.vh[i * nh + h];
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) v[f] += w * s_v[(f * epb + le) * NP + i];
-    }
-    entropy_to_cons<DIM, NSP>(v, gamma, u);
-    EcPrim<DIM, NSP> pr;
-    ec_prim<DIM, NSP>(u, gamma, pr);
-    s_prim[le * nh + h] = pr;
-    if (h >= nq) {
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) s_qf[(f * epb + le) * nft + (h - nq)] = u[f];
-    }
-  }
-  __syncthreads();
-  // 3: entropy-stable face flux at the face points, times fscale * w
-  double fkeep[NFLD];
-#pragma unroll
-  for (int f = 0; f < NFLD; ++f) fkeep[f] = 0.0;
-  if (ea.paired) {
-    // one element per workgroup: thread 64 + m keeps face point m's flux
-    // for its face row in phase 4
-    const int m = int(threadIdx.x) - 64;
-    if (m >= 0 && m < nft && ne == 1)
-      esdg_face_flux<DIM, P, NSP>(a, ea, s_qf, s_vp, s_q, e0, 0, m, gamma, fkeep);
-  } else {
-    for (int t = threadIdx.x; t < ne * nft; t += blockDim.x) {
-      const int le = t / nft;
-      const int m = t - le * nft;
-      double fs[NFLD];
-      esdg_face_flux<DIM, P, NSP>(a, ea, s_qf, s_vp, s_q, e0, le, m, gamma, fs);
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) s_fs[(f * epb + le) * nft + m] = fs[f];
-    }
-  }
-  __syncthreads();
-  // 4: rows of (Q_h - Q_h^T) o F_S, plus the face flux on face rows.  A
-  // volume row has nh entries, a face row nq (the face-face block is zero):
-  // volume rows are cut into C = ceil(nh / nq) pieces of nq entries so that
-  // every work item sums nq pairs; the pieces are added in phase 5.
-  if (ea.paired) {
-    esdg_pairs_64<DIM, NSP>(a, ea, s_prim, fkeep, s_acc, e0, nacc, gamma);
-  } else {
-  const int nitems = nq * C + nft;
-  for (int t = threadIdx.x; t < ne * nitems; t += blockDim.x) {
-    const int le = t / nitems;
-    const int it = t - le * nitems;
-    int h, c, j0, j1;
-    if (it < nq * C) {
-      c = it / nq;
-      h = it - c * nq;
-      j0 = c * nq;
-      j1 = j0 + nq < nh ? j0 + nq : nh;
-    } else {
-      c = 0;
-      h = nq + (it - nq * C);
-      j0 = 0;
-      j1 = nq;
-    }
-    const int64_t e = e0 + le;
-    const Pair ph = s_prim[le * nh + h];
-    const Pair* pj = s_prim + le * nh;
-    double acc[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
-    if (h < nq) {
-      double rx[DIM * DIM];
-#pragma unroll
-      for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
-      const int jv = j1 < nq ? j1 : nq;
-      for (int j = j0; j < jv; ++j) {
-        double sr[DIM], sd[DIM];
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) sr[r] = ea.skew[(r * nq + j) * nq + h];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          double v = 0.0;
-#pragma unroll
-          for (int r = 0; r < DIM; ++r) v += rx[r * DIM + d] * sr[r];
-          sd[d] = v;
-        }
-        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
-      }
-      for (int j = (j0 > nq ? j0 : nq); j < j1; ++j) {
-        const int m = j - nq;
-        const int fc = m / nfq;
-        const double b = ea.ext[m * nq + h] * a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
-        double sd[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) sd[d] = b * a.normals[(e * NF + fc) * DIM + d];
-        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
-      }
-    } else {
-      const int m = h - nq;
-      const int fc = m / nfq;
-      const double bsc = -a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
-      double nrm[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) nrm[d] = bsc * a.normals[(e * NF + fc) * DIM + d];
-      for (int j = 0; j < nq; ++j) {
-        const double ej = ea.ext_col[j * nft + m];
-        double sd[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) sd[d] = ej * nrm[d];
-        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
-      }
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) acc[f] += s_fs[(f * epb + le) * nft + m];
-    }
-    const int slot = h < nq ? c * nq + h : C * nq + (h - nq);
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_acc[(f * epb + le) * nacc + slot] = acc[f];
-  }
-  }
-  __syncthreads();
-  // 5: dQ/dt = -M^-1 V_h^T acc: the volume rows' pieces summed in place,
-  // then one (field, element, node) output per thread
-  if (C > 1) {
-    for (int t = threadIdx.x; t < NFLD * ne * nq; t += blockDim.x) {
-      const int fl = t / nq;
-      const int h = t - fl * nq;
-      const int f = fl / ne, le = fl - f * ne;
-      double* sa = s_acc + (f * epb + le) * nacc + h;
-      double v = sa[0];
-      for (int cc = 1; cc < C; ++cc) v += sa[cc * nq];
-      sa[0] = v;
-    }
-    __syncthreads();
-  }
-  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
-    const int fl = t / NP;
-    const int i = t - fl * NP;
-    const int f = fl / ne, le = fl - f * ne;
-    const double* sa = s_acc + (f * epb + le) * nacc;
-    const double* w = ea.apply + i;
-    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
-    int h = 0;
-    for (; h + 4 <= nq; h += 4) {
-      o0 += w[h * NP] * sa[h];
-      o1 += w[(h + 1) * NP] * sa[h + 1];
-      o2 += w[(h + 2) * NP] * sa[h + 2];
-      o3 += w[(h + 3) * NP] * sa[h + 3];
-    }
-    for (; h < nq; ++h) o0 += w[h * NP] * sa[h];
-    const double* sf = sa + C * nq;
-    const double* wf = w + nq * NP;
-    int m = 0;
-    for (; m + 4 <= nft; m += 4) {
-      o0 += wf[m * NP] * sf[m];
-      o1 += wf[(m + 1) * NP] * sf[m + 1];
-      o2 += wf[(m + 2) * NP] * sf[m + 2];
-      o3 += wf[(m + 3) * NP] * sf[m + 3];
-    }
-    for (; m < nft; ++m) o0 += wf[m * NP] * sf[m];
-    a.out[f * N + (e0 + le) * NP + i] = -((o0 + o1) + (o2 + o3));
-  }
-}
-
-// }}}
-
-// {{{ nodal net flux from stored gradients
-//
-// The pointwise phases (5 and 6) of grad_kernel<FLUX> as node-parallel
-// passes over gradients that grad_kernel<FLUX=false> stored.  Used for the
-// 12-field mixture: its flux kernel holds only 4 elements (40 nodes at p = 2)
-// per workgroup, so the pointwise phase, which carries the NASA7 and
-// transport work, kept 40 of 256 lanes busy.
-
-template <int DIM, int NSP, int MIX>
-__global__ __launch_bounds__(kBlock, 2) void node_flux_kernel(KArgs a) {
-  constexpr int NFLD = 2 + DIM + NSP;
-  const int64_t N = a.nnodes;
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= N) return;
-  Node<DIM, NSP> s;
-  load_q<DIM, NSP>(s, a.q, N, g);
-  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
-  double gq[NFLD][DIM], gt[DIM], F[NFLD][DIM];
-  load_grad<DIM, NSP>(gq, gt, a.grad, N, g);
-  viscous_flux<DIM, NSP, MIX>(s, gq, gt, F);
-  double FI[NFLD][DIM];
-  if (a.inviscid_on) {
-    inviscid_flux<DIM, NSP>(s, FI);
-  } else {
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) FI[f][d] = 0.0;
-  }
-#pragma unroll
-  for (int f = 0; f < NFLD; ++f)
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) a.out[(f * DIM + d) * N + g] = FI[f][d] - F[f][d];
-}
-
-// Boundary face nodes (one thread per element face node; interior ones
-// return at once): the exterior net normal flux with the interior gradients
-// (boundary.py:944-970), or, on Mengaldo nodes, the interior gradients
-// copied to wwork for wall_flux_kernel.
-template <int DIM, int P, int NSP, int MIX>
-__global__ __launch_bounds__(kBlock) void bnd_node_flux_kernel(KArgs a) {
-  using E = Elem<DIM, P>;
-  constexpr int NP = E::NP, NFP = E::NFP, NFN = E::NFN, NF = E::NF;
-  constexpr int NFLD = 2 + DIM + NSP, NS = NFLD + 1;
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= a.K * NFN) return;
-  const int32_t nb = a.vmap_p[t];
-  if (nb >= 0) return;
-  const int64_t e = t / NFN;
-  const int fn = int(t - e * NFN);
-  const int64_t b = -int64_t(nb) - 1;
-  const int f = fn / NFP;
-  const int64_t N = a.nnodes;
-  const int64_t gi = e * NP + a.fmask[fn];
-  if (wall_flags(a, b) & kBcMengaldo) {
-#pragma unroll
-    for (int k = 0; k < NS * DIM; ++k) a.wwork[k * a.nbnd + b] = a.grad[k * N + gi];
-    return;
-  }
-  double n[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + f) * DIM + d];
-  Node<DIM, NSP> s;
-  load_q<DIM, NSP>(s, a.qb, a.nbnd, b);
-  node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.tb[b] : 0.0);
-  double gq[NFLD][DIM], gt[DIM], fv[NFLD], fi[NFLD];
-  load_grad<DIM, NSP>(gq, gt, a.grad, N, gi);
-  if (a.inviscid_on) {
-    inviscid_flux_n<DIM, NSP>(s, n, fi);
-  } else {
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
-  }
-  viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
-#pragma unroll
-  for (int k = 0; k < NFLD; ++k) a.fnb_out[k * a.nbnd + b] = fi[k] - fv[k];
-}
-
-// }}}
-
-// {{{ Mengaldo boundary nodes
-//
-// One thread per boundary node; nodes without MG_BC_MENGALDO return at once.
-// FUSED: the gradients come from wwork (stored by grad_kernel<FLUX>) and the
-// node's exterior net normal flux F_I(q+).n - [2 F_V(bc) - F_V(q-)].n (- p_ref n
-// for the mixed path) is written to fnb_out.  Otherwise the gradients come
-// from the full gradient array and the effective exterior viscous flux
-// 2 F_V(bc).n - F_V(q-).n is written to wwork for rhs_kernel<kGrad>.
-template <int DIM, int NSP, int MIX, bool FUSED>
-__global__ __launch_bounds__(kBlock) void wall_flux_kernel(KArgs a, int shift_pref) {
-  constexpr int NFLD = 2 + DIM + NSP;
-  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (b >= a.nbnd) return;
-  const int fl = a.wflag[b];
-  if (!(fl & kBcMengaldo)) return;
-  const int64_t gm = a.wvol[b];
-  double n[DIM];
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) n[d] = a.wn[d * a.nbnd + b];
-  double gq[NFLD][DIM], gt[DIM], fv[NFLD];
-  if constexpr (FUSED) {
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k)
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) gq[k][d] = a.wwork[(k * DIM + d) * a.nbnd + b];
-#pragma unroll
-    for (int d = 0; d < DIM; ++d) gt[d] = a.wwork[(NFLD * DIM + d) * a.nbnd + b];
-  } else {
-    load_grad<DIM, NSP>(gq, gt, a.grad, a.nnodes, gm);
-  }
-  wall_viscous_flux_n<DIM, NSP, MIX>(a, b, gm, fl, n, gq, gt, fv);
-  if constexpr (FUSED) {
-    double fi[NFLD];
-    if (a.inviscid_on) {
-      Node<DIM, NSP> s;
-      load_q<DIM, NSP>(s, a.qb, a.nbnd, b);
-      node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.tb[b] : 0.0);
-      inviscid_flux_n<DIM, NSP>(s, n, fi);
-    } else {
-#pragma unroll
-      for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
-    }
-    const double p_ref = shift_pref ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k)
-      a.fnb_out[k * a.nbnd + b] =
-          fi[k] - fv[k] - ((k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0);
-  } else {
-#pragma unroll
-    for (int k = 0; k < NFLD; ++k) a.wwork[k * a.nbnd + b] = fv[k];
-  }
-}
-
-// }}}
-
-// {{{ pointwise mixture kernels
-
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void mix_temperature_kernel(
-    int64_t n, const double* __restrict__ q, const double* __restrict__ tseed,
-    int niter, double* __restrict__ tout) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const double rho = q[g];
-  double mm = 0.0;
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) mm += q[(2 + d) * n + g] * q[(2 + d) * n + g];
-  const double e = (q[n + g] - 0.5 * mm / rho) / rho;
-  double y[MIX_NSP];
-#pragma unroll
-  for (int k = 0; k < MIX_NSP; ++k) y[k] = q[(2 + DIM + k) * n + g] / rho;
-  tout[g] = mix_temperature(e, tseed[g], y, niter);
-}
-
-template <int DIM>
-__global__ __launch_bounds__(kBlock) void species_source_kernel(
-    int64_t n, const double* __restrict__ q, const double* __restrict__ temp,
-    double* __restrict__ src) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const double rho = q[g];
-  const double t = temp[g];
-  double conc[MIX_NSP];
-#pragma unroll
-  for (int k = 0; k < MIX_NSP; ++k) {
-    const double c = c_mech.inv_wt[k] * rho * (q[(2 + DIM + k) * n + g] / rho);
-    conc[k] = c < 0.0 ? 0.0 : c;
-  }
-  const double logt = log(t);
-  const double inv_t = 1.0 / t;
-  double kf[3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-    kf[r] = exp(c_mech.log_a[r] + c_mech.b[r] * logt - c_mech.ea_r[r] * inv_t);
-  // reverse of R2 (CO + 0.5 O2 <=> CO2): 1/Kc = exp(dG/RT + 0.5 ln(p0/RT))
-  double g_rt[MIX_NSP];
-#pragma unroll
-  for (int k = 0; k < MIX_NSP; ++k) {
-    double cp_r, h_rt;
-    nasa7_cp_h(k, t, inv_t, cp_r, h_rt);
-    g_rt[k] = h_rt - nasa7_s_r(k, t, logt);
-  }
-  const double c0 = log(c_mech.one_atm / (c_mech.gas_constant * t));
-  const double log_rkc = g_rt[2] - (g_rt[3] + 0.5 * g_rt[1]) + 0.5 * c0;
-  const double sq_o2 = sqrt(conc[1]);
-  const double r1 = kf[0] * sqrt(conc[0]) * pow(conc[1], 0.65);
-  const double r2 = kf[1] * (conc[3] * sq_o2 - exp(log_rkc) * conc[2]);
-  const double r3 = kf[2] * pow(conc[5], 0.75) * sq_o2;
-  const double w[MIX_NSP] = {-r1, -r1 - 0.5 * r2 - 0.5 * r3, r2, 2 * r1 - r2, r3,
-                             2 * r1 - r3, 0.0};
-#pragma unroll
-  for (int k = 0; k < MIX_NSP; ++k) src[k * n + g] = c_mech.wt[k] * w[k];
-}
-
-// Node-local stable timestep (inviscid.py:379-422, viscous.py:547-607):
-//   dt = h_e / (|v| + c)                                   inviscid
-//   dt = h_e / (|v| + c + (mu / rho + max_k d_k) / h_e)     viscous
-// with h_e the characteristic length of the node's element.
-template <int DIM, int NSP, int MIX>
-__global__ __launch_bounds__(kBlock) void local_timestep_kernel(
-    int64_t n, int np, const double* __restrict__ q, const double* __restrict__ temp,
-    const double* __restrict__ hlen, GasDev gas, int viscous,
-    double* __restrict__ dt) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  Node<DIM, NSP> s;
-  load_q<DIM, NSP>(s, q, n, g);
-  node_state<DIM, NSP, MIX, false>(s, gas, MIX ? temp[g] : 0.0);
-  const double h = hlen[g / np];
-  double rate = s.speed + s.c;
-  if (viscous) {
-    node_transport<DIM, NSP, MIX>(s, gas);
-    double dmax = 0.0;
-#pragma unroll
-    for (int k = 0; k < NSP; ++k) dmax = k == 0 ? s.d[0] : fmax(dmax, s.d[k]);
-    rate += (s.mu * s.inv_rho + dmax) / h;
-  }
-  dt[g] = h / rate;
-}
-
-// Transport variables of every node, from the same node_state /
-// node_transport code the Navier-Stokes kernels run.
-template <int DIM, int NSP, int MIX>
-__global__ __launch_bounds__(kBlock) void transport_vars_kernel(
-    GasDev gas, int64_t n, const double* __restrict__ q,
-    const double* __restrict__ temp, double* mu, double* mu_b, double* kappa,
-    double* diff) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  Node<DIM, NSP> s;
-#pragma unroll
-  for (int f = 0; f < Node<DIM, NSP>::NFLD; ++f) s.q[f] = q[f * n + g];
-  node_state<DIM, NSP, MIX, true>(s, gas, MIX ? temp[g] : 0.0);
-  mu[g] = s.mu;
-  mu_b[g] = s.mu_b;
-  kappa[g] = s.kappa;
-#pragma unroll
-  for (int k = 0; k < NSP; ++k) diff[k * n + g] = s.d[k];
-}
-
-__global__ __launch_bounds__(kBlock) void rk_axpy_kernel(
-    int64_t n, const double* y, const double* k, double a, double* out,
-    double* acc, double w) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= n) return;
-  const double kv = k[g];
-  out[g] = y[g] + a * kv;
-  if (acc) acc[g] = acc[g] + w * kv;
-}
-
-
-// Bound-preserving limiter (mirgecom/limiter.py:39-125; Zhang & Shu 2011,
-// eq. 2.9): u <- avg + theta (u - avg) per element, theta the largest factor
-// in [0, 1] that brings the element's extrema inside [mmin, mmax].  The cell
-// average is the mass-weighted mean: on an affine simplex the Jacobian is
-// constant and cancels, so avg = sum_j w_j u_j / sum_j w_j with w the row
-// sums of the reference mass matrix.  One thread per element; a block stages
-// kLimElems consecutive elements of one field through LDS so that loads and
-// stores stay contiguous (the element rows are np doubles long).
-constexpr int kLimElems = 128;
-constexpr int kLimFlagMax = 1;
-constexpr int kLimFlagModifyAvg = 2;
-
-template <int NP>
-__global__ __launch_bounds__(kLimElems) void bound_limiter_kernel(
-    int64_t nelements, const double* __restrict__ weights,
-    const double* __restrict__ field, double* out, double mmin, double mmax,
-    int flags) {
-  constexpr int S = NP | 1;  // odd LDS row stride
-  __shared__ double tile[kLimElems * S];
-  const int64_t e0 = int64_t(blockIdx.x) * kLimElems;
-  const int64_t ne = nelements - e0 < kLimElems ? nelements - e0 : kLimElems;
-  const int64_t base = (int64_t(blockIdx.y) * nelements + e0) * NP;
-  const int span = int(ne) * NP;
-  const int t = threadIdx.x;
-  for (int i = t; i < span; i += kLimElems) {
-    const int e = i / NP;
-    tile[e * S + (i - e * NP)] = field[base + i];
-  }
-  __syncthreads();
-  if (t < ne) {
-    double* u = tile + t * S;
-    double num = 0.0, den = 0.0, umin = u[0], umax = u[0];
-#pragma unroll
-    for (int j = 0; j < NP; ++j) {
-      num += weights[j] * u[j];
-      den += weights[j];
-      umin = fmin(umin, u[j]);
-      umax = fmax(umax, u[j]);
-    }
-    double avg = num / fabs(den);
-    const bool modify = (flags & kLimFlagModifyAvg) != 0;
-    if (modify && !(avg > mmin)) avg = mmin;
-    double theta = 1.0;
-    if (umin < mmin) theta = fmin(theta, fabs((mmin - avg) / (umin - avg + 1e-13)));
-    if (flags & kLimFlagMax) {
-      if (modify && avg > mmax) avg = mmax;
-      if (umax > mmax) theta = fmin(theta, fabs((mmax - avg) / (umax - avg + 1e-13)));
-    }
-#pragma unroll
-    for (int j = 0; j < NP; ++j) u[j] = theta * (u[j] - avg) + avg;
-  }
-  __syncthreads();
-  for (int i = t; i < span; i += kLimElems) {
-    const int e = i / NP;
-    out[base + i] = tile[e * S + (i - e * NP)];
-  }
-}
-
-// }}}
-
-}  // namespace mgdg
-
-// {{{ host side: dispatch and C ABI
 
 namespace {
-
-thread_local std::string g_err;
-
-mg_status fail(mg_status code, const std::string& msg) {
-  g_err = msg;
-  return code;
+std::vector<UploadFn>& uploads() {
+  static std::vector<UploadFn> v;
+  return v;
 }
-
-mg_status check_launch(const char* what) {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess)
-    return fail(MG_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-  return MG_OK;
+std::mutex& uploads_mutex() {
+  static std::mutex m;
+  return m;
 }
-
-mgdg::GasDev to_dev(const mg_gas* g) {
-  mgdg::GasDev d{};
-  d.eos_kind = g->eos_kind;
-  d.transport_kind = g->transport_kind;
-  d.gamma = g->gamma;
-  d.gas_const = g->gas_const;
-  d.mu = g->mu;
-  d.mu_b = g->mu_b;
-  d.kappa = g->kappa;
-  d.pl_alpha = g->pl_alpha;
-  d.pl_beta = g->pl_beta;
-  d.pl_sigma = g->pl_sigma;
-  d.pl_n = g->pl_n;
-  d.has_lewis = g->has_lewis;
-  for (int k = 0; k < 8; ++k) d.diff[k] = g->species_diffusivity[k];
-  d.ma_alpha = g->ma_alpha;
-  d.ma_factor = g->ma_factor;
-  d.ma_epsilon = g->ma_epsilon;
-  d.ma_singular = g->ma_singular_diffusivity;
-  return d;
-}
-
-mg_status check_mesh(const mg_mesh* m) {
-  using mgdg::binom;
-  if (!m) return fail(MG_ERR_ARG, "mesh is NULL");
-  if (m->dim < 2 || m->dim > 3) return fail(MG_ERR_UNSUPPORTED, "dim must be 2 or 3");
-  if (m->order < 1 || m->order > 4) return fail(MG_ERR_UNSUPPORTED, "order must be 1..4");
-  if (m->np != binom(m->order + m->dim, m->dim) ||
-      m->nfp != binom(m->order + m->dim - 1, m->dim - 1) || m->nfaces != m->dim + 1)
-    return fail(MG_ERR_ARG, "np/nfp/nfaces inconsistent with dim and order");
-  if (m->nelements <= 0) return fail(MG_ERR_ARG, "nelements must be positive");
-  if (m->nelements * m->np >= (int64_t(1) << 31))
-    return fail(MG_ERR_ARG, "node count exceeds int32 connectivity");
-  if (!m->rx || !m->normals || !m->fscale || !m->vmap_p || !m->weak_diff ||
-      !m->lift || !m->face_mask)
-    return fail(MG_ERR_ARG, "mesh has a NULL array");
-  if (!m->workspace)
-    return fail(MG_ERR_ARG, "mesh workspace not prepared (mg_mesh_prepare)");
-  return MG_OK;
-}
-
-mgdg::KArgs make_args(const mg_mesh* m, const mg_gas* g) {
-  mgdg::KArgs a{};
-  a.rx = m->rx;
-  a.normals = m->normals;
-  a.fscale = m->fscale;
-  a.vmap_p = m->vmap_p;
-  a.dw = m->weak_diff;
-  a.lift = m->lift;
-  a.fmask = m->face_mask;
-  a.ops = m->workspace;
-  a.K = m->nelements;
-  a.nbnd = m->nbnd_nodes;
-  a.nnodes = m->nelements * m->np;
-  a.gas = to_dev(g);
-  return a;
-}
-
-template <typename Kern>
-mg_status set_lds(Kern kern, size_t lds) {
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(kern),
-                                     hipFuncAttributeMaxDynamicSharedMemorySize,
-                                     int(lds));
-  if (e != hipSuccess)
-    return fail(MG_ERR_HIP, std::string("hipFuncSetAttribute: ") + hipGetErrorString(e));
-  return MG_OK;
-}
-
-template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
-mg_status launch_rhs_kernel(const mgdg::KArgs& a, hipStream_t st) {
-  using C = mgdg::RhsCfg<DIM, P, NSP, MIX, MODE, R>;
-  auto kern = mgdg::rhs_kernel<DIM, P, NSP, MIX, MODE, R, HLL>;
-  static bool attr = false;
-  if (!attr) {
-    mg_status s = set_lds(kern, C::LDS);
-    if (s) return s;
-    attr = true;
-  }
-  const int64_t nblk = (a.K + C::EPB - 1) / C::EPB;
-  hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(mgdg::kBlock), C::LDS, st, a,
-                     a.ops, a.out);
-  return check_launch("rhs_kernel");
-}
-
-template <int DIM, int P, int NSP, int MIX, int MODE, typename R>
-mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
-  const bool inv = MODE == mgdg::kEuler || a.inviscid_on;
-  if (inv && a.flux_kind == MG_FLUX_HLL)
-    return launch_rhs_kernel<DIM, P, NSP, MIX, MODE, R, true>(a, st);
-  return launch_rhs_kernel<DIM, P, NSP, MIX, MODE, R, false>(a, st);
-}
-
-template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
-mg_status launch_grad(const mgdg::KArgs& a, hipStream_t st) {
-  using C = mgdg::GradCfg<DIM, P, NSP, MIX, R>;
-  auto kern = mgdg::grad_kernel<DIM, P, NSP, MIX, FLUX, R>;
-  static bool attr = false;
-  if (!attr) {
-    mg_status s = set_lds(kern, C::LDS);
-    if (s) return s;
-    attr = true;
-  }
-  const int64_t nblk = (a.K + C::EPB - 1) / C::EPB;
-  hipLaunchKernelGGL(kern, dim3(unsigned(nblk)), dim3(mgdg::kBlock), C::LDS, st, a,
-                     a.ops, a.out);
-  return check_launch("grad_kernel");
-}
-
-// compiled (species count, EOS) combinations: (0, ideal), (7, ideal), (7, mixture)
-// mix: 0 ideal gas, 1 mixture, 2 mixture with mixture-averaged transport
-// (a separate instantiation only where the kernel evaluates transport).
-template <int DIM, int P, int MODE, typename R>
-mg_status rhs_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
-  if (nsp == 0 && !mix) return launch_rhs<DIM, P, 0, 0, MODE, R>(a, st);
-  if (nsp == 7 && !mix) return launch_rhs<DIM, P, 7, 0, MODE, R>(a, st);
-  if constexpr (MODE == mgdg::kGrad) {
-    if (nsp == 7 && mix == 2) return launch_rhs<DIM, P, 7, 2, MODE, R>(a, st);
-  }
-  if (nsp == 7 && mix) return launch_rhs<DIM, P, 7, 1, MODE, R>(a, st);
-  return fail(MG_ERR_UNSUPPORTED,
-              "unsupported species count / EOS combination (nspecies 0 or 7)");
-}
-
-template <int DIM, int P, bool FLUX, typename R>
-mg_status grad_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
-  if (nsp == 0 && !mix) return launch_grad<DIM, P, 0, 0, FLUX, R>(a, st);
-  if (nsp == 7 && !mix) return launch_grad<DIM, P, 7, 0, FLUX, R>(a, st);
-  if constexpr (FLUX) {
-    if (nsp == 7 && mix == 2) return launch_grad<DIM, P, 7, 2, FLUX, R>(a, st);
-  }
-  if (nsp == 7 && mix) return launch_grad<DIM, P, 7, 1, FLUX, R>(a, st);
-  return fail(MG_ERR_UNSUPPORTED,
-              "unsupported species count / EOS combination (nspecies 0 or 7)");
-}
-
-// Nodal net fluxes of the uiuc_7sp mixture from stored gradients (a.grad).
-template <int DIM, int P>
-mg_status node_flux_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
-  if (nsp != 7 || !mix)
-    return fail(MG_ERR_UNSUPPORTED, "node-parallel net flux is the mixture path");
-  const unsigned nblk = unsigned((a.nnodes + mgdg::kBlock - 1) / mgdg::kBlock);
-  if (mix == 2)
-    hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 2>), dim3(nblk),
-                       dim3(mgdg::kBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 1>), dim3(nblk),
-                       dim3(mgdg::kBlock), 0, st, a);
-  mg_status s = check_launch("node_flux_kernel");
-  if (s || a.nbnd == 0) return s;
-  const int64_t nt = a.K * mgdg::Elem<DIM, P>::NFN;
-  const unsigned nb = unsigned((nt + mgdg::kBlock - 1) / mgdg::kBlock);
-  if (mix == 2)
-    hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 2>), dim3(nb),
-                       dim3(mgdg::kBlock), 0, st, a);
-  else
-    hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 1>), dim3(nb),
-                       dim3(mgdg::kBlock), 0, st, a);
-  return check_launch("bnd_node_flux_kernel");
-}
-
-template <int DIM, int P>
-mg_status dispatch_p(const mgdg::KArgs& a, int nsp, int mix, int what,
-                     hipStream_t st) {
-  switch (what) {
-    case 0: return rhs_species<DIM, P, mgdg::kEuler, double>(a, nsp, mix, st);
-    case 1: return rhs_species<DIM, P, mgdg::kGrad, double>(a, nsp, mix, st);
-    case 2: return rhs_species<DIM, P, mgdg::kFlux, double>(a, nsp, mix, st);
-    case 3: return grad_species<DIM, P, false, double>(a, nsp, mix, st);
-    case 4: return grad_species<DIM, P, true, double>(a, nsp, mix, st);
-    case 5: return grad_species<DIM, P, true, float>(a, nsp, mix, st);
-    case 6: return rhs_species<DIM, P, mgdg::kFlux, float>(a, nsp, mix, st);
-    case 7: return node_flux_species<DIM, P>(a, nsp, mix, st);
-  }
-  return fail(MG_ERR_ARG, "kernel selector");
-}
-
-// what: 0 Euler RHS, 1 NS RHS from gradients, 2 NS RHS from nodal fluxes,
-// 3 gradients, 4 gradients + nodal fluxes, 5/6 the mixed-precision 4/2,
-// 7 nodal fluxes from stored gradients (mixture)
-mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, int mix,
-                   int what, hipStream_t st) {
-#define MG_CASE(D, P) \
-  if (m->dim == D && m->order == P) return dispatch_p<D, P>(a, nsp, mix, what, st);
-  MG_CASE(2, 1) MG_CASE(2, 2) MG_CASE(2, 3) MG_CASE(2, 4)
-  MG_CASE(3, 1) MG_CASE(3, 2) MG_CASE(3, 3) MG_CASE(3, 4)
-#undef MG_CASE
-  return fail(MG_ERR_UNSUPPORTED, "dim/order");
-}
-
-bool g_tran_loaded = false;
-
-// Kernel variant of a gas model: 0 ideal gas, 1 uiuc_7sp mixture, 2 mixture
-// with mixture-averaged transport.
-int gas_variant(const mg_gas* g) {
-  return g->eos_kind != 1 ? 0 : (g->transport_kind == 3 ? 2 : 1);
-}
-
-mg_status check_gas(const mg_gas* g, bool need_transport) {
-  if (!g) return fail(MG_ERR_ARG, "gas is NULL");
-  if (g->transport_kind < 0 || g->transport_kind > 3)
-    return fail(MG_ERR_ARG, "transport_kind");
-  if (g->transport_kind == 3 && g->eos_kind != 1)
-    return fail(MG_ERR_ARG, "mixture-averaged transport needs the mixture EOS");
-  if (g->transport_kind == 3 && !g_tran_loaded)
-    return fail(MG_ERR_ARG, "transport fits not loaded (mg_load_transport)");
-  if (g->eos_kind == 1 && g->nspecies != 7)
-    return fail(MG_ERR_ARG, "the mixture EOS needs the 7 uiuc_7sp species");
-  if (g->eos_kind != 0 && g->eos_kind != 1) return fail(MG_ERR_ARG, "eos_kind");
-  if (g->nspecies < 0 || g->nspecies > 8) return fail(MG_ERR_ARG, "nspecies");
-  if (need_transport && g->transport_kind == 0)
-    return fail(MG_ERR_ARG, "Navier-Stokes operator expects a viscous gas model");
-  return MG_OK;
-}
-
-bool g_mech_loaded = false;
-
-mg_status ensure_mech() {
-  if (g_mech_loaded) return MG_OK;
-  return fail(MG_ERR_ARG, "mixture mechanism not loaded (mg_load_mechanism)");
-}
-
-template <int D, int P>
-mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
-  const int n = mgdg::Ops<D, P>::TOTAL + mgdg::Ops<D, P>::MF;
-  auto kern = mgdg::pack_ops_kernel<D, P>;
-  hipLaunchKernelGGL(kern, dim3((n + 255) / 256), dim3(256), 0, st, m->weak_diff,
-                     m->lift, ws);
-  mg_status s = check_launch("pack_ops_kernel");
-  if (s) return s;
-  m->workspace = ws;
-  return MG_OK;
-}
-
 }  // namespace
 
-mg_status set_wall(mgdg::KArgs& a, const mg_mesh* m, const mg_wall* w, bool mix) {
-  if (!w || !w->flags || m->nbnd_nodes == 0) return MG_OK;
-  if (!w->q_bc || !w->t_bc || (mix && !w->t_state_bc) || !w->vol_index ||
-      !w->normals || !w->work)
-    return fail(MG_ERR_ARG,
-                "mg_wall: q_bc, t_bc, vol_index, normals, work (and t_state_bc for "
-                "mixtures) needed");
-  a.wq = w->q_bc;
-  a.wt = w->t_bc;
-  a.wts = w->t_state_bc;
-  a.wflag = w->flags;
-  a.wvol = w->vol_index;
-  a.wn = w->normals;
-  a.wwork = w->work;
-  return MG_OK;
+int register_upload(UploadFn fn) {
+  std::lock_guard<std::mutex> lk(uploads_mutex());
+  uploads().push_back(fn);
+  return int(uploads().size());
 }
 
-template <int DIM, int NSP, int MIX, bool FUSED>
-mg_status launch_wall(const mgdg::KArgs& a, int shift, hipStream_t st) {
-  const unsigned nblk = unsigned((a.nbnd + mgdg::kBlock - 1) / mgdg::kBlock);
-  hipLaunchKernelGGL((mgdg::wall_flux_kernel<DIM, NSP, MIX, FUSED>), dim3(nblk),
-                     dim3(mgdg::kBlock), 0, st, a, shift);
-  return check_launch("wall_flux_kernel");
-}
-
-// Mengaldo boundary nodes: after grad_kernel<FLUX> (fused) or before
-// rhs_kernel<kGrad>; nothing to do without wall data.
-template <bool FUSED>
-mg_status wall_pass(const mgdg::KArgs& a, int dim, int nsp, int mix, int shift,
-                    hipStream_t st) {
-  if (!a.wflag) return MG_OK;
-  if (dim == 2) {
-    if (nsp == 0 && !mix) return launch_wall<2, 0, 0, FUSED>(a, shift, st);
-    if (nsp == 7 && !mix) return launch_wall<2, 7, 0, FUSED>(a, shift, st);
-    if (nsp == 7 && mix == 1) return launch_wall<2, 7, 1, FUSED>(a, shift, st);
-    if (nsp == 7 && mix == 2) return launch_wall<2, 7, 2, FUSED>(a, shift, st);
-  } else {
-    if (nsp == 0 && !mix) return launch_wall<3, 0, 0, FUSED>(a, shift, st);
-    if (nsp == 7 && !mix) return launch_wall<3, 7, 0, FUSED>(a, shift, st);
-    if (nsp == 7 && mix == 1) return launch_wall<3, 7, 1, FUSED>(a, shift, st);
-    if (nsp == 7 && mix == 2) return launch_wall<3, 7, 2, FUSED>(a, shift, st);
+hipError_t upload_all(const mgdg::MechDev* m, const mgdg::PolyDev* p,
+                      const mgdg::TranDev* t) {
+  std::lock_guard<std::mutex> lk(uploads_mutex());
+  for (UploadFn fn : uploads()) {
+    hipError_t e = fn(m, p, t);
+    if (e != hipSuccess) return e;
   }
-  return fail(MG_ERR_UNSUPPORTED,
-              "unsupported species count / EOS combination (nspecies 0 or 7)");
+  return hipSuccess;
 }
 
-template <int DIM, int NSP, int MIX>
-mg_status launch_timestep(const mg_mesh* m, const mgdg::GasDev& g, const double* q,
-                          const double* t, const double* h, int viscous, double* dt,
-                          hipStream_t st) {
-  const int64_t n = m->nelements * m->np;
-  const unsigned nblk = unsigned((n + mgdg::kBlock - 1) / mgdg::kBlock);
-  hipLaunchKernelGGL((mgdg::local_timestep_kernel<DIM, NSP, MIX>), dim3(nblk),
-                     dim3(mgdg::kBlock), 0, st, n, int(m->np), q, t, h, g, viscous, dt);
-  return check_launch("local_timestep_kernel");
-}
+}  // namespace mgdg_shared
 
-template <int DIM>
-mg_status timestep_species(const mg_mesh* m, const mg_gas* gas, const double* q,
-                           const double* t, const double* h, int viscous, double* dt,
-                           hipStream_t st) {
-  const mgdg::GasDev g = to_dev(gas);
-  const int mix = gas_variant(gas);
-  if (gas->nspecies == 0 && !mix)
-    return launch_timestep<DIM, 0, 0>(m, g, q, t, h, viscous, dt, st);
-  if (gas->nspecies == 7 && !mix)
-    return launch_timestep<DIM, 7, 0>(m, g, q, t, h, viscous, dt, st);
-  if (gas->nspecies == 7 && mix == 1)
-    return launch_timestep<DIM, 7, 1>(m, g, q, t, h, viscous, dt, st);
-  if (gas->nspecies == 7 && mix == 2)
-    return launch_timestep<DIM, 7, 2>(m, g, q, t, h, viscous, dt, st);
-  return fail(MG_ERR_UNSUPPORTED,
-              "unsupported species count / EOS combination (nspecies 0 or 7)");
-}
+// }}}
 
 extern "C" {
 
-const char* mg_last_error(void) { return g_err.c_str(); }
+const char* mg_last_error(void) { return mgdg_shared::err().c_str(); }
 
 int32_t mg_abi_version(void) { return 4; }
 
@@ -2564,9 +69,6 @@ int32_t mg_abi_version(void) { return 4; }
 mg_status mg_load_mechanism(const double* packed, int64_t count) {
   if (count != int64_t(sizeof(mgdg::MechDev) / sizeof(double)))
     return fail(MG_ERR_ARG, "mechanism size mismatch");
-  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(mgdg::c_mech), packed,
-                                   sizeof(mgdg::MechDev), 0, hipMemcpyHostToDevice);
-  if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
   mgdg::MechDev m;
   std::memcpy(&m, packed, sizeof(m));
   mgdg::PolyDev poly;
@@ -2580,20 +82,20 @@ mg_status mg_load_mechanism(const double* packed, int64_t count) {
       for (int i = 1; i < 5; ++i) poly.s[r][k][i] = a[i] / double(i);
       poly.s[r][k][5] = a[6];
     }
-  e = hipMemcpyToSymbol(HIP_SYMBOL(mgdg::c_poly), &poly, sizeof(poly), 0,
-                        hipMemcpyHostToDevice);
+  hipError_t e = mgdg_shared::upload_all(&m, &poly, nullptr);
   if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
-  g_mech_loaded = true;
+  mgdg_shared::mech_loaded() = true;
   return MG_OK;
 }
 
 mg_status mg_load_transport(const double* packed, int64_t count) {
   if (!packed || count != int64_t(sizeof(mgdg::TranDev) / sizeof(double)))
     return fail(MG_ERR_ARG, "transport size mismatch");
-  hipError_t e = hipMemcpyToSymbol(HIP_SYMBOL(mgdg::c_tran), packed,
-                                   sizeof(mgdg::TranDev), 0, hipMemcpyHostToDevice);
+  mgdg::TranDev t;
+  std::memcpy(&t, packed, sizeof(t));
+  hipError_t e = mgdg_shared::upload_all(nullptr, nullptr, &t);
   if (e != hipSuccess) return fail(MG_ERR_HIP, hipGetErrorString(e));
-  g_tran_loaded = true;
+  mgdg_shared::tran_loaded() = true;
   return MG_OK;
 }
 
@@ -2882,169 +384,6 @@ mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double
                                dt, st);
   return timestep_species<3>(mesh, gas, q, temperature, lengthscale, viscous ? 1 : 0,
                              dt, st);
-}
-
-mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
-                            const mg_quad* quad, const double* q,
-                            const double* temperature, const double* q_bnd_quad,
-                            const double* t_bnd_quad, int32_t niter, int32_t flux_kind,
-                            double* rhs, void* stream) {
-  mg_status s = check_mesh(mesh);
-  if (s) return s;
-  if ((s = check_gas(gas, false))) return s;
-  const bool mix = gas->eos_kind == 1;
-  if (mix && (s = ensure_mech())) return s;
-  if (gas->nspecies != 0 && gas->nspecies != 7)
-    return fail(MG_ERR_UNSUPPORTED, "nspecies must be 0 or 7");
-  if (mix && (!temperature || niter < 0 || (mesh->nbnd_nodes > 0 && !t_bnd_quad)))
-    return fail(MG_ERR_ARG, "mixture needs temperature (and t_bnd_quad), niter >= 0");
-  if (!quad || !quad->interp || !quad->weak_diff || !quad->face_interp || !quad->lift ||
-      quad->nq < 1 || quad->nfq < 1)
-    return fail(MG_ERR_ARG, "mg_quad incomplete");
-  if (!q || !rhs) return fail(MG_ERR_ARG, "q/rhs NULL");
-  if (mesh->nbnd_nodes > 0 && !q_bnd_quad)
-    return fail(MG_ERR_ARG, "boundary exterior state missing");
-  if (mesh->nbnd_nodes % mesh->nfp) return fail(MG_ERR_ARG, "nbnd_nodes % nfp");
-  if (flux_kind < 0 || flux_kind > 2) return fail(MG_ERR_ARG, "flux_kind");
-  const int nfld = 2 + mesh->dim + gas->nspecies;
-  // LDS row strides = 4 mod 32 doubles (bank spread of the matrix-core reads)
-  auto bank_stride = [](int n) { return n + ((4 - n % 32) + 32) % 32; };
-  const int gs = bank_stride(mesh->dim * quad->nq);
-  const int fs = bank_stride(mesh->nfaces * quad->nfq);
-  const size_t per_elem =
-      sizeof(double) * (size_t(nfld) * (size_t(mesh->np) + size_t(gs) + size_t(fs) +
-                                        size_t(mesh->nfaces) * mesh->nfp) +
-                        size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp);
-  // up to 80 KiB per workgroup: two workgroups per CU, which is what the
-  // kernel's registers allow (at least one element; at most 128 KiB)
-  constexpr size_t kBudget = 128 * 1024;
-  if (per_elem > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS");
-  int epb = int((80 * 1024) / per_elem);
-  if (epb < 1) epb = 1;
-  if (epb > 16) epb = 16;
-  mgdg::KArgs a = make_args(mesh, gas);
-  a.q = q;
-  a.out = rhs;
-  a.flux_kind = flux_kind;
-  mgdg::QuadArgs qa{};
-  qa.nq = quad->nq;
-  qa.nfq = quad->nfq;
-  qa.epb = epb;
-  qa.gs = gs;
-  qa.fs = fs;
-  qa.interp = quad->interp;
-  qa.wdiff = quad->weak_diff;
-  qa.finterp = quad->face_interp;
-  qa.lift = quad->lift;
-  qa.qbq = q_bnd_quad;
-  qa.nbq = mesh->nbnd_nodes / mesh->nfp * quad->nfq;
-  qa.temp = temperature;
-  qa.tbq = t_bnd_quad;
-  qa.niter = niter;
-  const size_t lds = per_elem * epb;
-  const unsigned nblk = unsigned((a.K + epb - 1) / epb);
-  hipStream_t st = static_cast<hipStream_t>(stream);
-#define MG_QCASE(D, P, NS, MX)                                                    \
-  if (mesh->dim == D && mesh->order == P && gas->nspecies == NS && int(mix) == MX) { \
-    auto kern = mgdg::euler_quad_kernel<D, P, NS, MX>;                            \
-    if ((s = set_lds(kern, kBudget))) return s;                                   \
-    hipLaunchKernelGGL(kern, dim3(nblk), dim3(mgdg::kBlock), lds, st, a, qa);     \
-    return check_launch("euler_quad_kernel");                                     \
-  }
-  MG_QCASE(2, 1, 0, 0) MG_QCASE(2, 2, 0, 0) MG_QCASE(2, 3, 0, 0) MG_QCASE(2, 4, 0, 0)
-  MG_QCASE(3, 1, 0, 0) MG_QCASE(3, 2, 0, 0) MG_QCASE(3, 3, 0, 0) MG_QCASE(3, 4, 0, 0)
-  MG_QCASE(2, 1, 7, 0) MG_QCASE(2, 2, 7, 0) MG_QCASE(2, 3, 7, 0) MG_QCASE(2, 4, 7, 0)
-  MG_QCASE(3, 1, 7, 0) MG_QCASE(3, 2, 7, 0) MG_QCASE(3, 3, 7, 0) MG_QCASE(3, 4, 7, 0)
-  MG_QCASE(2, 1, 7, 1) MG_QCASE(2, 2, 7, 1) MG_QCASE(2, 3, 7, 1) MG_QCASE(2, 4, 7, 1)
-  MG_QCASE(3, 1, 7, 1) MG_QCASE(3, 2, 7, 1) MG_QCASE(3, 3, 7, 1) MG_QCASE(3, 4, 7, 1)
-#undef MG_QCASE
-  return fail(MG_ERR_UNSUPPORTED, "dim/order");
-}
-
-mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
-                            const mg_quad* quad, const mg_esdg* esdg, const double* q,
-                            const double* q_bnd_quad, int32_t dissipation,
-                            double* ev_work, double* rhs, void* stream) {
-  mg_status s = check_mesh(mesh);
-  if (s) return s;
-  if ((s = check_gas(gas, false))) return s;
-  if (gas->eos_kind != 0)
-    return fail(MG_ERR_UNSUPPORTED, "the entropy-stable path is built for the ideal gas");
-  if (gas->nspecies != 0 && gas->nspecies != 7)
-    return fail(MG_ERR_UNSUPPORTED, "nspecies must be 0 or 7");
-  if (!quad || !quad->interp || !quad->face_interp || quad->nq < 1 || quad->nfq < 1)
-    return fail(MG_ERR_ARG, "mg_quad incomplete");
-  if (!esdg || !esdg->proj || !esdg->vh || !esdg->skew || !esdg->ext || !esdg->ext_col ||
-      !esdg->apply || !esdg->face_weights ||
-      esdg->nh != quad->nq + mesh->nfaces * quad->nfq)
-    return fail(MG_ERR_ARG, "mg_esdg incomplete or nh != nq + nfaces*nfq");
-  if (!q || !rhs || !ev_work) return fail(MG_ERR_ARG, "q/rhs/ev_work NULL");
-  if (mesh->nbnd_nodes > 0 && !q_bnd_quad)
-    return fail(MG_ERR_ARG, "boundary exterior state missing");
-  if (mesh->nbnd_nodes % mesh->nfp) return fail(MG_ERR_ARG, "nbnd_nodes % nfp");
-  const int nfld = 2 + mesh->dim + gas->nspecies;
-  mgdg::KArgs a = make_args(mesh, gas);
-  a.q = q;
-  a.out = rhs;
-  mgdg::EsdgArgs ea{};
-  ea.nq = quad->nq;
-  ea.nfq = quad->nfq;
-  ea.nh = esdg->nh;
-  ea.interp = quad->interp;
-  ea.finterp = quad->face_interp;
-  ea.proj = esdg->proj;
-  ea.vh = esdg->vh;
-  ea.skew = esdg->skew;
-  ea.ext = esdg->ext;
-  ea.ext_col = esdg->ext_col;
-  ea.apply = esdg->apply;
-  ea.wf = esdg->face_weights;
-  ea.qbq = q_bnd_quad;
-  ea.nbq = mesh->nbnd_nodes / mesh->nfp * quad->nfq;
-  ea.ev = ev_work;
-  ea.dissipation = dissipation ? 1 : 0;
-  ea.skew_diag = esdg->skew_diag;
-  ea.ext_diag = esdg->ext_diag;
-  ea.paired = (mesh->dim == 3 && quad->nq == 64 && mesh->nfaces * quad->nfq == 192 &&
-               esdg->skew_diag && esdg->ext_diag && mgdg::kBlock == 256)
-                  ? 1
-                  : 0;
-  hipStream_t st = static_cast<hipStream_t>(stream);
-  constexpr size_t kBudget = 128 * 1024;
-  const size_t nft = size_t(mesh->nfaces) * quad->nfq;
-#define MG_ECASE(D, P, NS)                                                              \
-  if (mesh->dim == D && mesh->order == P && gas->nspecies == NS) {                      \
-    const size_t per_p = sizeof(double) * nfld * quad->nq;                              \
-    if (per_p > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
-    int epb_p = int(std::max<size_t>(1, std::min<size_t>(8, 256 / quad->nq)));        \
-    ea.epb = epb_p;                                                                     \
-    auto kp = mgdg::esdg_project_kernel<D, P, NS>;                                      \
-    if ((s = set_lds(kp, kBudget))) return s;                                           \
-    hipLaunchKernelGGL(kp, dim3(unsigned((a.K + epb_p - 1) / epb_p)),                   \
-                       dim3(mgdg::kBlock), per_p * epb_p, st, a, ea);                  \
-    if ((s = check_launch("esdg_project_kernel"))) return s;                            \
-    const size_t nchunk = (size_t(esdg->nh) + quad->nq - 1) / quad->nq;               \
-    const size_t nacc = std::max(nchunk * quad->nq + nft, nft);                         \
-    const size_t per_e = sizeof(mgdg::EcPair<D, NS>) * esdg->nh +                       \
-        sizeof(double) * nfld *                                                         \
-            (2 * size_t(mesh->np) + size_t(mesh->nfaces) * mesh->nfp +                  \
-             (ea.paired ? 0 : nft) + nacc);                                             \
-    if (per_e > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
-    int epb = ea.paired ? 1                                                             \
-                        : int(std::max<size_t>(1, std::min<size_t>(16, (64 * 1024) / per_e))); \
-    ea.epb = epb;                                                                       \
-    auto kern = mgdg::esdg_kernel<D, P, NS>;                                            \
-    if ((s = set_lds(kern, kBudget))) return s;                                         \
-    hipLaunchKernelGGL(kern, dim3(unsigned((a.K + epb - 1) / epb)), dim3(mgdg::kBlock), \
-                       per_e * epb, st, a, ea);                                         \
-    return check_launch("esdg_kernel");                                                 \
-  }
-  MG_ECASE(2, 1, 0) MG_ECASE(2, 2, 0) MG_ECASE(2, 3, 0) MG_ECASE(2, 4, 0)
-  MG_ECASE(3, 1, 0) MG_ECASE(3, 2, 0) MG_ECASE(3, 3, 0) MG_ECASE(3, 4, 0)
-  MG_ECASE(2, 1, 7) MG_ECASE(2, 2, 7) MG_ECASE(2, 3, 7) MG_ECASE(2, 4, 7)
-  MG_ECASE(3, 1, 7) MG_ECASE(3, 2, 7) MG_ECASE(3, 3, 7) MG_ECASE(3, 4, 7)
-#undef MG_ECASE
-  return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }
 
 mg_status mg_transport_vars(const mg_gas* gas, int32_t dim, int64_t n,

@@ -50,7 +50,9 @@ struct MechDev {
   double log_a[3], b[3], ea_r[3];
 };
 
-extern __constant__ MechDev c_mech;
+// one copy per translation unit (each .hip file is its own device
+// module); mg_load_mechanism / mg_load_transport fill every copy
+static __constant__ MechDev c_mech;
 
 // NASA7 polynomials rearranged for Horner evaluation, derived on the host
 // from the mechanism's coefficients a0..a6 when it is loaded ([0] low range,
@@ -64,7 +66,9 @@ struct PolyDev {
   double s[2][MIX_NSP][6];
 };
 
-extern __constant__ PolyDev c_poly;
+// one copy per translation unit (each .hip file is its own device
+// module); mg_load_mechanism / mg_load_transport fill every copy
+static __constant__ PolyDev c_poly;
 
 // The temperature range is chosen per lane, the species index is
 // wave-uniform: both ranges' polynomials are evaluated with scalar-operand
@@ -176,7 +180,9 @@ struct TranDev {
   double phi_b[MIX_NSP][MIX_NSP];
 };
 
-extern __constant__ TranDev c_tran;
+// one copy per translation unit (each .hip file is its own device
+// module); mg_load_mechanism / mg_load_transport fill every copy
+static __constant__ TranDev c_tran;
 
 // }}}
 

@@ -110,3 +110,12 @@ def test_boundaries(kind):
     err = rel_err(got, ref)
     print("err", err)
     assert err.max() < 1e-11
+
+
+def test_3d_p4_unsupported():
+    """3-D p = 4 has no parity coverage: the library refuses it."""
+    from mirgecom_amd._lib import MgError
+    pb = Problem(3, 4, (3, 3, 3), 0.0, 2 * np.pi, (True,) * 3)
+    q = smooth_perturbed_state(pb.x, seed=5)
+    with pytest.raises(MgError, match="3-D p = 4"):
+        _run(pb, q, 3)

@@ -106,3 +106,46 @@ def test_graphed_rhs_with_prescribed_boundary():
         eager = rhs(0.25, cv).q.clone()
         torch.cuda.synchronize()
         assert torch.equal(g(0.25, cv).q, eager)
+
+
+def test_graphed_quadrature_rhs_with_prescribed_boundary():
+    """The overintegrated Euler RHS (quadrature_tag=DISCR_TAG_QUAD) with a
+    prescribed boundary inside a capture: the boundary-coverage check runs
+    once per tag set outside the capture, every replay equals the eager
+    call."""
+    from mirgecom_amd.boundary import PrescribedFluidBoundary
+    from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
+    from mirgecom_amd.eos import IdealSingleGas
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from mirgecom_amd.graphs import GraphedRHS
+    from mirgecom_amd.initializers import Vortex2D
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+
+    pb = Problem(2, 2, (5, 5), (0.0, -5.0), (10.0, 5.0), (False, False))
+    vortex = Vortex2D(beta=5, center=(5, 0), velocity=(1, 0))
+    gm = GasModel(eos=IdealSingleGas())
+
+    def bstate(dcoll, dd_bdry, gas_model, state_minus, time=0.0, **kw):
+        return make_fluid_state(vortex(dcoll.nodes(dd_bdry), time=time), gas_model)
+
+    bnd = {"all": PrescribedFluidBoundary(boundary_state_func=bstate)}
+
+    def rhs(t, cv):
+        return euler_operator(pb.dcoll, make_fluid_state(cv, gm), gm, bnd, time=t,
+                              inviscid_numerical_flux_func=inviscid_facial_flux_rusanov,
+                              quadrature_tag=DISCR_TAG_QUAD)
+
+    cv0 = vortex(pb.dcoll.nodes(), time=0.0)
+    g = GraphedRHS(rhs, cv0, t=0.1)
+    for rep in range(2):
+        for s in (0.0, 0.05, 0.15):
+            cv = ConservedVars(2, vortex(pb.dcoll.nodes(), time=s).q)
+            eager = rhs(0.1, cv).q.clone()
+            torch.cuda.synchronize()
+            got = g(0.1, cv).q
+            torch.cuda.synchronize()
+            assert torch.equal(got, eager), (rep, s)
+            _ = (cv.q * 3.0).sum()
+            torch.cuda.synchronize()
