@@ -304,6 +304,23 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
                             const double* q_bnd_quad, int32_t dissipation,
                             double* ev_work, double* rhs, void* stream);
 
+/* Allocation-free variant of mg_euler_rhs_esdg.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The projected entropy variables that mg_euler_rhs_esdg keeps in the
+ * caller's ev_work live in a file-scope device array of the library instead:
+ * 2^23 doubles (64 MiB) per GPU, held until the process ends, so
+ * (2 + dim + nspecies) * nelements * np must not exceed 8,388,608 (the
+ * 24^3 x 6-tet p = 3 ideal-gas box fits); larger states are rejected with
+ * MG_ERR_ARG.  The call allocates nothing and frees nothing.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (other streams or
+ * host threads) race and must be serialised by the caller.  Results equal
+ * mg_euler_rhs_esdg's bit for bit. */
+mg_status mg_euler_rhs_esdg_noalloc(const mg_mesh* mesh, const mg_gas* gas,
+                                    const mg_quad* quad, const mg_esdg* esdg,
+                                    const double* q, const double* q_bnd_quad,
+                                    int32_t dissipation, double* rhs, void* stream);
+
 /* Upload the uiuc_7sp mechanism constants (NASA7 tables, molecular weights,
  * gas constant, standard pressure, SI Arrhenius parameters; 130 doubles in
  * the order of mirgecom_amd.mechanisms.packed_mechanism) to the device.

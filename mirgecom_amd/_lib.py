@@ -113,6 +113,11 @@ SIGNATURES = {
     "mg_euler_rhs_esdg": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  ctypes.POINTER(MgQuad), ctypes.POINTER(MgEsdg),
                                  _P, _P, _I32, _P, _P, _P]),
+    "mg_euler_rhs_esdg_noalloc": (_I32, [ctypes.POINTER(MgMesh),
+                                         ctypes.POINTER(MgGas),
+                                         ctypes.POINTER(MgQuad),
+                                         ctypes.POINTER(MgEsdg),
+                                         _P, _P, _I32, _P, _P]),
     "mg_bound_limiter": (_I32, [_I32, _I64, _I32, _P, _P, _P, _D, _D, _I32,
                                 _P]),
     "mg_load_mechanism": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),

@@ -1,6 +1,8 @@
 // Quadrature (overintegration) and entropy-stable Euler entry points of
 // libmgdg (include/mgdg.h): mg_euler_rhs_quad, mg_euler_rhs_esdg.
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 #include "mgdg_host.h"
 
@@ -83,10 +85,15 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
   return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }
 
-mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
-                            const mg_quad* quad, const mg_esdg* esdg, const double* q,
-                            const double* q_bnd_quad, int32_t dissipation,
-                            double* ev_work, double* rhs, void* stream) {
+}  // extern "C"
+
+namespace {
+
+// Checks shared by both entropy-stable entry points (everything that does
+// not depend on where the projected entropy variables live).
+mg_status esdg_check(const mg_mesh* mesh, const mg_gas* gas, const mg_quad* quad,
+                     const mg_esdg* esdg, const double* q, const double* q_bnd_quad,
+                     const double* rhs) {
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, false))) return s;
@@ -100,10 +107,19 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
       !esdg->apply || !esdg->face_weights ||
       esdg->nh != quad->nq + mesh->nfaces * quad->nfq)
     return fail(MG_ERR_ARG, "mg_esdg incomplete or nh != nq + nfaces*nfq");
-  if (!q || !rhs || !ev_work) return fail(MG_ERR_ARG, "q/rhs/ev_work NULL");
+  if (!q || !rhs) return fail(MG_ERR_ARG, "q/rhs NULL");
   if (mesh->nbnd_nodes > 0 && !q_bnd_quad)
     return fail(MG_ERR_ARG, "boundary exterior state missing");
   if (mesh->nbnd_nodes % mesh->nfp) return fail(MG_ERR_ARG, "nbnd_nodes % nfp");
+  return MG_OK;
+}
+
+mg_status esdg_run(const mg_mesh* mesh, const mg_gas* gas, const mg_quad* quad,
+                   const mg_esdg* esdg, const double* q, const double* q_bnd_quad,
+                   int32_t dissipation, double* ev_work, double* rhs, void* stream) {
+  mg_status s = esdg_check(mesh, gas, quad, esdg, q, q_bnd_quad, rhs);
+  if (s) return s;
+  if (!ev_work) return fail(MG_ERR_ARG, "ev_work NULL");
   const int nfld = 2 + mesh->dim + gas->nspecies;
   mgdg::KArgs a = make_args(mesh, gas);
   a.q = q;
@@ -171,6 +187,75 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
   if (mesh->dim == 3 && mesh->order == 4)
     return fail(MG_ERR_UNSUPPORTED, "entropy-stable path: 3-D p = 4 is not supported");
   return fail(MG_ERR_UNSUPPORTED, "dim/order");
+}
+
+// {{{ allocation-free variant: the projected entropy variables in `scratch`
+//
+// `scratch` is a file-scope device array: each GPU that loads the library
+// holds its own copy (64 MiB) until the process ends.  Host code never uses
+// the array's name as an address (on the host it is only a stand-in for the
+// symbol): the device address is looked up once per device with
+// hipGetSymbolAddress, with that device current, and kept in g_scratch_ptr.
+// Every call on a GPU shares that GPU's copy, so calls that overlap on one
+// GPU (different streams, or host threads) race; the projection kernel
+// overwrites every value the flux-differencing kernel reads, so nothing needs
+// clearing between calls.
+
+constexpr int64_t kEsdgScratchDoubles = int64_t(1) << 23;   // 64 MiB
+__device__ __align__(256) double scratch[kEsdgScratchDoubles];
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+mg_status device_scratch(double** out) {
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess)
+    return fail(MG_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices)
+    return fail(MG_ERR_UNSUPPORTED, "device index beyond the 64 the scratch table holds");
+  std::lock_guard<std::mutex> lk(g_scratch_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MG_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return MG_OK;
+}
+
+// }}}
+
+}  // namespace
+
+extern "C" {
+
+mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
+                            const mg_quad* quad, const mg_esdg* esdg, const double* q,
+                            const double* q_bnd_quad, int32_t dissipation,
+                            double* ev_work, double* rhs, void* stream) {
+  return esdg_run(mesh, gas, quad, esdg, q, q_bnd_quad, dissipation, ev_work, rhs,
+                  stream);
+}
+
+mg_status mg_euler_rhs_esdg_noalloc(const mg_mesh* mesh, const mg_gas* gas,
+                                    const mg_quad* quad, const mg_esdg* esdg,
+                                    const double* q, const double* q_bnd_quad,
+                                    int32_t dissipation, double* rhs, void* stream) {
+  mg_status s = esdg_check(mesh, gas, quad, esdg, q, q_bnd_quad, rhs);
+  if (s) return s;
+  const int64_t need = int64_t(2 + mesh->dim + gas->nspecies) * mesh->nelements * mesh->np;
+  if (need > kEsdgScratchDoubles)
+    return fail(MG_ERR_ARG, "state of " + std::to_string(need) +
+                                " values exceeds the allocation-free scratch of " +
+                                std::to_string(kEsdgScratchDoubles) +
+                                " doubles (fields x nodes); use mg_euler_rhs_esdg");
+  double* ev = nullptr;
+  if ((s = device_scratch(&ev))) return s;
+  return esdg_run(mesh, gas, quad, esdg, q, q_bnd_quad, dissipation, ev, rhs, stream);
 }
 
 }  // extern "C"
