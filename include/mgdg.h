@@ -199,6 +199,14 @@ mg_status mg_species_source(const mg_gas* gas, int32_t dim, int64_t n,
                             const double* q, const double* temperature,
                             double* src, void* stream);
 
+/* The same sources added into the species rows of a right-hand side
+ * rhs [F][n] (rows 2 + dim ..), in place: the reacting RHS
+ * euler_operator(...) + get_species_source_terms(...) without a separate
+ * source array or add pass. */
+mg_status mg_species_source_add(const mg_gas* gas, int32_t dim, int64_t n,
+                                const double* q, const double* temperature,
+                                double* rhs, void* stream);
+
 /* Node-local stable timestep into dt [K*np] from the element characteristic
  * lengths lengthscale [K]: h/(|v|+c) (viscous == 0; replaces
  * get_inviscid_timestep, mirgecom/inviscid.py:379-422) or
@@ -346,6 +354,93 @@ mg_status mg_load_transport(const double* packed, int64_t count);
 mg_status mg_transport_vars(const mg_gas* gas, int32_t dim, int64_t n,
                             const double* q, const double* temperature, double* mu,
                             double* mu_b, double* kappa, double* diff, void* stream);
+
+/* {{{ Learned chemistry surrogate (no counterpart in the reference: it
+ * stands in for get_species_source_terms, mirgecom/eos.py:858-875, on the
+ * uiuc_7sp mixture).
+ *
+ * Network 10 -> 256 -> 256 -> 256 -> 7 with GELU; inputs (log T, log p,
+ * Y_1..Y_7, dt) normalised, outputs the normalised change dY_k over a
+ * chemistry substep dt.  BF16 weights and activations on the matrix cores,
+ * FP32 accumulation.  params: FP32 master array of MG_MLP_NPARAM values
+ * (W_l as [K_in][N_out] row-major then b_l; widths padded to 32 -> 256 ->
+ * 256 -> 256 -> 32); wbf16: the BF16 copies of the weights, MG_MLP_NBF16
+ * values, kept in step by mg_mlp_adam / mg_mlp_sync_bf16. */
+#define MG_MLP_NPARAM 148256
+#define MG_MLP_NBF16 294912
+
+typedef struct mg_chem_norm {
+  double in_mean[10], in_scale[10];   /* x = (raw - mean) / scale            */
+  double out_mean[7], out_scale[7];   /* dY_k = out_k * scale_k + mean_k     */
+  double lo[10], hi[10];              /* gate: raw features inside [lo, hi]   */
+  double check_tol;                   /* gate: sampled |S - S_exact| bound in
+                                         output scales (times rho / dt)       */
+  int32_t sample_period, sample_offset; /* gate: nodes g with g % period ==
+                                         offset get the exact-rate check (0:
+                                         no sampled check)                    */
+} mg_chem_norm;
+
+/* Constant-pressure adiabatic reactors (the 0-D reactor of
+ * test/test_chemistry.py:51-110, integrated with classical RK4 at step dt, T
+ * from the conserved enthalpy by Newton after every stage): n reactors from
+ * t0 [n], p [n], y0 [7][n].  t_ign [n] receives the first time T exceeds
+ * t0 + t_rise (-1 if not within nmax steps).  With nsamp > 0, states
+ * [nsamp][n][10] = (T, p, Y_1..7, dt) are recorded at equal step spacing
+ * over min(2 t_ign, nmax dt) (nmax dt without ignition), and dy [nsamp][n][7]
+ * the change of Y over the following step. */
+mg_status mg_reactor_trajectories(int64_t n, const double* t0, const double* p,
+                                  const double* y0, double dt, int64_t nmax,
+                                  double t_rise, int32_t nsamp, double* t_ign,
+                                  double* states, double* dy, void* stream);
+
+/* Change of Y over dt (nsub RK4 steps at constant pressure and enthalpy)
+ * from states [n][10] = (T, p, Y_1..7, ...): dy [n][7]. */
+mg_status mg_chem_substep(int64_t n, const double* states, double dt, int32_t nsub,
+                          double* dy, void* stream);
+
+/* Network inputs from states [n][10]: x [n][32] BF16 (raw uint16 bits);
+ * with dy [n][7] given, the normalised targets [n][32] FP32. */
+mg_status mg_chem_features(int64_t n, const double* states, const double* dy,
+                           const mg_chem_norm* norm, uint16_t* x, float* target,
+                           void* stream);
+
+/* Training.  batch: a positive multiple of 256.  work: a caller-allocated
+ * device workspace of mg_mlp_workspace_bytes(batch) bytes that carries the
+ * activations from forward to backward and the gradient from backward to
+ * adam.  forward writes the network outputs (normalised, [batch][32] FP32)
+ * to out if out != NULL; backward takes the loss L = sum_{k<7} (out - target)^2
+ * / (7 batch), writes its per-block partial sums to loss_part
+ * ((32 batch + 255) / 256 floats; may be NULL) and forms dL/dparams; adam
+ * (Kingma & Ba, bias-corrected, step from 1) updates params, m, v and the
+ * BF16 copies, and copies the gradient to grad_out if != NULL. */
+int64_t mg_mlp_param_count(void);
+int64_t mg_mlp_bf16_count(void);
+int64_t mg_mlp_workspace_bytes(int64_t batch);
+mg_status mg_mlp_forward(int64_t batch, const uint16_t* x, const float* params,
+                         const uint16_t* wbf16, void* work, float* out, void* stream);
+mg_status mg_mlp_backward(int64_t batch, const uint16_t* x, const float* target,
+                          const float* params, const uint16_t* wbf16, void* work,
+                          float* loss_part, void* stream);
+mg_status mg_mlp_adam(float* params, float* m, float* v, const void* work, int64_t batch,
+                      int32_t step, float lr, float beta1, float beta2, float eps,
+                      uint16_t* wbf16, float* grad_out, void* stream);
+/* Re-emit the BF16 copies from params (after the host set params). */
+mg_status mg_mlp_sync_bf16(const float* params, uint16_t* wbf16, void* stream);
+
+/* Species sources from the surrogate for n nodes (n a multiple of np, the
+ * nodes of whole elements in order): S_k = rho dY_k / dt.  Gate, per
+ * element: a node whose raw features leave [lo, hi], or a sampled node whose
+ * S misses the exact rates by more than check_tol output scales, sends the
+ * whole element to the exact rates (mg_species_source's).  add != 0: out is
+ * a right-hand side [F][n] whose species rows receive += S; else out is
+ * [7][n].  fallback_count (optional, device int) is incremented by the
+ * number of elements that took the exact rates. */
+mg_status mg_chem_surrogate_source(const mg_gas* gas, int32_t dim, int32_t np, int64_t n,
+                                   const double* q, const double* temperature, double dt,
+                                   const float* params, const uint16_t* wbf16,
+                                   const mg_chem_norm* norm, int32_t add, double* out,
+                                   int32_t* fallback_count, void* stream);
+/* }}} */
 
 /* Message of the last failed call on this thread. */
 const char* mg_last_error(void);

@@ -69,6 +69,18 @@ class MgEsdg(ctypes.Structure):
     ]
 
 
+class MgChemNorm(ctypes.Structure):
+    """``mg_chem_norm`` (include/mgdg.h): surrogate normalisation and gate."""
+    _fields_ = [("in_mean", ctypes.c_double * 10),
+                ("in_scale", ctypes.c_double * 10),
+                ("out_mean", ctypes.c_double * 7),
+                ("out_scale", ctypes.c_double * 7),
+                ("lo", ctypes.c_double * 10), ("hi", ctypes.c_double * 10),
+                ("check_tol", ctypes.c_double),
+                ("sample_period", ctypes.c_int32),
+                ("sample_offset", ctypes.c_int32)]
+
+
 # exported symbol -> (restype, argtypes)
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -104,6 +116,8 @@ SIGNATURES = {
                                       _P, _I32, _P, _P]),
     "mg_species_source": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,
                                  _P, _P]),
+    "mg_species_source_add": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,
+                                     _P, _P]),
     "mg_local_timestep": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  _P, _P, _P, _I32, _P, _P]),
     "mg_rk_axpy": (_I32, [_I64, _P, _P, _D, _P, _P, _D, _P]),
@@ -124,6 +138,24 @@ SIGNATURES = {
     "mg_load_transport": (_I32, [ctypes.POINTER(ctypes.c_double), _I64]),
     "mg_transport_vars": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P, _P,
                                  _P, _P, _P, _P]),
+    "mg_reactor_trajectories": (_I32, [_I64, _P, _P, _P, _D, _I64, _D, _I32, _P,
+                                       _P, _P, _P]),
+    "mg_chem_substep": (_I32, [_I64, _P, _D, _I32, _P, _P]),
+    "mg_chem_features": (_I32, [_I64, _P, _P, ctypes.POINTER(MgChemNorm), _P, _P,
+                                _P]),
+    "mg_mlp_param_count": (_I64, []),
+    "mg_mlp_bf16_count": (_I64, []),
+    "mg_mlp_workspace_bytes": (_I64, [_I64]),
+    "mg_mlp_forward": (_I32, [_I64, _P, _P, _P, _P, _P, _P]),
+    "mg_mlp_backward": (_I32, [_I64, _P, _P, _P, _P, _P, _P, _P]),
+    "mg_mlp_adam": (_I32, [_P, _P, _P, _P, _I64, _I32, ctypes.c_float,
+                           ctypes.c_float, ctypes.c_float, ctypes.c_float, _P, _P,
+                           _P]),
+    "mg_mlp_sync_bf16": (_I32, [_P, _P, _P]),
+    "mg_chem_surrogate_source": (_I32, [ctypes.POINTER(MgGas), _I32, _I32, _I64,
+                                        _P, _P, _D, _P, _P,
+                                        ctypes.POINTER(MgChemNorm), _I32, _P, _P,
+                                        _P]),
     "mg_mesh_workspace_bytes": (_I64, [ctypes.POINTER(MgMesh)]),
     "mg_mesh_prepare": (_I32, [ctypes.POINTER(MgMesh), _P, _P]),
     "mg_last_error": (ctypes.c_char_p, []),

@@ -346,26 +346,48 @@ mg_status mg_mixture_temperature(const mg_gas* gas, int32_t dim, int64_t n,
   return check_launch("mix_temperature_kernel");
 }
 
-mg_status mg_species_source(const mg_gas* gas, int32_t dim, int64_t n,
-                            const double* q, const double* temperature,
-                            double* src, void* stream) {
+}  // extern "C"
+
+namespace {
+
+mg_status species_source_launch(const mg_gas* gas, int32_t dim, int64_t n,
+                                const double* q, const double* temperature, double* out,
+                                bool add, void* stream) {
   mg_status s;
   if ((s = check_gas(gas, false))) return s;
   if (gas->eos_kind != 1) return fail(MG_ERR_ARG, "mixture EOS required");
   if ((s = ensure_mech())) return s;
-  if (!q || !temperature || !src || n < 0) return fail(MG_ERR_ARG, "bad arguments");
+  if (!q || !temperature || !out || n < 0) return fail(MG_ERR_ARG, "bad arguments");
+  if (dim != 2 && dim != 3) return fail(MG_ERR_ARG, "dim");
   if (n == 0) return MG_OK;
   const unsigned nblk = unsigned((n + mgdg::kBlock - 1) / mgdg::kBlock);
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if (dim == 2)
-    hipLaunchKernelGGL(mgdg::species_source_kernel<2>, dim3(nblk), dim3(mgdg::kBlock), 0,
-                       st, n, q, temperature, src);
-  else if (dim == 3)
-    hipLaunchKernelGGL(mgdg::species_source_kernel<3>, dim3(nblk), dim3(mgdg::kBlock), 0,
-                       st, n, q, temperature, src);
-  else
-    return fail(MG_ERR_ARG, "dim");
+#define MG_SRC(D, A)                                                                 \
+  hipLaunchKernelGGL((mgdg::species_source_kernel<D, A>), dim3(nblk), dim3(mgdg::kBlock), \
+                     0, st, n, q, temperature, out);
+  if (dim == 2) {
+    if (add) MG_SRC(2, true) else MG_SRC(2, false)
+  } else {
+    if (add) MG_SRC(3, true) else MG_SRC(3, false)
+  }
+#undef MG_SRC
   return check_launch("species_source_kernel");
+}
+
+}  // namespace
+
+extern "C" {
+
+mg_status mg_species_source(const mg_gas* gas, int32_t dim, int64_t n,
+                            const double* q, const double* temperature,
+                            double* src, void* stream) {
+  return species_source_launch(gas, dim, n, q, temperature, src, false, stream);
+}
+
+mg_status mg_species_source_add(const mg_gas* gas, int32_t dim, int64_t n,
+                                const double* q, const double* temperature,
+                                double* rhs, void* stream) {
+  return species_source_launch(gas, dim, n, q, temperature, rhs, true, stream);
 }
 
 mg_status mg_local_timestep(const mg_mesh* mesh, const mg_gas* gas, const double* q,

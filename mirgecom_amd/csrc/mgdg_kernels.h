@@ -2063,44 +2063,28 @@ __global__ __launch_bounds__(kBlock) void mix_temperature_kernel(
   tout[g] = mix_temperature(e, tseed[g], y, niter);
 }
 
-template <int DIM>
+// src [7][n] = W_k omega_k, or with ADD rhs species rows += W_k omega_k
+// (rhs [F][n], the species rows start at 2 + DIM).
+template <int DIM, bool ADD>
 __global__ __launch_bounds__(kBlock) void species_source_kernel(
     int64_t n, const double* __restrict__ q, const double* __restrict__ temp,
     double* __restrict__ src) {
   const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (g >= n) return;
   const double rho = q[g];
-  const double t = temp[g];
-  double conc[MIX_NSP];
+  double y[MIX_NSP], s[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) y[k] = q[(2 + DIM + k) * n + g] / rho;
+  mix_species_source(rho, temp[g], y, s);
 #pragma unroll
   for (int k = 0; k < MIX_NSP; ++k) {
-    const double c = c_mech.inv_wt[k] * rho * (q[(2 + DIM + k) * n + g] / rho);
-    conc[k] = c < 0.0 ? 0.0 : c;
+    if constexpr (ADD) {
+      double* r = src + (2 + DIM + k) * n + g;
+      *r = *r + s[k];
+    } else {
+      src[k * n + g] = s[k];
+    }
   }
-  const double logt = log(t);
-  const double inv_t = 1.0 / t;
-  double kf[3];
-#pragma unroll
-  for (int r = 0; r < 3; ++r)
-    kf[r] = exp(c_mech.log_a[r] + c_mech.b[r] * logt - c_mech.ea_r[r] * inv_t);
-  // reverse of R2 (CO + 0.5 O2 <=> CO2): 1/Kc = exp(dG/RT + 0.5 ln(p0/RT))
-  double g_rt[MIX_NSP];
-#pragma unroll
-  for (int k = 0; k < MIX_NSP; ++k) {
-    double cp_r, h_rt;
-    nasa7_cp_h(k, t, inv_t, cp_r, h_rt);
-    g_rt[k] = h_rt - nasa7_s_r(k, t, logt);
-  }
-  const double c0 = log(c_mech.one_atm / (c_mech.gas_constant * t));
-  const double log_rkc = g_rt[2] - (g_rt[3] + 0.5 * g_rt[1]) + 0.5 * c0;
-  const double sq_o2 = sqrt(conc[1]);
-  const double r1 = kf[0] * sqrt(conc[0]) * pow(conc[1], 0.65);
-  const double r2 = kf[1] * (conc[3] * sq_o2 - exp(log_rkc) * conc[2]);
-  const double r3 = kf[2] * pow(conc[5], 0.75) * sq_o2;
-  const double w[MIX_NSP] = {-r1, -r1 - 0.5 * r2 - 0.5 * r3, r2, 2 * r1 - r2, r3,
-                             2 * r1 - r3, 0.0};
-#pragma unroll
-  for (int k = 0; k < MIX_NSP; ++k) src[k * n + g] = c_mech.wt[k] * w[k];
 }
 
 // Node-local stable timestep (inviscid.py:379-422, viscous.py:547-607):

@@ -164,6 +164,70 @@ __device__ __forceinline__ double mix_temperature(double e_in, double t_seed,
   return t;
 }
 
+// Species mass sources W_k omega_k of the uiuc_7sp mechanism at density rho,
+// temperature t and mass fractions y (mirgecom/eos.py:858-875 with the
+// concentrations of mirgecom/thermochemistry.py:87-94, zero level 0):
+//   R1  C2H4 + O2 => 2 CO + 2 H2      r1 = kf1 [C2H4]^0.5 [O2]^0.65
+//   R2  CO + 0.5 O2 <=> CO2           r2 = kf2 ([CO][O2]^0.5 - [CO2] / Kc2)
+//   R3  H2 + 0.5 O2 => H2O            r3 = kf3 [H2]^0.75 [O2]^0.5
+// with kf = exp(log A + b ln T - Ea/(R T)) (SI) and
+// 1/Kc2 = exp(dG/RT + 0.5 ln(p0/RT)) (uiuc_7sp.yaml:146-154).
+__device__ __forceinline__ void mix_species_source(double rho, double t, const double* y,
+                                                   double* src) {
+  double conc[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    const double c = c_mech.inv_wt[k] * rho * y[k];
+    conc[k] = c < 0.0 ? 0.0 : c;
+  }
+  const double logt = log(t);
+  const double inv_t = 1.0 / t;
+  double kf[3];
+#pragma unroll
+  for (int r = 0; r < 3; ++r)
+    kf[r] = exp(c_mech.log_a[r] + c_mech.b[r] * logt - c_mech.ea_r[r] * inv_t);
+  double g_rt[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    double cp_r, h_rt;
+    nasa7_cp_h(k, t, inv_t, cp_r, h_rt);
+    g_rt[k] = h_rt - nasa7_s_r(k, t, logt);
+  }
+  const double c0 = log(c_mech.one_atm / (c_mech.gas_constant * t));
+  const double log_rkc = g_rt[2] - (g_rt[3] + 0.5 * g_rt[1]) + 0.5 * c0;
+  const double sq_o2 = sqrt(conc[1]);
+  const double r1 = kf[0] * sqrt(conc[0]) * pow(conc[1], 0.65);
+  const double r2 = kf[1] * (conc[3] * sq_o2 - exp(log_rkc) * conc[2]);
+  const double r3 = kf[2] * pow(conc[5], 0.75) * sq_o2;
+  const double w[MIX_NSP] = {-r1, -r1 - 0.5 * r2 - 0.5 * r3, r2, 2 * r1 - r2, r3,
+                             2 * r1 - r3, 0.0};
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) src[k] = c_mech.wt[k] * w[k];
+}
+
+// Mixture enthalpy and cp (mass basis) at T, and the specific gas constant.
+__device__ __forceinline__ void mix_h_cp(double t, const double* y, double& h,
+                                         double& cp) {
+  double sh = 0.0, sc = 0.0;
+  const double inv_t = 1.0 / t;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    double cp_r, h_rt;
+    nasa7_cp_h(k, t, inv_t, cp_r, h_rt);
+    sh += c_mech.inv_wt[k] * h_rt * y[k];
+    sc += c_mech.inv_wt[k] * cp_r * y[k];
+  }
+  h = c_mech.gas_constant * t * sh;
+  cp = c_mech.gas_constant * sc;
+}
+
+__device__ __forceinline__ double mix_gas_const(const double* y) {
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) s += c_mech.inv_wt[k] * y[k];
+  return c_mech.gas_constant * s;
+}
+
 // Pure-species transport fits (degree-4 polynomials in ln T, lowest power
 // first), derived on the host from the mechanism's Lennard-Jones parameters:
 //   sqrt(mu_k / sqrt(T)),  lambda_k / sqrt(T),  p D_kj / T^1.5

@@ -202,3 +202,21 @@ class PyrometheusMixture:
                   _lib.ptr(out[2 + cv.dim]), _lib.stream_handle())
         return ConservedVars(cv.dim, out)
 
+    def add_species_source_terms(self, cv, temperature, rhs):
+        """``rhs + get_species_source_terms(cv, temperature)`` formed in place
+        in ``rhs`` (a ``ConservedVars`` shaped like ``cv``) by one kernel
+        (mg_species_source_add); returns ``rhs``."""
+        from mirgecom_amd import _lib
+        q = cv.q.contiguous()
+        t = temperature.contiguous()
+        r = rhs.q
+        if (r.shape != q.shape or r.dtype != torch.float64 or not r.is_contiguous()
+                or r.device != q.device):
+            raise ValueError("rhs must be a contiguous float64 tensor shaped like "
+                             "the state, on its device")
+        _lib.ensure_mechanism()
+        gas = _lib.MgGas(eos_kind=1, nspecies=7)
+        _lib.call("mg_species_source_add", gas, cv.dim, cv.mass.numel(),
+                  _lib.ptr(q), _lib.ptr(t), _lib.ptr(r), _lib.stream_handle())
+        return rhs
+
