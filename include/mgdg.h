@@ -442,6 +442,70 @@ mg_status mg_chem_surrogate_source(const mg_gas* gas, int32_t dim, int32_t np, i
                                    int32_t* fallback_count, void* stream);
 /* }}} */
 
+/* {{{ Batched ensemble of independent cases (no counterpart in the
+ * reference, which runs one driver per case: mirgecom/steppers.py:350-449).
+ *
+ * Every case is a periodic 3-D uiuc_7sp mixture with a non-mixture-averaged
+ * transport (its own mesh, order 1..4 and gas parameters, e.g. a Reynolds
+ * number through mu), evaluated with the mixed-precision Navier-Stokes path
+ * (mg_ns_flux_mixed / mg_ns_div_mixed with the Rusanov flux) after the
+ * Newton temperature from t_seed; then the species sources are added into
+ * rhs: the learned surrogate (mg_chem_surrogate_source) when mlp_params is
+ * given, else the exact rates.  Results equal those of the single-case entry
+ * points bit for bit. */
+typedef struct mg_ens_case {
+  mg_mesh mesh;            /* prepared (mg_mesh_prepare), nbnd_nodes == 0   */
+  mg_gas gas;
+  const double* q;         /* [F][N] state to evaluate                       */
+  const double* t_seed;    /* [N] Newton seed                                */
+  double* temperature;     /* [N] Newton result                              */
+  float* fnet;             /* [F*3][N] FP32 nodal net flux (work)           */
+  double* rhs;             /* [F][N] right-hand side (output)                */
+} mg_ens_case;
+
+/* fused != 0: one launch per phase for all orders (the order is selected per
+ * work item); else one per order.  persistent != 0: workgroups stride over
+ * the work items, wg_per_cu[p] of them per compute unit for order p's
+ * launches ([0]: the fused launch); else one workgroup per item. */
+typedef struct mg_ens_tuning {
+  int32_t fused;
+  int32_t persistent;
+  int32_t wg_per_cu[5];
+} mg_ens_tuning;
+
+/* Host handle of a prepared ensemble (filled by mg_ensemble_prepare; the
+ * tables and item lists live in the caller's device buffer dev). */
+typedef struct mg_ens_plan {
+  void* dev;
+  int32_t ncases, fused, niter, cus, surrogate, persistent;
+  int64_t grad_begin[5], grad_count[5], div_begin[5], div_count[5];
+  int64_t pt_count, src_count;
+  int32_t grid_grad[5], grid_div[5], grid_pt, grid_src;
+  int64_t off_grad_tab, off_div_tab, off_pt_tab, off_src_tab;
+  int64_t off_grad_items, off_div_items, off_pt_items, off_src_items;
+} mg_ens_plan;
+
+/* Device bytes of the plan of these cases (-1 on invalid cases). */
+int64_t mg_ensemble_plan_bytes(int32_t ncases, const mg_ens_case* cases);
+
+/* Build the per-case descriptors and work items into dev (synchronises the
+ * stream once; not capturable).  niter: Newton iterations; dt: the chemistry
+ * substep of the surrogate; fallback_count (optional device int) counts the
+ * elements the surrogate's gate sends to the exact rates; tuning may be
+ * NULL (fused, one workgroup per item). */
+mg_status mg_ensemble_prepare(int32_t ncases, const mg_ens_case* cases, int32_t niter,
+                              double dt, const float* mlp_params,
+                              const uint16_t* mlp_wbf16, const mg_chem_norm* norm,
+                              int32_t* fallback_count, const mg_ens_tuning* tuning,
+                              void* dev, int64_t dev_bytes, mg_ens_plan* plan,
+                              void* stream);
+
+/* One right-hand side of every case of the plan: four launches (Newton,
+ * gradients + net flux, divergence, sources) with the fused tuning, 2 + 2x4
+ * otherwise.  Capturable in a HIP graph. */
+mg_status mg_ensemble_ns_rhs(const mg_ens_plan* plan, void* stream);
+/* }}} */
+
 /* Message of the last failed call on this thread. */
 const char* mg_last_error(void);
 

@@ -81,6 +81,35 @@ class MgChemNorm(ctypes.Structure):
                 ("sample_offset", ctypes.c_int32)]
 
 
+class MgEnsCase(ctypes.Structure):
+    """``mg_ens_case`` (include/mgdg.h): one case of a batched ensemble."""
+    _fields_ = [("mesh", MgMesh), ("gas", MgGas), ("q", ctypes.c_void_p),
+                ("t_seed", ctypes.c_void_p), ("temperature", ctypes.c_void_p),
+                ("fnet", ctypes.c_void_p), ("rhs", ctypes.c_void_p)]
+
+
+class MgEnsTuning(ctypes.Structure):
+    _fields_ = [("fused", ctypes.c_int32), ("persistent", ctypes.c_int32),
+                ("wg_per_cu", ctypes.c_int32 * 5)]
+
+
+class MgEnsPlan(ctypes.Structure):
+    _I5 = ctypes.c_int64 * 5
+    _fields_ = [("dev", ctypes.c_void_p), ("ncases", ctypes.c_int32),
+                ("fused", ctypes.c_int32), ("niter", ctypes.c_int32),
+                ("cus", ctypes.c_int32), ("surrogate", ctypes.c_int32),
+                ("persistent", ctypes.c_int32),
+                ("grad_begin", _I5), ("grad_count", _I5), ("div_begin", _I5),
+                ("div_count", _I5), ("pt_count", ctypes.c_int64),
+                ("src_count", ctypes.c_int64),
+                ("grid_grad", ctypes.c_int32 * 5), ("grid_div", ctypes.c_int32 * 5),
+                ("grid_pt", ctypes.c_int32), ("grid_src", ctypes.c_int32),
+                ("off_grad_tab", ctypes.c_int64), ("off_div_tab", ctypes.c_int64),
+                ("off_pt_tab", ctypes.c_int64), ("off_src_tab", ctypes.c_int64),
+                ("off_grad_items", ctypes.c_int64), ("off_div_items", ctypes.c_int64),
+                ("off_pt_items", ctypes.c_int64), ("off_src_items", ctypes.c_int64)]
+
+
 # exported symbol -> (restype, argtypes)
 _P = ctypes.c_void_p
 _I32 = ctypes.c_int32
@@ -156,6 +185,12 @@ SIGNATURES = {
                                         _P, _P, _D, _P, _P,
                                         ctypes.POINTER(MgChemNorm), _I32, _P, _P,
                                         _P]),
+    "mg_ensemble_plan_bytes": (_I64, [_I32, ctypes.POINTER(MgEnsCase)]),
+    "mg_ensemble_prepare": (_I32, [_I32, ctypes.POINTER(MgEnsCase), _I32, _D, _P, _P,
+                                   ctypes.POINTER(MgChemNorm), _P,
+                                   ctypes.POINTER(MgEnsTuning), _P, _I64,
+                                   ctypes.POINTER(MgEnsPlan), _P]),
+    "mg_ensemble_ns_rhs": (_I32, [ctypes.POINTER(MgEnsPlan), _P]),
     "mg_mesh_workspace_bytes": (_I64, [ctypes.POINTER(MgMesh)]),
     "mg_mesh_prepare": (_I32, [ctypes.POINTER(MgMesh), _P, _P]),
     "mg_last_error": (ctypes.c_char_p, []),

@@ -386,9 +386,8 @@ struct RhsCfg {
 // HLL selects the HLL face flux (compiled separately: its two full face
 // states cost registers that the Rusanov / central path does not need).
 template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
-__global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
-                                                     const double* __restrict__ ws,
-                                                     double* __restrict__ out) {
+__device__ __forceinline__ void rhs_body(const KArgs& a, const double* __restrict__ ws,
+                                         double* __restrict__ out, int64_t blk) {
   using E = Elem<DIM, P>;
   using O = Ops<DIM, P>;
   using C = RhsCfg<DIM, P, NSP, MIX, MODE, R>;
@@ -409,7 +408,7 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
-  const int64_t e0 = xcd_block() * EPB;
+  const int64_t e0 = blk * EPB;
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   const bool inv = MODE == kEuler || a.inviscid_on;
@@ -761,6 +760,13 @@ __global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
   write_staged<NP>(stage, NFLD, EPB, SP, e0, a.K, out);
 }
 
+template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
+__global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
+                                                     const double* __restrict__ ws,
+                                                     double* __restrict__ out) {
+  rhs_body<DIM, P, NSP, MIX, MODE, R, HLL>(a, ws, out, xcd_block());
+}
+
 // }}}
 
 // {{{ gradient (and the fused nodal net flux)
@@ -813,9 +819,8 @@ __device__ __forceinline__ double node_temperature(const double* qv,
 }
 
 template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
-__global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
-                                                      const double* __restrict__ ws,
-                                                      double* __restrict__ out) {
+__device__ __forceinline__ void grad_body(const KArgs& a, const double* __restrict__ ws,
+                                          double* __restrict__ out, int64_t blk) {
   using E = Elem<DIM, P>;
   using O = Ops<DIM, P>;
   using C = GradCfg<DIM, P, NSP, MIX, R>;
@@ -834,7 +839,7 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
   const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
-  const int64_t e0 = xcd_block() * EPB;
+  const int64_t e0 = blk * EPB;
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
@@ -1184,6 +1189,13 @@ __global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
       }
     }
   }
+}
+
+template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
+__global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
+                                                      const double* __restrict__ ws,
+                                                      double* __restrict__ out) {
+  grad_body<DIM, P, NSP, MIX, FLUX, R>(a, ws, out, xcd_block());
 }
 
 // }}}
