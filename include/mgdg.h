@@ -193,6 +193,16 @@ mg_status mg_mixture_temperature(const mg_gas* gas, int32_t dim, int64_t n,
                                  const double* q, const double* t_seed,
                                  int32_t niter, double* t_out, void* stream);
 
+/* The same Newton iteration run to a tolerance instead of a fixed count:
+ * per node until an update |dT| <= tol (K) or max_iter updates; iters
+ * (optional, [n]) receives the updates taken, -1 where the last one still
+ * exceeded tol.  No counterpart in the reference, which iterates a fixed
+ * number of times (mirgecom/thermochemistry.py:149-167). */
+mg_status mg_mixture_temperature_tol(const mg_gas* gas, int32_t dim, int64_t n,
+                                     const double* q, const double* t_seed, double tol,
+                                     int32_t max_iter, double* t_out, int32_t* iters,
+                                     void* stream);
+
 /* Species mass sources W_k * omega_k into src [nspecies][n]
  * (replaces get_species_source_terms, mirgecom/eos.py:858-875). */
 mg_status mg_species_source(const mg_gas* gas, int32_t dim, int64_t n,
@@ -479,6 +489,7 @@ typedef struct mg_ens_plan {
   void* dev;
   int32_t ncases, fused, niter, cus, surrogate, persistent;
   int64_t grad_begin[5], grad_count[5], div_begin[5], div_count[5];
+  int32_t grad_seg_begin[5], grad_seg_count[5], div_seg_begin[5], div_seg_count[5];
   int64_t pt_count, src_count;
   int32_t grid_grad[5], grid_div[5], grid_pt, grid_src;
   int64_t off_grad_tab, off_div_tab, off_pt_tab, off_src_tab;

@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libmgdg.so")
+# MGDG_LIB overrides the library path (a build elsewhere, e.g. for A/B timing)
+LIB_PATH = os.environ.get("MGDG_LIB") or os.path.join(_HERE, "libmgdg.so")
 
 MG_FLUX_RUSANOV = 0
 MG_FLUX_HLL = 1
@@ -100,7 +101,12 @@ class MgEnsPlan(ctypes.Structure):
                 ("cus", ctypes.c_int32), ("surrogate", ctypes.c_int32),
                 ("persistent", ctypes.c_int32),
                 ("grad_begin", _I5), ("grad_count", _I5), ("div_begin", _I5),
-                ("div_count", _I5), ("pt_count", ctypes.c_int64),
+                ("div_count", _I5),
+                ("grad_seg_begin", ctypes.c_int32 * 5),
+                ("grad_seg_count", ctypes.c_int32 * 5),
+                ("div_seg_begin", ctypes.c_int32 * 5),
+                ("div_seg_count", ctypes.c_int32 * 5),
+                ("pt_count", ctypes.c_int64),
                 ("src_count", ctypes.c_int64),
                 ("grid_grad", ctypes.c_int32 * 5), ("grid_div", ctypes.c_int32 * 5),
                 ("grid_pt", ctypes.c_int32), ("grid_src", ctypes.c_int32),
@@ -143,6 +149,8 @@ SIGNATURES = {
                                _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "mg_mixture_temperature": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P,
                                       _P, _I32, _P, _P]),
+    "mg_mixture_temperature_tol": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,
+                                          _D, _I32, _P, _P, _P]),
     "mg_species_source": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,
                                  _P, _P]),
     "mg_species_source_add": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P, _P,

@@ -26,7 +26,11 @@
 
 namespace mgens {
 
-using mgdg::KArgs;
+// Descriptors are read from memory: their pointers are global-qualified
+// (mgdg::KArgsT<1>), so that the bodies access memory with global_load /
+// global_store as the single-case kernels do (flat accesses made the
+// divergence phase 35-80 % slower).
+using KArgs = mgdg::KArgsT<1>;
 using mgdg::kBlock;
 
 constexpr int kDim = 3, kNsp = 7, kMix = 1;
@@ -38,21 +42,43 @@ struct Item {
   int64_t blk;
 };
 
-struct PointCase {
-  const double* q;
-  const double* tseed;
-  double* tout;
-  int64_t n;
+// A case's run of work items: items first .. first + (its block count) - 1
+// of a launch belong to case icase (order p).  A workgroup finds its item's
+// case by binary search over the launch's few segments (wave-uniform, scalar
+// loads of a small table that stays in the caches), instead of loading a
+// per-item record that would be a cache miss for every workgroup.
+struct Seg {
+  int32_t icase;
+  int32_t p;
+  int64_t first;
 };
 
-// The item is the same for the whole workgroup: moving its fields to scalar
-// registers lets the case descriptor be read with scalar loads (a per-lane
-// copy of the ~400-byte descriptor would cost a hundred VGPRs).
-__device__ __forceinline__ Item uniform_item(const Item& v) {
+struct PointCase {
+  __attribute__((address_space(1))) const double* q;
+  __attribute__((address_space(1))) const double* tseed;
+  __attribute__((address_space(1))) double* tout;
+  int64_t n;
+};
+using InferArgs = mgchem::InferArgsT<1>;
+
+// Item `it` of a launch (the same for the whole workgroup): the case
+// descriptor is then read with scalar loads.
+__device__ __forceinline__ Item find_item(const Seg* __restrict__ segs, int nseg,
+                                          int64_t it) {
+  int lo = 0, hi = nseg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (segs[mid].first <= it)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  lo = __builtin_amdgcn_readfirstlane(lo);
+  const Seg sg = segs[lo];
   Item w;
-  w.icase = __builtin_amdgcn_readfirstlane(v.icase);
-  w.p = __builtin_amdgcn_readfirstlane(v.p);
-  const uint64_t b = uint64_t(v.blk);
+  w.icase = __builtin_amdgcn_readfirstlane(sg.icase);
+  w.p = __builtin_amdgcn_readfirstlane(sg.p);
+  const uint64_t b = uint64_t(it - sg.first);
   w.blk = int64_t((uint64_t(uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(b >> 32)))) << 32) |
                   uint32_t(__builtin_amdgcn_readfirstlane(uint32_t(b))));
   return w;
@@ -66,9 +92,11 @@ using DCfg = mgdg::RhsCfg<kDim, P, kNsp, kMix, mgdg::kFlux, R>;
 template <int PH, int P>
 __device__ __forceinline__ void run_body(const KArgs& a, int64_t blk) {
   if constexpr (PH == 0)
-    mgdg::grad_body<kDim, P, kNsp, kMix, true, R>(a, a.ops, a.out, blk);
+    mgdg::grad_body<kDim, P, kNsp, kMix, true, R>(a, (const double*)a.ops, (double*)a.out,
+                                                  blk);
   else
-    mgdg::rhs_body<kDim, P, kNsp, kMix, mgdg::kFlux, R, false>(a, a.ops, a.out, blk);
+    mgdg::rhs_body<kDim, P, kNsp, kMix, mgdg::kFlux, R, false>(a, (const double*)a.ops,
+                                                               (double*)a.out, blk);
 }
 
 template <int PH, int PSEL>
@@ -97,7 +125,8 @@ __device__ __forceinline__ void run_item(const KArgs* __restrict__ cases, const 
 // kernels, persistently by giving XCD x the x-th eighth of the item list.
 template <int PH, int PSEL, bool PERSIST>
 __global__ __launch_bounds__(kBlock) void ens_dg_kernel(
-    const KArgs* __restrict__ cases, const Item* __restrict__ items, int64_t nitems) {
+    const KArgs* __restrict__ cases, const Seg* __restrict__ segs, int nseg, int64_t base,
+    int64_t nitems) {
   if constexpr (PERSIST) {
     const int64_t G = gridDim.x, b = blockIdx.x;
     int64_t lo = b, hi = nitems, step = G;
@@ -108,20 +137,20 @@ __global__ __launch_bounds__(kBlock) void ens_dg_kernel(
       lo = nitems * x / mgdg::kXcds + i;
     }
     for (int64_t it = lo; it < hi; it += step) {
-      run_item<PH, PSEL>(cases, uniform_item(items[it]));
+      run_item<PH, PSEL>(cases, find_item(segs, nseg, base + it));
       __syncthreads();   // the next item reuses the LDS
     }
   } else {
     const int64_t it = mgdg::xcd_block();
-    if (it < nitems) run_item<PH, PSEL>(cases, uniform_item(items[it]));
+    if (it < nitems) run_item<PH, PSEL>(cases, find_item(segs, nseg, base + it));
   }
 }
 
 __global__ __launch_bounds__(kBlock) void ens_temperature_kernel(
-    const PointCase* __restrict__ cases, const Item* __restrict__ items, int64_t nitems,
-    int niter) {
+    const PointCase* __restrict__ cases, const Seg* __restrict__ segs, int nseg,
+    int64_t nitems, int niter) {
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const Item w = uniform_item(items[it]);
+    const Item w = find_item(segs, nseg, it);
     const PointCase c = cases[w.icase];
     const int64_t g = w.blk * kBlock + threadIdx.x;
     if (g < c.n) {
@@ -140,11 +169,11 @@ __global__ __launch_bounds__(kBlock) void ens_temperature_kernel(
 }
 
 __global__ __launch_bounds__(kBlock) void ens_exact_source_kernel(
-    const mgchem::InferArgs* __restrict__ cases, const Item* __restrict__ items,
+    const InferArgs* __restrict__ cases, const Seg* __restrict__ segs, int nseg,
     int64_t nitems) {
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const Item w = uniform_item(items[it]);
-    const mgchem::InferArgs& c = cases[w.icase];
+    const Item w = find_item(segs, nseg, it);
+    const InferArgs& c = cases[w.icase];
     const int64_t g = w.blk * kBlock + threadIdx.x;
     if (g < c.n) {
       const int64_t n = c.n;
@@ -154,19 +183,16 @@ __global__ __launch_bounds__(kBlock) void ens_exact_source_kernel(
       for (int k = 0; k < kNsp; ++k) y[k] = c.q[(2 + kDim + k) * n + g] / rho;
       mgdg::mix_species_source(rho, c.temp[g], y, s);
 #pragma unroll
-      for (int k = 0; k < kNsp; ++k) {
-        double* r = c.out + (2 + kDim + k) * n + g;
-        *r = *r + s[k];
-      }
+      for (int k = 0; k < kNsp; ++k) c.out[(2 + kDim + k) * n + g] += s[k];
     }
   }
 }
 
 __global__ __launch_bounds__(256, 2) void ens_infer_kernel(
-    const mgchem::InferArgs* __restrict__ cases, const Item* __restrict__ items,
+    const InferArgs* __restrict__ cases, const Seg* __restrict__ segs, int nseg,
     int64_t nitems) {
   for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
-    const Item w = uniform_item(items[it]);
+    const Item w = find_item(segs, nseg, it);
     mgchem::infer_body(cases[w.icase], w.blk);
     __syncthreads();
   }
@@ -177,7 +203,17 @@ __global__ __launch_bounds__(256, 2) void ens_infer_kernel(
 namespace {
 
 using mgens::Item;
+using mgens::Seg;
 constexpr int kBlock = mgdg::kBlock;
+
+// Host image of mgens::PointCase (generic pointers, same layout)
+struct PointHost {
+  const double* q;
+  const double* tseed;
+  double* tout;
+  int64_t n;
+};
+static_assert(sizeof(PointHost) == sizeof(mgens::PointCase), "point descriptor layout");
 
 template <int P>
 int grad_epb() { return mgens::GCfg<P>::EPB; }
@@ -217,21 +253,18 @@ struct Layout {
 
 int64_t align256(int64_t x) { return (x + 255) / 256 * 256; }
 
-int64_t count_items(int32_t ncases, const mg_ens_case* cases, int phase) {
-  int64_t n = 0;
-  for (int c = 0; c < ncases; ++c) {
-    const mg_mesh& m = cases[c].mesh;
-    if (phase < 2) {
-      const int epb = epb_of(phase, m.order);
-      n += (m.nelements + epb - 1) / epb;
-    } else if (phase == 2) {
-      n += (m.nelements * m.np + kBlock - 1) / kBlock;
-    } else {
-      const int rows = (mgchem::kRows / m.np) * m.np;
-      n += (m.nelements * m.np + rows - 1) / rows;
-    }
+// Work items of one case in a phase: element blocks (0: gradients, 1:
+// divergence), node blocks (2: Newton), row blocks (3: sources; with the
+// surrogate, whole elements of at most 128 nodes, else node blocks).
+int64_t case_items(const mg_mesh& m, int phase, bool surrogate) {
+  if (phase < 2) {
+    const int epb = epb_of(phase, m.order);
+    return (m.nelements + epb - 1) / epb;
   }
-  return n;
+  const int64_t n = m.nelements * m.np;
+  const int64_t per =
+      (phase == 3 && surrogate) ? (mgchem::kRows / m.np) * m.np : int64_t(kBlock);
+  return (n + per - 1) / per;
 }
 
 Layout layout(int32_t ncases, const mg_ens_case* cases) {
@@ -246,13 +279,13 @@ Layout layout(int32_t ncases, const mg_ens_case* cases) {
   L.src_tab = off;
   off = align256(off + int64_t(sizeof(mgchem::InferArgs)) * ncases);
   L.grad_items = off;
-  off = align256(off + int64_t(sizeof(Item)) * count_items(ncases, cases, 0));
+  off = align256(off + int64_t(sizeof(Seg)) * ncases);
   L.div_items = off;
-  off = align256(off + int64_t(sizeof(Item)) * count_items(ncases, cases, 1));
+  off = align256(off + int64_t(sizeof(Seg)) * ncases);
   L.pt_items = off;
-  off = align256(off + int64_t(sizeof(Item)) * count_items(ncases, cases, 2));
+  off = align256(off + int64_t(sizeof(Seg)) * ncases);
   L.src_items = off;
-  off = align256(off + int64_t(sizeof(Item)) * count_items(ncases, cases, 3));
+  off = align256(off + int64_t(sizeof(Seg)) * ncases);
   L.bytes = off;
   return L;
 }
@@ -293,17 +326,18 @@ mg_status dg_attrs() {
 }
 
 mg_status launch_dg(int phase, int psel, bool persist, int grid, size_t lds,
-                    const mgdg::KArgs* tab, const Item* items, int64_t n, hipStream_t st) {
+                    const mgens::KArgs* tab, const Seg* segs, int nseg, int64_t base,
+                    int64_t n, hipStream_t st) {
   if (n == 0) return MG_OK;
   if (!persist && n > 0x7fffffff) return fail(MG_ERR_ARG, "too many items for one launch");
 #define MG_ENS(PH, PS)                                                                    \
   if (phase == PH && psel == PS) {                                                        \
     if (persist)                                                                          \
       hipLaunchKernelGGL((mgens::ens_dg_kernel<PH, PS, true>), dim3(unsigned(grid)),       \
-                         dim3(kBlock), lds, st, tab, items, n);                           \
+                         dim3(kBlock), lds, st, tab, segs, nseg, base, n);                \
     else                                                                                  \
       hipLaunchKernelGGL((mgens::ens_dg_kernel<PH, PS, false>), dim3(unsigned(n)),         \
-                         dim3(kBlock), lds, st, tab, items, n);                           \
+                         dim3(kBlock), lds, st, tab, segs, nseg, base, n);                \
     return check_launch("ens_dg_kernel");                                                 \
   }
   MG_ENS(0, 0) MG_ENS(0, 1) MG_ENS(0, 2) MG_ENS(0, 3) MG_ENS(0, 4)
@@ -357,9 +391,11 @@ mg_status mg_ensemble_prepare(int32_t ncases, const mg_ens_case* cases, int32_t 
   if (dev_bytes < L.bytes) return fail(MG_ERR_ARG, "plan buffer too small");
   if ((s = dg_attrs())) return s;
   std::vector<char> host(size_t(L.bytes), 0);
+  // host tables: the generic argument blocks, byte-identical to the device
+  // side's global-qualified ones
   auto* gt = reinterpret_cast<mgdg::KArgs*>(host.data() + L.grad_tab);
   auto* dt_ = reinterpret_cast<mgdg::KArgs*>(host.data() + L.div_tab);
-  auto* pt = reinterpret_cast<mgens::PointCase*>(host.data() + L.pt_tab);
+  auto* pt = reinterpret_cast<PointHost*>(host.data() + L.pt_tab);
   auto* st_ = reinterpret_cast<mgchem::InferArgs*>(host.data() + L.src_tab);
   for (int c = 0; c < ncases; ++c) {
     const mg_ens_case& k = cases[c];
@@ -376,7 +412,7 @@ mg_status mg_ensemble_prepare(int32_t ncases, const mg_ens_case* cases, int32_t 
     d.fnet32 = k.fnet;
     d.out = k.rhs;
     dt_[c] = d;
-    pt[c] = mgens::PointCase{k.q, k.t_seed, k.temperature, k.mesh.nelements * k.mesh.np};
+    pt[c] = PointHost{k.q, k.t_seed, k.temperature, k.mesh.nelements * k.mesh.np};
     mgchem::InferArgs ia{};
     ia.n = k.mesh.nelements * k.mesh.np;
     ia.np = k.mesh.np;
@@ -392,45 +428,42 @@ mg_status mg_ensemble_prepare(int32_t ncases, const mg_ens_case* cases, int32_t 
     if (surrogate) ia.nm = to_norm(norm);
     st_[c] = ia;
   }
-  // items: by decreasing order, case by case
+  // segments: by decreasing order, case by case
   std::memset(plan, 0, sizeof(*plan));
   const bool fused = !tuning || tuning->fused;
   const int cus = device_cus();
   for (int phase = 0; phase < 2; ++phase) {
-    Item* it = reinterpret_cast<Item*>(host.data() + (phase ? L.div_items : L.grad_items));
+    Seg* sg = reinterpret_cast<Seg*>(host.data() + (phase ? L.div_items : L.grad_items));
+    int32_t* sb = phase ? plan->div_seg_begin : plan->grad_seg_begin;
+    int32_t* sn = phase ? plan->div_seg_count : plan->grad_seg_count;
+    int64_t* ib = phase ? plan->div_begin : plan->grad_begin;
+    int64_t* in = phase ? plan->div_count : plan->grad_count;
+    int nseg = 0;
     int64_t pos = 0;
     for (int p = 4; p >= 1; --p) {
-      const int64_t begin = pos;
+      sb[p] = nseg;
+      ib[p] = pos;
       for (int c = 0; c < ncases; ++c) {
         if (cases[c].mesh.order != p) continue;
-        const int epb = epb_of(phase, p);
-        const int64_t nb = (cases[c].mesh.nelements + epb - 1) / epb;
-        for (int64_t b = 0; b < nb; ++b) it[pos++] = Item{c, p, b};
+        sg[nseg++] = Seg{c, p, pos};
+        pos += case_items(cases[c].mesh, phase, surrogate);
       }
-      (phase ? plan->div_begin : plan->grad_begin)[p] = begin;
-      (phase ? plan->div_count : plan->grad_count)[p] = pos - begin;
+      sn[p] = nseg - sb[p];
+      in[p] = pos - ib[p];
     }
-    (phase ? plan->div_begin : plan->grad_begin)[0] = 0;
-    (phase ? plan->div_count : plan->grad_count)[0] = pos;
+    sb[0] = 0;
+    sn[0] = nseg;
+    ib[0] = 0;
+    in[0] = pos;
   }
-  {
-    Item* it = reinterpret_cast<Item*>(host.data() + L.pt_items);
+  for (int phase = 2; phase < 4; ++phase) {
+    Seg* sg = reinterpret_cast<Seg*>(host.data() + (phase == 2 ? L.pt_items : L.src_items));
     int64_t pos = 0;
     for (int c = 0; c < ncases; ++c) {
-      const int64_t nb = (cases[c].mesh.nelements * cases[c].mesh.np + kBlock - 1) / kBlock;
-      for (int64_t b = 0; b < nb; ++b) it[pos++] = Item{c, 0, b};
+      sg[c] = Seg{c, 0, pos};
+      pos += case_items(cases[c].mesh, phase, surrogate);
     }
-    plan->pt_count = pos;
-    Item* is = reinterpret_cast<Item*>(host.data() + L.src_items);
-    pos = 0;
-    for (int c = 0; c < ncases; ++c) {
-      const int64_t n = cases[c].mesh.nelements * cases[c].mesh.np;
-      const int64_t per = surrogate ? (mgchem::kRows / cases[c].mesh.np) * cases[c].mesh.np
-                                    : kBlock;
-      const int64_t nb = (n + per - 1) / per;
-      for (int64_t b = 0; b < nb; ++b) is[pos++] = Item{c, 0, b};
-    }
-    plan->src_count = pos;
+    (phase == 2 ? plan->pt_count : plan->src_count) = pos;
   }
   plan->dev = dev;
   plan->ncases = ncases;
@@ -467,43 +500,47 @@ mg_status mg_ensemble_ns_rhs(const mg_ens_plan* plan, void* stream) {
   if ((s = dg_attrs())) return s;
   hipStream_t st = static_cast<hipStream_t>(stream);
   char* base = static_cast<char*>(plan->dev);
-  const auto* gt = reinterpret_cast<const mgdg::KArgs*>(base + plan->off_grad_tab);
-  const auto* dtab = reinterpret_cast<const mgdg::KArgs*>(base + plan->off_div_tab);
+  const auto* gt = reinterpret_cast<const mgens::KArgs*>(base + plan->off_grad_tab);
+  const auto* dtab = reinterpret_cast<const mgens::KArgs*>(base + plan->off_div_tab);
   const auto* pt = reinterpret_cast<const mgens::PointCase*>(base + plan->off_pt_tab);
-  const auto* stab = reinterpret_cast<const mgchem::InferArgs*>(base + plan->off_src_tab);
-  const auto* gi = reinterpret_cast<const Item*>(base + plan->off_grad_items);
-  const auto* di = reinterpret_cast<const Item*>(base + plan->off_div_items);
-  const auto* pi = reinterpret_cast<const Item*>(base + plan->off_pt_items);
-  const auto* si = reinterpret_cast<const Item*>(base + plan->off_src_items);
+  const auto* stab = reinterpret_cast<const mgens::InferArgs*>(base + plan->off_src_tab);
+  const auto* gi = reinterpret_cast<const Seg*>(base + plan->off_grad_items);
+  const auto* di = reinterpret_cast<const Seg*>(base + plan->off_div_items);
+  const auto* pi = reinterpret_cast<const Seg*>(base + plan->off_pt_items);
+  const auto* si = reinterpret_cast<const Seg*>(base + plan->off_src_items);
   if (plan->pt_count > 0) {
     hipLaunchKernelGGL(mgens::ens_temperature_kernel, dim3(unsigned(plan->grid_pt)),
-                       dim3(kBlock), 0, st, pt, pi, plan->pt_count, plan->niter);
+                       dim3(kBlock), 0, st, pt, pi, plan->ncases, plan->pt_count,
+                       plan->niter);
     if ((s = check_launch("ens_temperature_kernel"))) return s;
   }
   for (int phase = 0; phase < 2; ++phase) {
     const auto* tab = phase ? dtab : gt;
-    const auto* items = phase ? di : gi;
+    const Seg* segs = phase ? di : gi;
+    const int32_t* sb = phase ? plan->div_seg_begin : plan->grad_seg_begin;
+    const int32_t* sn = phase ? plan->div_seg_count : plan->grad_seg_count;
     const int64_t* begin = phase ? plan->div_begin : plan->grad_begin;
     const int64_t* count = phase ? plan->div_count : plan->grad_count;
     const int32_t* grid = phase ? plan->grid_div : plan->grid_grad;
     if (plan->fused) {
       if ((s = launch_dg(phase, 0, plan->persistent, grid[0], lds_of(phase, 0), tab,
-                         items + begin[0], count[0], st)))
+                         segs + sb[0], sn[0], begin[0], count[0], st)))
         return s;
     } else {
       for (int p = 4; p >= 1; --p)
-        if ((s = launch_dg(phase, p, plan->persistent, grid[p], lds_of(phase, p), tab,
-                           items + begin[p], count[p], st)))
+        if (sn[p] > 0 &&
+            (s = launch_dg(phase, p, plan->persistent, grid[p], lds_of(phase, p), tab,
+                           segs + sb[p], sn[p], begin[p], count[p], st)))
           return s;
     }
   }
   if (plan->src_count > 0) {
     if (plan->surrogate)
       hipLaunchKernelGGL(mgens::ens_infer_kernel, dim3(unsigned(plan->grid_src)), dim3(256),
-                         mgchem::kInferLds, st, stab, si, plan->src_count);
+                         mgchem::kInferLds, st, stab, si, plan->ncases, plan->src_count);
     else
       hipLaunchKernelGGL(mgens::ens_exact_source_kernel, dim3(unsigned(plan->grid_src)),
-                         dim3(kBlock), 0, st, stab, si, plan->src_count);
+                         dim3(kBlock), 0, st, stab, si, plan->ncases, plan->src_count);
     if ((s = check_launch("ensemble source"))) return s;
   }
   return MG_OK;

@@ -31,12 +31,28 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ float gelu(float z) {
-  return 0.5f * z * (1.0f + erff(z * 0.70710678118654752f));
+// GELU in its tanh form (Hendrycks & Gimpel 2016, eq. for the fast
+// approximation): gelu(z) = z (1 + tanh u) / 2, u = sqrt(2/pi) (z + 0.044715 z^3).
+// With E = exp(2u) and r = 1 / (1 + E): tanh u = 1 - 2r, so
+//   gelu(z)  = z (1 - r)
+//   gelu'(z) = (1 - r) + 2 z r (1 - r) sqrt(2/pi) (1 + 3 * 0.044715 z^2)
+// evaluated with the hardware exp2 and reciprocal (v_exp_f32, v_rcp_f32),
+// about 1e-7 relative: far below the BF16 rounding of the activations.  The
+// erf form costs several times the VALU work and, at 128 activations per
+// lane and layer, outweighed the matrix-core time of the fused inference.
+constexpr float kGeluC0 = 0.7978845608028654f;   // sqrt(2 / pi)
+constexpr float kGeluC1 = 0.044715f;
+constexpr float kLog2e2 = 2.0f * 1.4426950408889634f;
+
+__device__ __forceinline__ float gelu_r(float z) {
+  const float u = kGeluC0 * (z + kGeluC1 * z * z * z);
+  const float e = __builtin_amdgcn_exp2f(kLog2e2 * u);
+  return __builtin_amdgcn_rcpf(1.0f + e);
 }
+__device__ __forceinline__ float gelu(float z) { return z * (1.0f - gelu_r(z)); }
 __device__ __forceinline__ float gelu_grad(float z) {
-  return 0.5f * (1.0f + erff(z * 0.70710678118654752f)) +
-         z * 0.39894228040143268f * expf(-0.5f * z * z);
+  const float r = gelu_r(z);
+  return (1.0f - r) + 2.0f * z * r * (1.0f - r) * kGeluC0 * (1.0f + 3.0f * kGeluC1 * z * z);
 }
 
 // {{{ constant-pressure reactor (FP64, one reactor per thread)
@@ -455,22 +471,92 @@ static __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ pr
 constexpr int kRows = 128;
 constexpr int kLdsStride = kH + 8;
 
-struct InferArgs {
+// One hidden layer of the fused inference, in place on the activations
+// act [128][kLdsStride] (BF16, LDS): wave w owns output columns
+// [64 w, 64 w + 64) of every row, 8 x 4 accumulator fragments; the weight
+// fragments of the next K step are fetched while the current one's 32
+// matrix-core products run (two register sets).
+template <int KDIM>
+__device__ __forceinline__ void hidden_layer(__bf16* act, const __bf16* __restrict__ wt,
+                                             const float* __restrict__ bias, int w,
+                                             int lane) {
+  constexpr int NKS = KDIM / 32;
+  const int r16 = lane & 15, kq = (lane >> 4) * 8;
+  const int c0 = w * 64;
+  floatx4 acc[8][4];
+#pragma unroll
+  for (int i = 0; i < 8; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto load_b = [&](bf16x8* fb, int k0) {
+#pragma unroll
+    for (int j = 0; j < 4; ++j)
+      fb[j] = *reinterpret_cast<const bf16x8*>(wt + int64_t(c0 + 16 * j + r16) * KDIM + k0 +
+                                               kq);
+  };
+  auto mma = [&](const bf16x8* fb, int k0) {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const bf16x8 fa =
+          *reinterpret_cast<const bf16x8*>(act + (16 * i + r16) * kLdsStride + k0 + kq);
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[j], acc[i][j], 0, 0, 0);
+    }
+  };
+  bf16x8 fb0[4], fb1[4];
+  load_b(fb0, 0);
+  if constexpr (NKS == 1) {
+    mma(fb0, 0);
+  } else {
+#pragma unroll 1
+    for (int ks = 0; ks < NKS; ks += 2) {
+      load_b(fb1, (ks + 1) * 32);
+      mma(fb0, ks * 32);
+      if (ks + 2 < NKS) load_b(fb0, (ks + 2) * 32);
+      mma(fb1, (ks + 1) * 32);
+    }
+  }
+  __syncthreads();   // every wave has read this layer's input
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int col = c0 + 16 * j + r16;
+    const float b = bias[col];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int row0 = 16 * i + (lane >> 4) * 4;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        act[(row0 + e) * kLdsStride + col] = __bf16(gelu(acc[i][j][e] + b));
+    }
+  }
+  __syncthreads();
+}
+
+// AS 1: global-qualified pointers, for argument blocks read from memory
+// (see mgdg::KArgsT)
+#define MG_P(T) typename mgdg::ArgPtr<AS, T>::type
+template <int AS>
+struct InferArgsT {
   int64_t n;          // nodes
   int np;             // nodes per element
   int dim;
   double dt;
-  const double* q;    // [F][n]
-  const double* temp; // [n]
-  double* out;        // add: rhs [F][n] (species rows += source); else src [7][n]
+  MG_P(const double) q;    // [F][n]
+  MG_P(const double) temp; // [n]
+  MG_P(double) out;        // add: rhs [F][n] (species rows += source); else src [7][n]
   int add;
-  const float* prm;
-  const __bf16* wbf;
-  int* fallback;      // optional counter of elements sent to the exact rates
+  MG_P(const float) prm;
+  MG_P(const __bf16) wbf;
+  MG_P(int) fallback;      // optional counter of elements sent to the exact rates
   Norm nm;
 };
+#undef MG_P
+using InferArgs = InferArgsT<0>;
+static_assert(sizeof(InferArgsT<0>) == sizeof(InferArgsT<1>), "argument block layouts");
 
-__device__ __forceinline__ void infer_body(const InferArgs& a, int64_t blk) {
+template <typename A>
+__device__ __forceinline__ void infer_body(const A& a, int64_t blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __bf16* act = reinterpret_cast<__bf16*>(smem_raw);                  // [128][264]
   float* outp = reinterpret_cast<float*>(act + kRows * kLdsStride);    // [128][32]
@@ -514,53 +600,18 @@ __device__ __forceinline__ void infer_body(const InferArgs& a, int64_t blk) {
 
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   // hidden layers
-#pragma unroll 1
-  for (int layer = 0; layer < 3; ++layer) {
-    const int kdim = layer == 0 ? kIn : kH;
-    const __bf16* wt = a.wbf + (layer == 0 ? kBfW1 : layer == 1 ? kBfW2 : kBfW3);
-    const float* bias = a.prm + (layer == 0 ? kOffB1 : layer == 1 ? kOffB2 : kOffB3);
-    floatx4 acc[8][4];
-#pragma unroll
-    for (int i = 0; i < 8; ++i)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int c0 = w * 64;
-#pragma unroll 1
-    for (int k0 = 0; k0 < kdim; k0 += 32) {
-      bf16x8 fb[4];
-#pragma unroll
-      for (int j = 0; j < 4; ++j)
-        fb[j] = *reinterpret_cast<const bf16x8*>(wt + int64_t(c0 + 16 * j + r16) * kdim +
-                                                 k0 + kq);
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const bf16x8 fa =
-            *reinterpret_cast<const bf16x8*>(act + (16 * i + r16) * kLdsStride + k0 + kq);
-#pragma unroll
-        for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[j], acc[i][j], 0, 0, 0);
-      }
-    }
-    __syncthreads();   // every wave has read this layer's input
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int col = c0 + 16 * j + r16;
-      const float b = bias[col];
-#pragma unroll
-      for (int i = 0; i < 8; ++i) {
-        const int row0 = 16 * i + (lane >> 4) * 4;
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          act[(row0 + e) * kLdsStride + col] = __bf16(gelu(acc[i][j][e] + b));
-      }
-    }
-    __syncthreads();
-  }
+  // generic views of the weights (the compiler keeps the global address
+  // space the argument block states)
+  const __bf16* wbf = (const __bf16*)a.wbf;
+  const float* prm = (const float*)a.prm;
+  hidden_layer<kIn>(act, wbf + kBfW1, prm + kOffB1, w, lane);
+  hidden_layer<kH>(act, wbf + kBfW2, prm + kOffB2, w, lane);
+  hidden_layer<kH>(act, wbf + kBfW3, prm + kOffB3, w, lane);
   // output layer (32 columns: 2 fragments): wave w owns row fragments
   // 2w, 2w + 1
   {
-    const __bf16* wt = a.wbf + kBfW4;
-    const float* bias = a.prm + kOffB4;
+    const __bf16* wt = wbf + kBfW4;
+    const float* bias = prm + kOffB4;
     floatx4 acc[2][2];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
@@ -624,8 +675,7 @@ __device__ __forceinline__ void infer_body(const InferArgs& a, int64_t blk) {
 #pragma unroll
     for (int k = 0; k < MIX_NSP; ++k) {
       if (a.add) {
-        double* r = a.out + (sp0 + k) * n + g;
-        *r = *r + src[k];
+        a.out[(sp0 + k) * n + g] += src[k];
       } else {
         a.out[k * n + g] = src[k];
       }
@@ -635,7 +685,7 @@ __device__ __forceinline__ void infer_body(const InferArgs& a, int64_t blk) {
     int c = 0;
     for (int e = 0; e < epb; ++e)
       if (node0 + int64_t(e) * a.np < n) c += eflag[e];
-    if (c) atomicAdd(a.fallback, c);
+    if (c) atomicAdd((int*)a.fallback, c);
   }
 }
 

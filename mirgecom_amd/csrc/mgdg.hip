@@ -346,6 +346,29 @@ mg_status mg_mixture_temperature(const mg_gas* gas, int32_t dim, int64_t n,
   return check_launch("mix_temperature_kernel");
 }
 
+mg_status mg_mixture_temperature_tol(const mg_gas* gas, int32_t dim, int64_t n,
+                                     const double* q, const double* t_seed, double tol,
+                                     int32_t max_iter, double* t_out, int32_t* iters,
+                                     void* stream) {
+  mg_status s;
+  if ((s = check_gas(gas, false))) return s;
+  if (gas->eos_kind != 1) return fail(MG_ERR_ARG, "mixture EOS required");
+  if ((s = ensure_mech())) return s;
+  if (!q || !t_seed || !t_out || n < 0 || max_iter < 1 || !(tol > 0.0))
+    return fail(MG_ERR_ARG, "bad arguments (tol > 0, max_iter >= 1)");
+  if (dim != 2 && dim != 3) return fail(MG_ERR_ARG, "dim");
+  if (n == 0) return MG_OK;
+  const unsigned nblk = unsigned((n + mgdg::kBlock - 1) / mgdg::kBlock);
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  if (dim == 2)
+    hipLaunchKernelGGL(mgdg::mix_temperature_tol_kernel<2>, dim3(nblk), dim3(mgdg::kBlock), 0,
+                       st, n, q, t_seed, tol, int(max_iter), t_out, iters);
+  else
+    hipLaunchKernelGGL(mgdg::mix_temperature_tol_kernel<3>, dim3(nblk), dim3(mgdg::kBlock), 0,
+                       st, n, q, t_seed, tol, int(max_iter), t_out, iters);
+  return check_launch("mix_temperature_tol_kernel");
+}
+
 }  // extern "C"
 
 namespace {

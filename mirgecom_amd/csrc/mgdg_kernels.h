@@ -21,7 +21,7 @@
 #include <cstring>
 #include <string>
 
-#include "../../include/mgdg.h"
+#include "mgdg.h"
 #include "mgdg_physics.h"
 
 namespace mgdg {
@@ -38,35 +38,51 @@ struct Elem {
   static constexpr int MAT = DIM * NP * NP + NFN * NP;  // doubles of Dw + LIFT
 };
 
-struct KArgs {
-  const double* q;
-  const double* temp;
-  const double* grad;
-  const double* qb;
-  const double* tb;
-  double* out;
-  const double* rx;
-  const double* normals;
-  const double* fscale;
-  const int32_t* vmap_p;
-  const double* dw;
-  const double* lift;
-  const int32_t* fmask;
-  const double* ops;   // packed operators (mg_mesh_prepare)
-  const double* fnet;  // nodal net flux [F*DIM][N] (kFlux)
-  const double* fnb;   // exterior net normal flux [F][nbnd] (kFlux)
-  const float* fnet32; // nodal net flux, FP32 (mixed precision)
-  float* fnet32_out;   // FP32 nodal net flux output (mixed precision)
-  double* grad_out;    // optional gradient output of the fused flux kernel
-  double* fnb_out;     // exterior net normal flux output of the flux kernel
+// Argument block of the element-batched kernels.  AS 0: generic pointers
+// (a kernel argument: the compiler treats its pointers as global memory);
+// AS 1: global-qualified pointers, for blocks read from memory (ensemble
+// descriptors), whose generic pointers would otherwise be accessed with
+// flat instructions.  Both have the same layout.
+template <int AS, typename T>
+struct ArgPtr {
+  using type = T*;
+};
+template <typename T>
+struct ArgPtr<1, T> {
+  using type = __attribute__((address_space(1))) T*;
+};
+
+#define MG_P(T) typename ArgPtr<AS, T>::type
+template <int AS>
+struct KArgsT {
+  MG_P(const double) q;
+  MG_P(const double) temp;
+  MG_P(const double) grad;
+  MG_P(const double) qb;
+  MG_P(const double) tb;
+  MG_P(double) out;
+  MG_P(const double) rx;
+  MG_P(const double) normals;
+  MG_P(const double) fscale;
+  MG_P(const int32_t) vmap_p;
+  MG_P(const double) dw;
+  MG_P(const double) lift;
+  MG_P(const int32_t) fmask;
+  MG_P(const double) ops;   // packed operators (mg_mesh_prepare)
+  MG_P(const double) fnet;  // nodal net flux [F*DIM][N] (kFlux)
+  MG_P(const double) fnb;   // exterior net normal flux [F][nbnd] (kFlux)
+  MG_P(const float) fnet32; // nodal net flux, FP32 (mixed precision)
+  MG_P(float) fnet32_out;   // FP32 nodal net flux output (mixed precision)
+  MG_P(double) grad_out;    // optional gradient output of the fused flux kernel
+  MG_P(double) fnb_out;     // exterior net normal flux output of the flux kernel
   // Mengaldo boundary data per boundary node (NULL wflag: none)
-  const double* wq;     // CV_bc [F][nbnd]
-  const double* wt;     // T_bc [nbnd]
-  const double* wts;    // temperature of state_bc [nbnd] (mixtures)
-  const int32_t* wflag; // MG_BC_* bits [nbnd]
-  const int32_t* wvol;  // interior volume node of each boundary node [nbnd]
-  const double* wn;     // unit normal [DIM][nbnd]
-  double* wwork;        // boundary gradients / viscous fluxes [(F+1)*DIM][nbnd]
+  MG_P(const double) wq;     // CV_bc [F][nbnd]
+  MG_P(const double) wt;     // T_bc [nbnd]
+  MG_P(const double) wts;    // temperature of state_bc [nbnd] (mixtures)
+  MG_P(const int32_t) wflag; // MG_BC_* bits [nbnd]
+  MG_P(const int32_t) wvol;  // interior volume node of each boundary node [nbnd]
+  MG_P(const double) wn;     // unit normal [DIM][nbnd]
+  MG_P(double) wwork;        // boundary gradients / viscous fluxes [(F+1)*DIM][nbnd]
   int64_t K;
   int64_t nbnd;
   int64_t nnodes;      // K * NP
@@ -74,10 +90,14 @@ struct KArgs {
   int flux_kind;
   int inviscid_on;
 };
+#undef MG_P
+using KArgs = KArgsT<0>;
+static_assert(sizeof(KArgsT<0>) == sizeof(KArgsT<1>), "argument block layouts");
 
-template <int DIM, int P>
+
+template <int DIM, int P, typename A>
 __device__ __forceinline__ void load_matrices(double* s_dw, double* s_lift,
-                                              int* s_fmask, const KArgs& a) {
+                                              int* s_fmask, const A& a) {
   using E = Elem<DIM, P>;
   // s_dw[(r*NP + j)*NP + i] = Dw_r[i][j];  s_lift[m*NP + i] = LIFT[i][m]
   for (int t = threadIdx.x; t < DIM * E::NP * E::NP; t += blockDim.x) {
@@ -95,17 +115,16 @@ __device__ __forceinline__ void load_matrices(double* s_dw, double* s_lift,
   for (int t = threadIdx.x; t < E::NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
 }
 
-template <int DIM, int NSP>
-__device__ __forceinline__ void load_q(Node<DIM, NSP>& s, const double* q,
-                                       int64_t stride, int64_t g) {
+// PT: a pointer to const double, generic or global-qualified
+template <int DIM, int NSP, typename PT>
+__device__ __forceinline__ void load_q(Node<DIM, NSP>& s, PT q, int64_t stride, int64_t g) {
 #pragma unroll
   for (int f = 0; f < Node<DIM, NSP>::NFLD; ++f) s.q[f] = q[f * stride + g];
 }
 
-template <int DIM, int NSP>
-__device__ __forceinline__ void load_grad(double (*gq)[DIM], double* gt,
-                                          const double* grad, int64_t stride,
-                                          int64_t g) {
+template <int DIM, int NSP, typename PT>
+__device__ __forceinline__ void load_grad(double (*gq)[DIM], double* gt, PT grad,
+                                          int64_t stride, int64_t g) {
   constexpr int NFLD = 2 + DIM + NSP;
 #pragma unroll
   for (int f = 0; f < NFLD; ++f)
@@ -257,8 +276,8 @@ constexpr int odd_stride(int n) { return n | 1; }
 // diagonal does not change the DG divergence (the weak divergence of a
 // constant flux cancels its lift exactly), but it removes the large
 // low-Mach background from the values that are rounded to FP32.
-template <int DIM, int NSP, int MIX>
-__device__ __forceinline__ double reference_pressure(const KArgs& a) {
+template <int DIM, int NSP, int MIX, typename A>
+__device__ __forceinline__ double reference_pressure(const A& a) {
   Node<DIM, NSP> s;
   load_q<DIM, NSP>(s, a.q, a.nnodes, 0);
   node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[0] : 0.0);
@@ -270,15 +289,16 @@ __device__ __forceinline__ double reference_pressure(const KArgs& a) {
 // the boundary conditions on the gradients applied to the interior ones.
 constexpr int kBcMengaldo = 1, kBcSlip = 2, kBcImpermeable = 4, kBcAdiabatic = 8;
 
-__device__ __forceinline__ int wall_flags(const KArgs& a, int64_t b) {
+template <typename A>
+__device__ __forceinline__ int wall_flags(const A& a, int64_t b) {
   return a.wflag ? a.wflag[b] : 0;
 }
 
 // Effective exterior viscous normal flux 2 F_V(bc).n - F_V(q-, grad-).n: the
 // face flux averages both sides, which then gives F_V(bc).n.  gq/gt hold the
 // interior gradients on entry and the boundary-condition gradients on exit.
-template <int DIM, int NSP, int MIX>
-__device__ void wall_viscous_flux_n(const KArgs& a, int64_t b, int64_t gm, int fl,
+template <int DIM, int NSP, int MIX, typename A>
+__device__ void wall_viscous_flux_n(const A& a, int64_t b, int64_t gm, int fl,
                                     const double* n, double (*gq)[DIM], double* gt,
                                     double* fv) {
   constexpr int NFLD = 2 + DIM + NSP;
@@ -385,8 +405,8 @@ struct RhsCfg {
 
 // HLL selects the HLL face flux (compiled separately: its two full face
 // states cost registers that the Rusanov / central path does not need).
-template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
-__device__ __forceinline__ void rhs_body(const KArgs& a, const double* __restrict__ ws,
+template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL, typename A>
+__device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ ws,
                                          double* __restrict__ out, int64_t blk) {
   using E = Elem<DIM, P>;
   using O = Ops<DIM, P>;
@@ -628,7 +648,7 @@ __device__ __forceinline__ void rhs_body(const KArgs& a, const double* __restric
       const bool local = nb >= g_lo && nb < g_hi;
       const bool bnd = nb < 0;
       const int64_t b = bnd ? int64_t(-nb - 1) : 0;
-      const double* qsrc = bnd ? a.qb : a.q;
+      const auto qsrc = bnd ? a.qb : a.q;
       const int64_t qstr = bnd ? a.nbnd : N;
       const int64_t qix = bnd ? b : int64_t(nb);
       const int lp = local ? int(nb - g_lo) : 0;
@@ -818,8 +838,8 @@ __device__ __forceinline__ double node_temperature(const double* qv,
   }
 }
 
-template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
-__device__ __forceinline__ void grad_body(const KArgs& a, const double* __restrict__ ws,
+template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R, typename A>
+__device__ __forceinline__ void grad_body(const A& a, const double* __restrict__ ws,
                                           double* __restrict__ out, int64_t blk) {
   using E = Elem<DIM, P>;
   using O = Ops<DIM, P>;
@@ -2073,6 +2093,25 @@ __global__ __launch_bounds__(kBlock) void mix_temperature_kernel(
 #pragma unroll
   for (int k = 0; k < MIX_NSP; ++k) y[k] = q[(2 + DIM + k) * n + g] / rho;
   tout[g] = mix_temperature(e, tseed[g], y, niter);
+}
+
+template <int DIM>
+__global__ __launch_bounds__(kBlock) void mix_temperature_tol_kernel(
+    int64_t n, const double* __restrict__ q, const double* __restrict__ tseed, double tol,
+    int max_iter, double* __restrict__ tout, int32_t* __restrict__ iters) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= n) return;
+  const double rho = q[g];
+  double mm = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) mm += q[(2 + d) * n + g] * q[(2 + d) * n + g];
+  const double e = (q[n + g] - 0.5 * mm / rho) / rho;
+  double y[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) y[k] = q[(2 + DIM + k) * n + g] / rho;
+  int it;
+  tout[g] = mix_temperature_tol(e, tseed[g], y, tol, max_iter, &it);
+  if (iters) iters[g] = it;
 }
 
 // src [7][n] = W_k omega_k, or with ADD rhs species rows += W_k omega_k

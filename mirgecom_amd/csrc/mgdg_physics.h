@@ -164,6 +164,28 @@ __device__ __forceinline__ double mix_temperature(double e_in, double t_seed,
   return t;
 }
 
+// Newton solve to a tolerance: iterate T <- T + (e - e(T))/cv(T) until the
+// update is at most tol (checked after each update) or max_iter updates;
+// *iters receives the number of updates, or -1 when the last update still
+// exceeded tol.  The update is the fixed-iteration solve's (mix_e_cv).
+__device__ __forceinline__ double mix_temperature_tol(double e_in, double t_seed,
+                                                      const double* y, double tol,
+                                                      int max_iter, int* iters) {
+  double t = t_seed;
+  int it = 0;
+  bool done = false;
+  while (!done && it < max_iter) {
+    double e, cv;
+    mix_e_cv(t, y, e, cv);
+    const double dt = (e_in - e) / cv;
+    t = t + dt;
+    ++it;
+    done = fabs(dt) <= tol;
+  }
+  *iters = done ? it : -1;
+  return t;
+}
+
 // Species mass sources W_k omega_k of the uiuc_7sp mechanism at density rho,
 // temperature t and mass fractions y (mirgecom/eos.py:858-875 with the
 // concentrations of mirgecom/thermochemistry.py:87-94, zero level 0):

@@ -195,7 +195,7 @@ class ReactingEnsemble:
         _lib.ensure_mechanism()
         lib = _lib.load()
         tun = _lib.MgEnsTuning()
-        tun.fused = int(self.tuning.get("fused", 1))
+        tun.fused = int(self.tuning.get("fused", 0))
         tun.persistent = int(self.tuning.get("persistent", 0))
         wg = self.tuning.get("wg_per_cu", [4] * 5)
         for p in range(5):
@@ -315,7 +315,7 @@ def load_tuning():
     if os.path.exists(TUNING_PATH):
         with open(TUNING_PATH) as f:
             return json.load(f)["best"]
-    return {"fused": 1, "persistent": 0, "wg_per_cu": [4, 4, 4, 4, 4]}
+    return {"fused": 0, "persistent": 0, "wg_per_cu": [4, 4, 4, 4, 4]}
 
 
 def _time_rhs(ens, reps):

@@ -95,11 +95,18 @@ class PyrometheusMixture:
     species_names = mech.SPECIES
 
     def __init__(self, pyrometheus_mech=None, temperature_guess=300.0,
-                 temperature_niter=5):
+                 temperature_niter=5, temperature_tolerance=None,
+                 temperature_max_iter=50):
         if pyrometheus_mech not in (None, "uiuc_7sp"):
             raise NotImplementedError("only the uiuc_7sp mechanism is built in")
         self._tguess = temperature_guess
         self.temperature_niter = int(temperature_niter)
+        # tolerance mode (not in the reference): iterate each node until its
+        # Newton update is at most temperature_tolerance K
+        self.temperature_tolerance = (None if temperature_tolerance is None
+                                      else float(temperature_tolerance))
+        self.temperature_max_iter = int(temperature_max_iter)
+        self.last_temperature_iterations = None
         self._wt = mech.molecular_weights()
         self._lo, self._hi = mech.nasa7_tables()
 
@@ -178,6 +185,15 @@ class PyrometheusMixture:
         _lib.ensure_mechanism()
         gas = _lib.MgGas(eos_kind=1, nspecies=7)
         n = cv.mass.numel()
+        if self.temperature_tolerance is not None:
+            iters = torch.empty(cv.mass.shape, dtype=torch.int32, device=q.device)
+            _lib.call("mg_mixture_temperature_tol", gas, cv.dim, n, _lib.ptr(q),
+                      _lib.ptr(tseed), self.temperature_tolerance,
+                      self.temperature_max_iter, _lib.ptr(out), _lib.ptr(iters),
+                      _lib.stream_handle())
+            # per-node updates taken (-1: not converged within max_iter)
+            self.last_temperature_iterations = iters
+            return out
         _lib.call("mg_mixture_temperature", gas, cv.dim, n, _lib.ptr(q),
                   _lib.ptr(tseed), self.temperature_niter, _lib.ptr(out),
                   _lib.stream_handle())

@@ -6,8 +6,9 @@ statement of the network that mirgecom_amd/csrc/mg_chem.hip runs in BF16 on
 the matrix cores, used to check its forward pass and its gradients:
 
 * layers 32 -> 256 -> 256 -> 256 -> 32 (10 and 7 real inputs / outputs),
-  ``z_l = a_{l-1} W_l + b_l``, ``a_l = gelu(z_l)`` with the erf form
-  ``gelu(z) = z (1 + erf(z / sqrt 2)) / 2`` on the hidden layers;
+  ``z_l = a_{l-1} W_l + b_l``, ``a_l = gelu(z_l)`` with the tanh form
+  ``gelu(z) = z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) / 2`` on the
+  hidden layers;
 * loss ``L = sum_{k<7} (out_k - target_k)^2 / (7 B)``;
 * reverse mode: ``dz_4 = 2 (out - target) / (7 B)``,
   ``dW_l = a_{l-1}^T dz_l``, ``db_l = sum_b dz_l``,
@@ -26,7 +27,6 @@ import math
 
 import numpy as np
 
-SQRT_HALF = 1.0 / math.sqrt(2.0)
 N_OUT = 7
 
 
@@ -38,18 +38,18 @@ def to_bf16(a):
     return r.view(np.float32).astype(np.float64)
 
 
-def _erf(x):
-    from scipy.special import erf
-    return erf(x)
+C0 = math.sqrt(2.0 / math.pi)
+C1 = 0.044715
 
 
 def gelu(z):
-    return 0.5 * z * (1.0 + _erf(z * SQRT_HALF))
+    """GELU, tanh form: z (1 + tanh(sqrt(2/pi) (z + 0.044715 z^3))) / 2."""
+    return 0.5 * z * (1.0 + np.tanh(C0 * (z + C1 * z**3)))
 
 
 def gelu_grad(z):
-    return 0.5 * (1.0 + _erf(z * SQRT_HALF)) + z * np.exp(-0.5 * z * z) / math.sqrt(
-        2.0 * math.pi)
+    t = np.tanh(C0 * (z + C1 * z**3))
+    return 0.5 * (1.0 + t) + 0.5 * z * (1.0 - t * t) * C0 * (1.0 + 3.0 * C1 * z * z)
 
 
 def _r(a, bf16):

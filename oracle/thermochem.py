@@ -139,6 +139,28 @@ def get_temperature(e, t_guess, y, niter=5):
     return t
 
 
+def get_temperature_tol(e, t_guess, y, tol, max_iter=50):
+    """The same Newton update run per node until |dT| <= tol (tolerance mode
+    of the device solver; the reference iterates a fixed count).  Returns
+    ``(T, iterations)`` with -1 where max_iter updates did not reach tol."""
+    t = np.asarray(t_guess, dtype=np.float64) * np.ones_like(e)
+    iters = np.zeros(t.shape, dtype=np.int64)
+    active = np.ones(t.shape, dtype=bool)
+    for _ in range(max_iter):
+        if not active.any():
+            break
+        ta = t[active]
+        ya = y[:, active]
+        dt = (e[active] - mixture_energy_mass(ta, ya)) / mixture_cv_mass(ta, ya)
+        t[active] = ta + dt
+        iters[active] += 1
+        conv = np.abs(dt) <= tol
+        idx = np.flatnonzero(active)
+        active[idx[conv]] = False
+    iters[active] = -1
+    return t, iters
+
+
 def concentrations(rho, y, zero_level=0.0):
     """mirgecom/thermochemistry.py:87-94"""
     c = rho * y / _w(rho)

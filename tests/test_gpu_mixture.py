@@ -105,3 +105,28 @@ def test_mixture_rhs(viscous, seed_kind):
     err = rel_err(_host(rhs.q), ref)
     print("mixture rhs", viscous, err)
     assert err.max() < 1e-11
+
+
+@pytest.mark.parametrize("seed_kind", ["near", "300K"])
+def test_newton_tolerance_mode(seed_kind):
+    """mg_mixture_temperature_tol against the oracle's tolerance loop:
+    temperatures within 1e-12 relative, and the same per-node update count
+    (tolerance 1e-6 K, the reacting workload's setting)."""
+    from mirgecom_amd.eos import PyrometheusMixture
+    from mirgecom_amd.fluid import ConservedVars
+    from oracle import thermochem as tc
+
+    x = np.random.default_rng(4).uniform(0, 2 * np.pi, size=(3, 3000))
+    q, t = mixture_state(x, seed=8)
+    seed = t + 3.0 if seed_kind == "near" else np.full_like(t, 300.0)
+    eos = PyrometheusMixture(temperature_tolerance=1e-6)
+    tg = _host(eos.temperature(ConservedVars(3, _dev(q)), _dev(seed)))
+    ig = _host(eos.last_temperature_iterations)
+    rho = q[0]
+    y = q[5:] / rho
+    e = (q[1] - 0.5 * (q[2]**2 + q[3]**2 + q[4]**2) / rho) / rho
+    to, io = tc.get_temperature_tol(e, seed, y, 1e-6)
+    assert np.abs(tg - to).max() <= 1e-12 * to.max()
+    assert np.all(ig > 0)
+    assert np.mean(ig == io) > 0.999
+    assert np.abs(tg - t).max() < 1e-6

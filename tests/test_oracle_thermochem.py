@@ -127,3 +127,19 @@ def test_autoignition_bounds():
                           / tc.mixture_cp_mass(np.array(t), y))
     assert 1800.0 < t < 3200.0
     assert y[0] < 1e-3 * x[0]        # fuel consumed
+
+
+def test_newton_tolerance_mode():
+    """Tolerance mode: converged to the tolerance from far seeds, the last
+    update within it; a fixed-count solve with as many iterations agrees."""
+    rho, t, y = _random_state(300, 11, tlo=1000.0, thi=2600.0)
+    e = tc.mixture_energy_mass(t, y)
+    seed = np.full_like(t, 300.0)
+    tt, it = tc.get_temperature_tol(e, seed, y, 1e-6)
+    assert np.all(it > 0)
+    assert np.abs(tt - t).max() < 1e-6
+    nmax = int(it.max())
+    tf = tc.get_temperature(e, seed, y, nmax)
+    assert np.abs(tf - tt).max() < 1e-6
+    _, it2 = tc.get_temperature_tol(e, seed, y, 1e-6, max_iter=1)
+    assert np.all(it2 == -1)
