@@ -47,18 +47,29 @@ typedef struct mg_mesh {
   const double* lift;       /* [np][nfaces*nfp]  M^-1 E                    */
   const int32_t* face_mask; /* [nfaces][nfp] volume node of each face node */
   const double* workspace;  /* packed operators, set by mg_mesh_prepare     */
+  /* Optional (NULL = natural order): elem_rank[e] is element e's position
+   * in a locality-preserving traversal (a permutation of 0..K-1, e.g. a
+   * space-filling curve through the element centroids).  Workgroups then
+   * take their element blocks in that order, so that the blocks one XCD
+   * runs together are mesh neighbours and the face gathers hit its L2.
+   * Results do not depend on it; a rank array that is not a permutation
+   * falls back to the natural order. */
+  const int32_t* elem_rank;
+  const int32_t* block_order; /* visiting-order tables, set by mg_mesh_prepare */
 } mg_mesh;
 
-/* Bytes of the packed-operator workspace of a mesh (-1 if dim/order are
- * unsupported). */
+/* Bytes of the mesh workspace: the packed operators, and the visiting-order
+ * tables when elem_rank is set (-1 if dim/order are unsupported). */
 int64_t mg_mesh_workspace_bytes(const mg_mesh* mesh);
 
 /* Pack weak_diff and lift into the caller-allocated workspace in the layout
  * the contraction phase reads (rows of each wave contiguous, read with
- * wave-uniform loads) and store the pointer in mesh->workspace.  Once per
- * mesh; asynchronous on the stream.  There is no reference counterpart: it
- * is the device-side part of building the discretization
- * (mirgecom/discretization.py:41-83). */
+ * wave-uniform loads) and store the pointer in mesh->workspace; with
+ * elem_rank set, also build the block visiting-order tables (one per
+ * elements-per-workgroup count the kernels use) and store them in
+ * mesh->block_order.  Once per mesh; asynchronous on the stream.  There is
+ * no reference counterpart: it is the device-side part of building the
+ * discretization (mirgecom/discretization.py:41-83). */
 mg_status mg_mesh_prepare(mg_mesh* mesh, double* workspace, void* stream);
 
 /* Gas model: equation of state and transport (mirgecom/gas_model.py:82-97,

@@ -32,6 +32,7 @@ class MgMesh(ctypes.Structure):
         ("weak_diff", ctypes.c_void_p), ("lift", ctypes.c_void_p),
         ("face_mask", ctypes.c_void_p),
         ("workspace", ctypes.c_void_p),
+        ("elem_rank", ctypes.c_void_p), ("block_order", ctypes.c_void_p),
     ]
 
 

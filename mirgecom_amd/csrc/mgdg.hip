@@ -62,7 +62,7 @@ extern "C" {
 
 const char* mg_last_error(void) { return mgdg_shared::err().c_str(); }
 
-int32_t mg_abi_version(void) { return 4; }
+int32_t mg_abi_version(void) { return 5; }
 
 /* Upload the uiuc_7sp mechanism constants (host array of 7*7*2 + 7 + 7 + 7
  * + 2 + 9 doubles, packed as in mgdg::MechDev). */
@@ -101,13 +101,18 @@ mg_status mg_load_transport(const double* packed, int64_t count) {
 
 int64_t mg_mesh_workspace_bytes(const mg_mesh* m) {
   if (!m) return -1;
-  int total = -1;
+  int64_t total = -1;
 #define MG_WS_CASE(D, P) \
-  if (m->dim == D && m->order == P) total = mgdg::Ops<D, P>::BYTES;
+  if (m->dim == D && m->order == P) total = ops_bytes_aligned<D, P>();
   MG_WS_CASE(2, 1) MG_WS_CASE(2, 2) MG_WS_CASE(2, 3) MG_WS_CASE(2, 4)
   MG_WS_CASE(3, 1) MG_WS_CASE(3, 2) MG_WS_CASE(3, 3) MG_WS_CASE(3, 4)
 #undef MG_WS_CASE
-  return total < 0 ? -1 : int64_t(total);
+  if (total < 0) return -1;
+  if (m->elem_rank) {
+    if (m->nelements <= 0) return -1;
+    total += 4 * mgdg::order_table_offset(m->nelements, mgdg::kOrderTables);
+  }
+  return total;
 }
 
 mg_status mg_mesh_prepare(mg_mesh* m, double* workspace, void* stream) {

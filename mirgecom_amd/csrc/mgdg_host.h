@@ -107,6 +107,7 @@ mgdg::KArgs make_args(const mg_mesh* m, const mg_gas* g) {
   a.lift = m->lift;
   a.fmask = m->face_mask;
   a.ops = m->workspace;
+  a.border = m->block_order;
   a.K = m->nelements;
   a.nbnd = m->nbnd_nodes;
   a.nnodes = m->nelements * m->np;
@@ -303,6 +304,11 @@ mg_status ensure_mech() {
 }
 
 template <int D, int P>
+constexpr int64_t ops_bytes_aligned() {
+  return (int64_t(mgdg::Ops<D, P>::BYTES) + 255) / 256 * 256;
+}
+
+template <int D, int P>
 mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
   const int n = mgdg::Ops<D, P>::TOTAL + mgdg::Ops<D, P>::MF;
   auto kern = mgdg::pack_ops_kernel<D, P>;
@@ -311,6 +317,24 @@ mg_status pack_ops(mg_mesh* m, double* ws, hipStream_t st) {
   mg_status s = check_launch("pack_ops_kernel");
   if (s) return s;
   m->workspace = ws;
+  m->block_order = nullptr;
+  if (m->elem_rank) {
+    const int64_t K = m->nelements;
+    int32_t* tab = reinterpret_cast<int32_t*>(reinterpret_cast<char*>(ws) +
+                                              ops_bytes_aligned<D, P>());
+    for (int i = 0; i < mgdg::kOrderTables; ++i) {
+      const int epb = mgdg::kOrderEpb[i];
+      const int64_t nb = (K + epb - 1) / epb;
+      hipLaunchKernelGGL(mgdg::order_fill_kernel, dim3((K + 255) / 256), dim3(256), 0, st, tab,
+                         K);
+      hipLaunchKernelGGL(mgdg::order_mark_kernel, dim3((nb + 255) / 256), dim3(256), 0, st,
+                         m->elem_rank, tab, K, epb);
+      hipLaunchKernelGGL(mgdg::order_compact_kernel, dim3(1), dim3(1024), 0, st, tab,
+                         tab + mgdg::order_table_offset(K, i), K, epb);
+      if ((s = check_launch("order tables"))) return s;
+    }
+    m->block_order = tab;
+  }
   return MG_OK;
 }
 

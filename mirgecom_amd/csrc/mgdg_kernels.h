@@ -69,6 +69,7 @@ struct KArgsT {
   MG_P(const double) lift;
   MG_P(const int32_t) fmask;
   MG_P(const double) ops;   // packed operators (mg_mesh_prepare)
+  MG_P(const int32_t) border; // block visiting-order tables, or NULL
   MG_P(const double) fnet;  // nodal net flux [F*DIM][N] (kFlux)
   MG_P(const double) fnb;   // exterior net normal flux [F][nbnd] (kFlux)
   MG_P(const float) fnet32; // nodal net flux, FP32 (mixed precision)
@@ -266,6 +267,92 @@ __device__ __forceinline__ int64_t xcd_block() {
   return x * q + (x < r ? x : r) + i;
 }
 
+// Block visiting order.  A kernel taking EPB elements per workgroup has
+// ceil(K / EPB) element blocks; with a traversal order of the elements
+// (mg_mesh::elem_rank) the workgroup that xcd_block() numbers b takes the
+// block whose first element is the b-th in that order.  Since xcd_block()
+// hands each XCD a contiguous range of b, each XCD then works through a
+// compact region of the mesh, block after neighbouring block, and the
+// gathers of face neighbours find their lines in its L2 instead of
+// fetching them again.  Layout (int32): K entries of scratch used while
+// building, then one table per EPB of kOrderEpb.
+constexpr int kOrderEpb[] = {4, 5, 6, 7, 8, 9, 10, 12, 16};
+constexpr int kOrderTables = sizeof(kOrderEpb) / sizeof(int);
+
+constexpr int order_slot(int epb) {
+  for (int i = 0; i < kOrderTables; ++i)
+    if (kOrderEpb[i] == epb) return i;
+  return -1;
+}
+
+__host__ __device__ inline int64_t order_table_offset(int64_t K, int slot) {
+  int64_t off = K;
+  for (int i = 0; i < slot; ++i) off += (K + kOrderEpb[i] - 1) / kOrderEpb[i];
+  return off;
+}
+
+template <int EPB, typename A>
+__device__ __forceinline__ int64_t visit_block(const A& a, int64_t blk) {
+  constexpr int slot = order_slot(EPB);
+  if constexpr (slot < 0) {
+    return blk;
+  } else {
+    if (!a.border) return blk;
+    return a.border[order_table_offset(a.K, slot) + blk];
+  }
+}
+
+// Building a table: mark[r] = the block whose first element has rank r
+// (-1 elsewhere), then the marked entries in rank order.
+static __global__ __launch_bounds__(kBlock) void order_fill_kernel(int32_t* mark, int64_t K) {
+  const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < K) mark[i] = -1;
+}
+
+static __global__ __launch_bounds__(kBlock) void order_mark_kernel(const int32_t* rank, int32_t* mark,
+                                                            int64_t K, int epb) {
+  const int64_t nb = (K + epb - 1) / epb;
+  const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nb) return;
+  const int32_t r = rank[b * epb];
+  if (r >= 0 && r < K) mark[r] = int32_t(b);
+}
+
+// One workgroup of 1024: a running compaction of mark into out; a table
+// that does not come out with exactly ceil(K / epb) entries (a rank array
+// that is not a permutation) is replaced by the natural order.
+static __global__ __launch_bounds__(1024) void order_compact_kernel(const int32_t* mark, int32_t* out,
+                                                             int64_t K, int epb) {
+  __shared__ int s_wave[16];
+  __shared__ int64_t s_base;
+  const int64_t nb = (K + epb - 1) / epb;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (threadIdx.x == 0) s_base = 0;
+  __syncthreads();
+  for (int64_t i0 = 0; i0 < K; i0 += 1024) {
+    const int64_t i = i0 + threadIdx.x;
+    const int32_t v = i < K ? mark[i] : -1;
+    const bool keep = v >= 0;
+    const uint64_t bal = __ballot(keep);
+    const int before = __popcll(bal & ((uint64_t(1) << lane) - 1));
+    if (lane == 0) s_wave[w] = __popcll(bal);
+    __syncthreads();
+    int64_t pos = s_base;
+    for (int j = 0; j < w; ++j) pos += s_wave[j];
+    pos += before;
+    if (keep && pos < nb) out[pos] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int64_t t = s_base;
+      for (int j = 0; j < 16; ++j) t += s_wave[j];
+      s_base = t;
+    }
+    __syncthreads();
+  }
+  if (s_base != nb)
+    for (int64_t b = threadIdx.x; b < nb; b += 1024) out[b] = int32_t(b);
+}
+
 // LDS row strides are padded to an odd number of elements: rows written by
 // consecutive lanes then fall into different banks (an even stride of 60-64
 // doubles maps whole waves onto a few banks).
@@ -396,8 +483,13 @@ struct RhsCfg {
   static constexpr int SWS = EPB * E::NP;          // wavespeed |v| + c per node
   static constexpr int STAGE = NFLD * SP;
   static_assert(STAGE <= SF, "staging aliases the flux buffer");
-  static constexpr size_t LDS = sizeof(R) * (SF + SFACE) +
-                                sizeof(double) * (SGEO + SREF + SWS) +
+  // FP64 with few fields: the face fluxes are held in registers until the
+  // volume contraction has read the nodal fluxes, then written over them,
+  // which cuts the LDS per workgroup by the face buffer (the occupancy of
+  // these kernels is set by LDS otherwise).
+  static constexpr bool ALIAS = sizeof(R) == 8 && NFLD <= 6;
+  static constexpr int SFF = ALIAS ? (SF > SFACE ? SF : SFACE) : SF + SFACE;
+  static constexpr size_t LDS = sizeof(R) * SFF + sizeof(double) * (SGEO + SREF + SWS) +
                                 sizeof(int) * E::NFN;
   static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
@@ -423,12 +515,16 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   double* s_ref = s_geo + C::SGEO;                        // [EPB][NODEF] (mixed)
   double* s_ws = s_ref + C::SREF;                         // [EPB][NP]
   R* s_f = reinterpret_cast<R*>(s_ws + C::SWS);           // [EPB][NP][NFLD][DIM]
-  R* s_face = s_f + C::SF;                                // [NFN][FROW]
-  int* s_fmask = reinterpret_cast<int*>(s_face + C::SFACE);
+  R* s_face = C::ALIAS ? s_f : s_f + C::SF;              // [NFN][FROW]
+  int* s_fmask = reinterpret_cast<int*>(s_f + C::SFF);
+  constexpr bool ALIAS = C::ALIAS;
+  constexpr int NIT = (EPB * NFN + kBlock - 1) / kBlock;
+  R hold[ALIAS ? NIT : 1][ALIAS ? NFLD : 1];  // face fluxes until s_f is free
+  int hold_at[ALIAS ? NIT : 1];
   const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
-  const int64_t e0 = blk * EPB;
+  const int64_t e0 = visit_block<EPB>(a, blk) * EPB;
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   const bool inv = MODE == kEuler || a.inviscid_on;
@@ -507,11 +603,13 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   // doubles per thread and cut occupancy to one wave per SIMD; with five
   // fields the arrays are cheap and let all loads issue up front.
   constexpr bool STREAM = !HLL && NFLD > 6;
+  static_assert(!(STREAM && ALIAS), "streamed face fluxes are written to LDS directly");
   // fixed trip count, unrolled: the gathers of a thread's face nodes can
   // issue together
 #pragma unroll
-  for (int it = 0; it < (EPB * NFN + kBlock - 1) / kBlock; ++it) {
+  for (int it = 0; it < NIT; ++it) {
     const int t = int(threadIdx.x) + it * kBlock;
+    if constexpr (ALIAS) hold_at[it] = -1;
     if (t >= EPB * NFN) continue;
     const int le = t / NFN;
     const int fn = t - le * NFN;
@@ -638,9 +736,15 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   #pragma unroll
         for (int k = 0; k < NFLD; ++k) fl[k] = (fnm[k] + fnp[k] + lam * (qm[k] - qp[k])) / 2;
       }
+      if constexpr (ALIAS) {
+        hold_at[it] = fn * FROW + le * NFLD;
   #pragma unroll
-      for (int k = 0; k < NFLD; ++k)
-        s_face[fn * FROW + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
+        for (int k = 0; k < NFLD; ++k) hold[it][k] = R(-(fs * (fl[k] - shift[k])));
+      } else {
+  #pragma unroll
+        for (int k = 0; k < NFLD; ++k)
+          s_face[fn * FROW + le * NFLD + k] = R(-(fs * (fl[k] - shift[k])));
+      }
     } else {
       // Rusanov / central, field by field
       const int64_t gm = e * NP + im;
@@ -725,7 +829,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
       }
     }
   }
-  __syncthreads();
+  if constexpr (!ALIAS) __syncthreads();
 
   // phase 3: rhs[i] = sum_j sum_r Dw_r[i][j] G_r[j] + sum_m LIFT[i][m] B[m]
   // per (element, field) column, G_r = sum_d rx[r][d] F_d; rows split over
@@ -759,6 +863,16 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
 #pragma unroll
       for (int rr = 0; rr < RPW; ++rr) acc[rr] += DV[(j * DIM + r) * RPW + rr] * gr;
     }
+  }
+  if constexpr (ALIAS) {
+    // every nodal flux read: the face fluxes go over them
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < NIT; ++it)
+      if (hold_at[it] >= 0)
+#pragma unroll
+        for (int k = 0; k < NFLD; ++k) s_face[hold_at[it] + k] = hold[it][k];
+    __syncthreads();
   }
 #pragma unroll 4
   for (int m = 0; m < NFN; ++m) {
@@ -859,7 +973,7 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
   const R* ops = packed_ops<R, DIM, P>(ws);
 
   const int64_t N = a.K * NP;
-  const int64_t e0 = blk * EPB;
+  const int64_t e0 = visit_block<EPB>(a, blk) * EPB;
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];

@@ -53,6 +53,13 @@ class DeviceDiscretization:
         self.bnd_normals = t64(dcoll.bnd_normals)
         self._tag_index = {}
         self.checked_tag_sets = set()
+        if dcoll.traversal == "morton":
+            from mirgecom_amd.discretization import element_traversal_rank
+            self.elem_rank = t32(element_traversal_rank(dcoll.mesh))
+        elif dcoll.traversal == "natural":
+            self.elem_rank = None
+        else:
+            raise ValueError(f"unknown traversal {dcoll.traversal!r}")
         self.mesh = _lib.MgMesh(
             dim=dcoll.dim, order=re.order, np=re.nnodes, nfp=nfp,
             nfaces=re.nfaces, pad0=0, nelements=dcoll.nelements,
@@ -60,7 +67,8 @@ class DeviceDiscretization:
             rx=self.rx.data_ptr(), normals=self.normals.data_ptr(),
             fscale=self.fscale.data_ptr(), vmap_p=self.vmap_p.data_ptr(),
             weak_diff=self.weak_diff.data_ptr(), lift=self.lift.data_ptr(),
-            face_mask=self.face_mask.data_ptr(), workspace=None)
+            face_mask=self.face_mask.data_ptr(), workspace=None,
+            elem_rank=None if self.elem_rank is None else self.elem_rank.data_ptr())
         lib = _lib.load()
         nbytes = lib.mg_mesh_workspace_bytes(self.mesh)
         if nbytes <= 0:

@@ -157,6 +157,9 @@ class Discretization:
     bnd_nodes: np.ndarray      # (dim, Nbf*Nfp)
     bnd_normals: np.ndarray    # (dim, Nbf*Nfp)
     quadrature_order: int = 0  # degree of DISCR_TAG_QUAD (2p by default)
+    # order in which the device kernels take element blocks: "morton"
+    # (element_traversal_rank) or "natural" (element index order)
+    traversal: str = "morton"
     _device_cache: dict = field(default_factory=dict, repr=False)
     _host_cache: dict = field(default_factory=dict, repr=False)
 
@@ -434,3 +437,27 @@ def characteristic_lengthscales(dcoll: Discretization):
     d = np.linalg.norm(x[:, None, :] - x[None, :, :], axis=2)
     d[np.eye(len(d), dtype=bool)] = np.inf
     return inradius * d.min()
+
+
+def element_traversal_rank(mesh: BoxMesh):
+    """Position ``(K,)`` int32 of every element in the Z-order (Morton)
+    traversal of the element centroids: the centroids are quantised on a
+    ``2^b`` lattice over their bounding box (b = 10 bits per axis in 3-D, 15
+    in 2-D), the lattice coordinates' bits interleaved, and the elements
+    sorted by the interleaved code (ties by element index).  Consecutive
+    ranks are mesh neighbours except across the curve's jumps, which is what
+    the device kernels' block visiting order needs (it changes no result)."""
+    dim = mesh.dim
+    c = mesh.vertices[mesh.elements].mean(axis=1)
+    lo, hi = c.min(axis=0), c.max(axis=0)
+    bits = 10 if dim == 3 else 15
+    span = np.where(hi > lo, hi - lo, 1.0)
+    q = np.rint((c - lo) / span * ((1 << bits) - 1)).astype(np.int64)
+    code = np.zeros(len(c), dtype=np.int64)
+    for b in range(bits):
+        for d in range(dim):
+            code |= ((q[:, d] >> b) & 1) << (dim * b + d)
+    order = np.argsort(code, kind="stable")
+    rank = np.empty(len(c), dtype=np.int32)
+    rank[order] = np.arange(len(c), dtype=np.int32)
+    return rank
