@@ -491,6 +491,11 @@ struct RhsCfg {
   static constexpr int SFF = ALIAS ? (SF > SFACE ? SF : SFACE) : SF + SFACE;
   static constexpr size_t LDS = sizeof(R) * SFF + sizeof(double) * (SGEO + SREF + SWS) +
                                 sizeof(int) * E::NFN;
+  // waves per SIMD the register allocation aims at: 5 (96 VGPRs) for the
+  // mixed-precision many-field kernels, which come out a few registers
+  // above that otherwise; the compiler's choice elsewhere (forcing 5 on the
+  // FP64 five-field kernels spills and runs slower)
+  static constexpr int WAVES = (sizeof(R) == 4 && NFLD > 6) ? 5 : 1;
   static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
@@ -538,6 +543,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
       const int k = t - le * NODEF;
       s_ref[t] = le < nel ? double(a.fnet32[k * N + (e0 + le) * NP]) : 0.0;
     }
+    __syncthreads();
   }
 
   // phase 1: net flux at the volume nodes (coalesced; fixed trip count)
@@ -557,9 +563,11 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
       node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[g] : 0.0);
       s_ws[t] = s.speed + s.c;
       if constexpr (MIXED) {
-        const int64_t g0 = e * NP;
+        // relative to the element's first node (s_ref holds its FP32 values
+        // exactly)
+        const double* ref = s_ref + le * NODEF;
 #pragma unroll
-        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet32[k * N + g] - a.fnet32[k * N + g0];
+        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet32[k * N + g] - float(ref[k]);
       } else {
 #pragma unroll
         for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet[k * N + g];
@@ -895,7 +903,9 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
 }
 
 template <int DIM, int P, int NSP, int MIX, int MODE, typename R, bool HLL>
-__global__ __launch_bounds__(kBlock) void rhs_kernel(KArgs a,
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu(HLL ? 1 : RhsCfg<DIM, P, NSP, MIX, MODE, R>::WAVES)))
+void rhs_kernel(KArgs a,
                                                      const double* __restrict__ ws,
                                                      double* __restrict__ out) {
   rhs_body<DIM, P, NSP, MIX, MODE, R, HLL>(a, ws, out, xcd_block());
