@@ -184,12 +184,15 @@ mg_status mg_ns_div(const mg_mesh* mesh, const mg_gas* gas, const double* q,
  * momentum-flux diagonal, divergence contracted in FP32 from fluxes held
  * relative to each element's first node, face fluxes formed in FP64.  Both
  * shifts leave the DG divergence unchanged in exact arithmetic.  The state q
- * and the RHS stay FP64.  fnet_bnd carries the p_ref shift as well. */
+ * and the RHS stay FP64.  fnet_bnd carries the p_ref shift as well.
+ * Mixtures take two passes: FP32 gradients [(F+1)][dim][N] into grad
+ * (caller-allocated staging, required for mixtures, unused otherwise), then
+ * the net flux node by node. */
 mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas,
                            const double* q, const double* temperature,
                            const double* q_bnd, const double* t_bnd,
                            const mg_wall* wall, int32_t inviscid_on, float* fnet,
-                           double* fnet_bnd, void* stream);
+                           double* fnet_bnd, float* grad, void* stream);
 mg_status mg_ns_div_mixed(const mg_mesh* mesh, const mg_gas* gas,
                           const double* q, const double* temperature,
                           const float* fnet, const double* q_bnd,
@@ -481,6 +484,7 @@ typedef struct mg_ens_case {
   const double* t_seed;    /* [N] Newton seed                                */
   double* temperature;     /* [N] Newton result                              */
   float* fnet;             /* [F*3][N] FP32 nodal net flux (work)           */
+  float* grad;             /* [(F+1)*3][N] FP32 gradients (work)            */
   double* rhs;             /* [F][N] right-hand side (output)                */
 } mg_ens_case;
 

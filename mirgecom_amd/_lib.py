@@ -87,7 +87,8 @@ class MgEnsCase(ctypes.Structure):
     """``mg_ens_case`` (include/mgdg.h): one case of a batched ensemble."""
     _fields_ = [("mesh", MgMesh), ("gas", MgGas), ("q", ctypes.c_void_p),
                 ("t_seed", ctypes.c_void_p), ("temperature", ctypes.c_void_p),
-                ("fnet", ctypes.c_void_p), ("rhs", ctypes.c_void_p)]
+                ("fnet", ctypes.c_void_p), ("grad", ctypes.c_void_p),
+                ("rhs", ctypes.c_void_p)]
 
 
 class MgEnsTuning(ctypes.Structure):
@@ -145,7 +146,7 @@ SIGNATURES = {
     "mg_ns_div": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                          _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "mg_ns_flux_mixed": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
-                                _P, _P, _P, _P, _W, _I32, _P, _P, _P]),
+                                _P, _P, _P, _P, _W, _I32, _P, _P, _P, _P]),
     "mg_ns_div_mixed": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                _P, _P, _P, _P, _P, _P, _I32, _I32, _P, _P]),
     "mg_mixture_temperature": (_I32, [ctypes.POINTER(MgGas), _I32, _I64, _P,

@@ -4,10 +4,12 @@
 // sequence per case (the reference runs one driver per case:
 // mirgecom/steppers.py:350-449 around mirgecom/navierstokes.py:350-575).
 //
-// One right-hand side of every case = four phases, each one launch over the
-// work items of all cases (or, untuned, one launch per polynomial order):
+// One right-hand side of every case = five phases, each one launch over the
+// work items of all cases (the element phases G and D, untuned, one launch
+// per polynomial order):
 //   T   Newton temperature from each case's seed   (items: case x 256 nodes)
-//   G   gradients + nodal net flux, FP32 contraction (grad_body<FLUX, float>)
+//   G   FP32 gradients, FP32 contraction            (grad_body<!FLUX, float>)
+//   N   FP32 nodal net flux, node by node           (node_flux_body<F32>)
 //   D   divergence + face fluxes, FP32 contraction   (rhs_body<kFlux, float>)
 //   S   species sources added into the RHS: the learned surrogate with its
 //       gate (infer_body), or the exact rates
@@ -92,8 +94,8 @@ using DCfg = mgdg::RhsCfg<kDim, P, kNsp, kMix, mgdg::kFlux, R>;
 template <int PH, int P>
 __device__ __forceinline__ void run_body(const KArgs& a, int64_t blk) {
   if constexpr (PH == 0)
-    mgdg::grad_body<kDim, P, kNsp, kMix, true, R>(a, (const double*)a.ops, (double*)a.out,
-                                                  blk);
+    mgdg::grad_body<kDim, P, kNsp, kMix, false, R>(a, (const double*)a.ops, (double*)a.out,
+                                                   blk);
   else
     mgdg::rhs_body<kDim, P, kNsp, kMix, mgdg::kFlux, R, false>(a, (const double*)a.ops,
                                                                (double*)a.out, blk);
@@ -114,7 +116,7 @@ __device__ __forceinline__ void run_item(const KArgs* __restrict__ cases, const 
   }
 }
 
-// PH 0: gradients + nodal net flux, 1: divergence.  PSEL 0: any order (the
+// PH 0: gradients, 1: divergence.  PSEL 0: any order (the
 // body is selected per item), else only items of order PSEL.  PERSIST:
 // workgroups stride over the items (grid = workgroups per CU x CUs; the
 // loop keeps more values live, ~50 more VGPRs for the gradient phase), else
@@ -165,6 +167,23 @@ __global__ __launch_bounds__(kBlock) void ens_temperature_kernel(
       for (int k = 0; k < kNsp; ++k) y[k] = c.q[(2 + kDim + k) * n + g] / rho;
       c.tout[g] = mgdg::mix_temperature(e, c.tseed[g], y, niter);
     }
+  }
+}
+
+// Phase N: the FP32 nodal net flux of every node (items: case x 256 nodes;
+// descriptors: the gradient phase's, which carry grad32 and fnet32_out).
+__global__ __launch_bounds__(kBlock, 2) void ens_node_flux_kernel(
+    const KArgs* __restrict__ cases, const Seg* __restrict__ segs, int nseg,
+    int64_t nitems) {
+  __shared__ double s_pref;
+  for (int64_t it = blockIdx.x; it < nitems; it += gridDim.x) {
+    const Item w = find_item(segs, nseg, it);
+    const KArgs& c = cases[w.icase];
+    __syncthreads();   // the previous item's s_pref has been read
+    if (threadIdx.x == 0) s_pref = mgdg::reference_pressure<kDim, kNsp, kMix>(c);
+    __syncthreads();
+    const int64_t g = w.blk * kBlock + threadIdx.x;
+    if (g < c.nnodes) mgdg::node_flux_body<kDim, kNsp, kMix, true>(c, g, s_pref);
   }
 }
 
@@ -303,7 +322,7 @@ mg_status check_cases(int32_t ncases, const mg_ens_case* cases) {
     if (k.gas.eos_kind != 1 || k.gas.transport_kind == 3)
       return fail(MG_ERR_UNSUPPORTED,
                   "ensemble cases: uiuc_7sp mixture with a non-mixture-averaged transport");
-    if (!k.q || !k.t_seed || !k.temperature || !k.fnet || !k.rhs)
+    if (!k.q || !k.t_seed || !k.temperature || !k.fnet || !k.grad || !k.rhs)
       return fail(MG_ERR_ARG, "case " + std::to_string(c) + ": NULL buffer");
   }
   return MG_OK;
@@ -404,7 +423,9 @@ mg_status mg_ensemble_prepare(int32_t ncases, const mg_ens_case* cases, int32_t 
     a.temp = k.temperature;
     a.flux_kind = MG_FLUX_RUSANOV;
     a.inviscid_on = 1;
-    mgdg::KArgs g = a;
+    mgdg::KArgs g = a;   // gradient and node-flux phases
+    g.grad32_out = k.grad;
+    g.grad32 = k.grad;
     g.fnet32_out = k.fnet;
     g.out = nullptr;
     gt[c] = g;
@@ -515,6 +536,12 @@ mg_status mg_ensemble_ns_rhs(const mg_ens_plan* plan, void* stream) {
     if ((s = check_launch("ens_temperature_kernel"))) return s;
   }
   for (int phase = 0; phase < 2; ++phase) {
+    if (phase == 1 && plan->pt_count > 0) {
+      // phase N between the gradients and the divergence
+      hipLaunchKernelGGL(mgens::ens_node_flux_kernel, dim3(unsigned(plan->grid_pt)),
+                         dim3(kBlock), 0, st, gt, pi, plan->ncases, plan->pt_count);
+      if ((s = check_launch("ens_node_flux_kernel"))) return s;
+    }
     const auto* tab = phase ? dtab : gt;
     const Seg* segs = phase ? di : gi;
     const int32_t* sb = phase ? plan->div_seg_begin : plan->grad_seg_begin;

@@ -62,7 +62,7 @@ extern "C" {
 
 const char* mg_last_error(void) { return mgdg_shared::err().c_str(); }
 
-int32_t mg_abi_version(void) { return 5; }
+int32_t mg_abi_version(void) { return 6; }
 
 /* Upload the uiuc_7sp mechanism constants (host array of 7*7*2 + 7 + 7 + 7
  * + 2 + 9 doubles, packed as in mgdg::MechDev). */
@@ -276,7 +276,7 @@ mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas, const double*
                            const double* temperature, const double* q_bnd,
                            const double* t_bnd, const mg_wall* wall,
                            int32_t inviscid_on, float* fnet, double* fnet_bnd,
-                           void* stream) {
+                           float* grad, void* stream) {
   mg_status s = check_mesh(mesh);
   if (s) return s;
   if ((s = check_gas(gas, true))) return s;
@@ -296,7 +296,19 @@ mg_status mg_ns_flux_mixed(const mg_mesh* mesh, const mg_gas* gas, const double*
   a.fnb_out = fnet_bnd;
   a.inviscid_on = inviscid_on ? 1 : 0;
   hipStream_t st = static_cast<hipStream_t>(stream);
-  if ((s = dispatch(mesh, a, gas->nspecies, mix, 5, st))) return s;
+  if (mix) {
+    // mixture: FP32 gradients through `grad`, then node-parallel FP32 net
+    // fluxes (the fused kernel holds four 12-field elements per workgroup,
+    // too few nodes for the pointwise transport and NASA7 work)
+    if (!grad) return fail(MG_ERR_ARG, "the mixture path needs grad (FP32 gradient staging)");
+    a.grad32_out = grad;
+    if ((s = dispatch(mesh, a, gas->nspecies, mix, 5, st))) return s;
+    a.grad32_out = nullptr;
+    a.grad32 = grad;
+    if ((s = dispatch(mesh, a, gas->nspecies, mix, 7, st))) return s;
+  } else if ((s = dispatch(mesh, a, gas->nspecies, mix, 5, st))) {
+    return s;
+  }
   return wall_pass<true>(a, mesh->dim, gas->nspecies, mix, 1, st);
 }
 

@@ -198,8 +198,17 @@ mg_status node_flux_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t 
   if (nsp != 7 || !mix)
     return fail(MG_ERR_UNSUPPORTED, "node-parallel net flux is the mixture path");
   const unsigned nblk = unsigned((a.nnodes + mgdg::kBlock - 1) / mgdg::kBlock);
-  if (mix == 2)
+  // FP32 form when the FP32 net-flux output is set (mixed precision)
+  const bool f32 = a.fnet32_out != nullptr;
+  if (f32 && !a.grad32) return fail(MG_ERR_ARG, "FP32 net flux needs FP32 gradients");
+  if (mix == 2 && f32)
+    hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 2, true>), dim3(nblk),
+                       dim3(mgdg::kBlock), 0, st, a);
+  else if (mix == 2)
     hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 2>), dim3(nblk),
+                       dim3(mgdg::kBlock), 0, st, a);
+  else if (f32)
+    hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 1, true>), dim3(nblk),
                        dim3(mgdg::kBlock), 0, st, a);
   else
     hipLaunchKernelGGL((mgdg::node_flux_kernel<DIM, 7, 1>), dim3(nblk),
@@ -208,8 +217,14 @@ mg_status node_flux_species(const mgdg::KArgs& a, int nsp, int mix, hipStream_t 
   if (s || a.nbnd == 0) return s;
   const int64_t nt = a.K * mgdg::Elem<DIM, P>::NFN;
   const unsigned nb = unsigned((nt + mgdg::kBlock - 1) / mgdg::kBlock);
-  if (mix == 2)
+  if (mix == 2 && f32)
+    hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 2, true>), dim3(nb),
+                       dim3(mgdg::kBlock), 0, st, a);
+  else if (mix == 2)
     hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 2>), dim3(nb),
+                       dim3(mgdg::kBlock), 0, st, a);
+  else if (f32)
+    hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 1, true>), dim3(nb),
                        dim3(mgdg::kBlock), 0, st, a);
   else
     hipLaunchKernelGGL((mgdg::bnd_node_flux_kernel<DIM, P, 7, 1>), dim3(nb),
@@ -224,7 +239,15 @@ mg_status dispatch_p(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
   if constexpr (W == 2) return rhs_species<DIM, P, mgdg::kFlux, double>(a, nsp, mix, st);
   if constexpr (W == 3) return grad_species<DIM, P, false, double>(a, nsp, mix, st);
   if constexpr (W == 4) return grad_species<DIM, P, true, double>(a, nsp, mix, st);
-  if constexpr (W == 5) return grad_species<DIM, P, true, float>(a, nsp, mix, st);
+  if constexpr (W == 5) {
+    // mixtures: FP32 gradients for the node-parallel net flux when an FP32
+    // gradient output is set; otherwise the fused gradient + net flux kernel
+    if (mix && a.grad32_out) {
+      if (nsp != 7) return fail(MG_ERR_UNSUPPORTED, "mixture path is the 7-species mechanism");
+      return launch_grad<DIM, P, 7, 1, false, float>(a, st);
+    }
+    return grad_species<DIM, P, true, float>(a, nsp, mix, st);
+  }
   if constexpr (W == 6) return rhs_species<DIM, P, mgdg::kFlux, float>(a, nsp, mix, st);
   if constexpr (W == 7) return node_flux_species<DIM, P>(a, nsp, mix, st);
   return fail(MG_ERR_ARG, "kernel selector");

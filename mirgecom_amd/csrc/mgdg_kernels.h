@@ -76,6 +76,8 @@ struct KArgsT {
   MG_P(float) fnet32_out;   // FP32 nodal net flux output (mixed precision)
   MG_P(double) grad_out;    // optional gradient output of the fused flux kernel
   MG_P(double) fnb_out;     // exterior net normal flux output of the flux kernel
+  MG_P(float) grad32_out;   // FP32 gradient output (mixed-precision mixture path)
+  MG_P(const float) grad32; // FP32 gradients read by the node-parallel net flux
   // Mengaldo boundary data per boundary node (NULL wflag: none)
   MG_P(const double) wq;     // CV_bc [F][nbnd]
   MG_P(const double) wt;     // T_bc [nbnd]
@@ -973,7 +975,6 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
   constexpr int ROW = C::ROW, SP = C::SP;
   constexpr int RPW = O::RPW, GEO = C::GEO;
   constexpr bool MIXED = sizeof(R) == 4;
-  static_assert(!MIXED || FLUX, "mixed precision is the fused NS path");
   extern __shared__ double smem[];
   double* s_geo = smem;                          // [EPB][GEO]
   double* s_ref = s_geo + EPB * GEO;             // [EPB][NS] (mixed)
@@ -1229,7 +1230,10 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
   }
   __syncthreads();
   if constexpr (!FLUX) {
-    write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, out);
+    if constexpr (MIXED)
+      write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, (float*)a.grad32_out);
+    else
+      write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, out);
   } else {
     if (a.grad_out) write_staged<NP>(stage, NS * DIM, EPB, SP, e0, a.K, a.grad_out);
     const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
@@ -2073,17 +2077,22 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
 // per workgroup, so the pointwise phase, which carries the NASA7 and
 // transport work, kept 40 of 256 lanes busy.
 
-template <int DIM, int NSP, int MIX>
-__global__ __launch_bounds__(kBlock, 2) void node_flux_kernel(KArgs a) {
+//
+// F32: the mixed-precision form, from FP32 gradients (a.grad32) to FP32 net
+// fluxes (a.fnet32_out) that exclude p_ref I, as the mixed flux kernel
+// writes them.
+template <int DIM, int NSP, int MIX, bool F32, typename A>
+__device__ __forceinline__ void node_flux_body(const A& a, int64_t g, double p_ref) {
   constexpr int NFLD = 2 + DIM + NSP;
   const int64_t N = a.nnodes;
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= N) return;
   Node<DIM, NSP> s;
   load_q<DIM, NSP>(s, a.q, N, g);
   node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.temp[g] : 0.0);
   double gq[NFLD][DIM], gt[DIM], F[NFLD][DIM];
-  load_grad<DIM, NSP>(gq, gt, a.grad, N, g);
+  if constexpr (F32)
+    load_grad<DIM, NSP>(gq, gt, a.grad32, N, g);
+  else
+    load_grad<DIM, NSP>(gq, gt, a.grad, N, g);
   viscous_flux<DIM, NSP, MIX>(s, gq, gt, F);
   double FI[NFLD][DIM];
   if (a.inviscid_on) {
@@ -2097,19 +2106,47 @@ __global__ __launch_bounds__(kBlock, 2) void node_flux_kernel(KArgs a) {
 #pragma unroll
   for (int f = 0; f < NFLD; ++f)
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) a.out[(f * DIM + d) * N + g] = FI[f][d] - F[f][d];
+    for (int d = 0; d < DIM; ++d) {
+      if constexpr (F32)
+        a.fnet32_out[(f * DIM + d) * N + g] =
+            float(FI[f][d] - F[f][d] - ((f == 2 + d) ? p_ref : 0.0));
+      else
+        a.out[(f * DIM + d) * N + g] = FI[f][d] - F[f][d];
+    }
+}
+
+template <int DIM, int NSP, int MIX, bool F32 = false>
+__global__ __launch_bounds__(kBlock, 2) void node_flux_kernel(KArgs a) {
+  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double p_ref = 0.0;
+  if constexpr (F32) {
+    __shared__ double s_pref;
+    if (threadIdx.x == 0) s_pref = reference_pressure<DIM, NSP, MIX>(a);
+    __syncthreads();
+    p_ref = s_pref;
+  }
+  if (g < a.nnodes) node_flux_body<DIM, NSP, MIX, F32>(a, g, p_ref);
 }
 
 // Boundary face nodes (one thread per element face node; interior ones
 // return at once): the exterior net normal flux with the interior gradients
 // (boundary.py:944-970), or, on Mengaldo nodes, the interior gradients
 // copied to wwork for wall_flux_kernel.
-template <int DIM, int P, int NSP, int MIX>
+// F32: gradients from a.grad32 and exterior fluxes that exclude p_ref n (the
+// mixed-precision form).
+template <int DIM, int P, int NSP, int MIX, bool F32 = false>
 __global__ __launch_bounds__(kBlock) void bnd_node_flux_kernel(KArgs a) {
   using E = Elem<DIM, P>;
   constexpr int NP = E::NP, NFP = E::NFP, NFN = E::NFN, NF = E::NF;
   constexpr int NFLD = 2 + DIM + NSP, NS = NFLD + 1;
   const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  double p_ref = 0.0;
+  if constexpr (F32) {
+    __shared__ double s_pref;
+    if (threadIdx.x == 0) s_pref = reference_pressure<DIM, NSP, MIX>(a);
+    __syncthreads();
+    p_ref = s_pref;
+  }
   if (t >= a.K * NFN) return;
   const int32_t nb = a.vmap_p[t];
   if (nb >= 0) return;
@@ -2121,7 +2158,8 @@ __global__ __launch_bounds__(kBlock) void bnd_node_flux_kernel(KArgs a) {
   const int64_t gi = e * NP + a.fmask[fn];
   if (wall_flags(a, b) & kBcMengaldo) {
 #pragma unroll
-    for (int k = 0; k < NS * DIM; ++k) a.wwork[k * a.nbnd + b] = a.grad[k * N + gi];
+    for (int k = 0; k < NS * DIM; ++k)
+      a.wwork[k * a.nbnd + b] = F32 ? double(a.grad32[k * N + gi]) : a.grad[k * N + gi];
     return;
   }
   double n[DIM];
@@ -2131,7 +2169,10 @@ __global__ __launch_bounds__(kBlock) void bnd_node_flux_kernel(KArgs a) {
   load_q<DIM, NSP>(s, a.qb, a.nbnd, b);
   node_state<DIM, NSP, MIX>(s, a.gas, MIX ? a.tb[b] : 0.0);
   double gq[NFLD][DIM], gt[DIM], fv[NFLD], fi[NFLD];
-  load_grad<DIM, NSP>(gq, gt, a.grad, N, gi);
+  if constexpr (F32)
+    load_grad<DIM, NSP>(gq, gt, a.grad32, N, gi);
+  else
+    load_grad<DIM, NSP>(gq, gt, a.grad, N, gi);
   if (a.inviscid_on) {
     inviscid_flux_n<DIM, NSP>(s, n, fi);
   } else {
@@ -2140,7 +2181,9 @@ __global__ __launch_bounds__(kBlock) void bnd_node_flux_kernel(KArgs a) {
   }
   viscous_flux_n<DIM, NSP, MIX>(s, gq, gt, n, fv);
 #pragma unroll
-  for (int k = 0; k < NFLD; ++k) a.fnb_out[k * a.nbnd + b] = fi[k] - fv[k];
+  for (int k = 0; k < NFLD; ++k)
+    a.fnb_out[k * a.nbnd + b] =
+        fi[k] - fv[k] - ((k >= 2 && k < 2 + DIM) ? p_ref * n[k - 2] : 0.0);
 }
 
 // }}}

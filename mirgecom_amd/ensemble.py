@@ -145,6 +145,7 @@ class ReactingEnsemble:
         self.y, self.k, self.tmp, self.tmp2, self.acc = (buf(f) for _ in range(5))
         self.temperature = buf(1)
         self.fnet = buf(3 * f, torch.float32)
+        self.grad32 = buf(3 * (f + 1), torch.float32)
         self.fallback = torch.zeros(1, dtype=torch.int32, device=self.device)
         for i, sp in enumerate(self.specs):
             cv, t = case_initial_state(self.dcolls[i], sp.seed)
@@ -171,6 +172,9 @@ class ReactingEnsemble:
     def case_fnet(self, i):
         return self.case_view(self.fnet, i, 3 * NFIELDS)
 
+    def case_grad(self, i):
+        return self.case_view(self.grad32, i, 3 * (NFIELDS + 1))
+
     # }}}
 
     # {{{ plans
@@ -185,6 +189,7 @@ class ReactingEnsemble:
             c.t_seed = self.case_t(i).data_ptr()
             c.temperature = self.case_t(i).data_ptr()
             c.fnet = self.case_fnet(i).data_ptr()
+            c.grad = self.case_grad(i).data_ptr()
             c.rhs = self.case_view(self.k, i).data_ptr()
         return arr
 
@@ -244,7 +249,7 @@ class ReactingEnsemble:
             _lib.call("mg_mixture_temperature", gas, 3, n, _lib.ptr(q), _lib.ptr(t), 5,
                       _lib.ptr(t), st)
             _lib.call("mg_ns_flux_mixed", mesh, gas, _lib.ptr(q), _lib.ptr(t), None, None,
-                      None, 1, _lib.ptr(fn), None, st)
+                      None, 1, _lib.ptr(fn), None, _lib.ptr(self.case_grad(i)), st)
             _lib.call("mg_ns_div_mixed", mesh, gas, _lib.ptr(q), _lib.ptr(t), _lib.ptr(fn),
                       None, None, None, 0, 1, _lib.ptr(r), st)
             if self.surrogate is not None:

@@ -110,9 +110,15 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
         if prep.qb is not None:
             fnb = torch.empty((nf, prep.qb.shape[1]), dtype=prep.q.dtype,
                               device=prep.q.device)
+        g32 = None
+        if state.is_mixture:
+            # the mixture path stages FP32 gradients through this buffer
+            g32 = torch.empty((nf + 1, dim) + tuple(prep.q.shape[1:]),
+                              dtype=torch.float32, device=prep.q.device)
         _lib.call("mg_ns_flux_mixed", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
                   _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb),
-                  prep.wall, inv, _lib.ptr(fnet), _lib.ptr(fnb), prep.stream)
+                  prep.wall, inv, _lib.ptr(fnet), _lib.ptr(fnb), _lib.ptr(g32),
+                  prep.stream)
         rhs = torch.empty_like(prep.q)
         _lib.call("mg_ns_div_mixed", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
                   _lib.ptr(prep.temp), _lib.ptr(fnet), _lib.ptr(prep.qb),

@@ -34,7 +34,7 @@ def test_library_exports_every_declared_symbol():
     for name in _declared():
         assert hasattr(lib, name), name
         assert name in _lib.SIGNATURES, name
-    assert lib.mg_abi_version() == 5
+    assert lib.mg_abi_version() == 6
 
 
 def test_null_arguments_rejected():

@@ -148,6 +148,11 @@ def test_ns_walls(kind):
     err2 = rel_err(_host(rhs2.q), ref)
     print("from gradients", err2)
     assert err2.max() < 1e-11
+    with pytest.warns(UserWarning):
+        rhs3 = ns_operator(pb.dcoll, gm, st, {"all": gbc}, precision="mixed")
+    err3 = rel_err(_host(rhs3.q), ref)
+    print("mixed", err3)
+    assert err3.max() < 1e-3
 
 
 def test_timestep():

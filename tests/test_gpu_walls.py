@@ -165,3 +165,10 @@ def test_ns_walls_mixture(kind):
     err = rel_err(_host(rhs.q), ref)
     print("ns walls mixture", kind, err)
     assert err.max() < 1e-11
+    # mixed precision (FP32 gradients, node-parallel FP32 net fluxes with the
+    # p_ref shift, shifted exterior fluxes)
+    with pytest.warns(UserWarning):
+        rhs3 = ns_operator(pb.dcoll, gm, st, {"all": gbc}, precision="mixed")
+    err3 = rel_err(_host(rhs3.q), ref)
+    print("ns walls mixture mixed", kind, err3)
+    assert err3.max() < 1e-3
