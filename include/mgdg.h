@@ -131,6 +131,15 @@ mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas,
                        const double* q_bnd, const double* t_bnd,
                        int32_t flux_kind, double* rhs, void* stream);
 
+/* Mixed-precision mg_euler_rhs: the same right-hand side with the nodal
+ * fluxes held in LDS relative to each element's first node and contracted
+ * in FP32 (the weak divergence of that constant cancels its lift exactly),
+ * physics, face fluxes, state and RHS in FP64. */
+mg_status mg_euler_rhs_mixed(const mg_mesh* mesh, const mg_gas* gas,
+                             const double* q, const double* temperature,
+                             const double* q_bnd, const double* t_bnd,
+                             int32_t flux_kind, double* rhs, void* stream);
+
 /* DG gradients of the conserved fields and of T with the central flux on
  * interior faces and the exterior value (CV_bc / T_bc on Mengaldo nodes) on
  * boundary faces (replaces grad_cv_operator + grad_t_operator,

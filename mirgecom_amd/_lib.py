@@ -137,6 +137,8 @@ _W = ctypes.POINTER(MgWall)
 SIGNATURES = {
     "mg_euler_rhs": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                             _P, _P, _P, _P, _I32, _P, _P]),
+    "mg_euler_rhs_mixed": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                                  _P, _P, _P, _P, _I32, _P, _P]),
     "mg_ns_gradient": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                               _P, _P, _P, _P, _W, _P, _P]),
     "mg_ns_rhs": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),

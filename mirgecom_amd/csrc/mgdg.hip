@@ -154,6 +154,31 @@ mg_status mg_euler_rhs(const mg_mesh* mesh, const mg_gas* gas, const double* q,
   return dispatch(mesh, a, gas->nspecies, mix, 0, static_cast<hipStream_t>(stream));
 }
 
+mg_status mg_euler_rhs_mixed(const mg_mesh* mesh, const mg_gas* gas, const double* q,
+                             const double* temperature, const double* q_bnd,
+                             const double* t_bnd, int32_t flux_kind, double* rhs,
+                             void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, false))) return s;
+  const int mix = gas_variant(gas);
+  if (mix && (s = ensure_mech())) return s;
+  if (!q || !rhs) return fail(MG_ERR_ARG, "q/rhs NULL");
+  if (mix && !temperature) return fail(MG_ERR_ARG, "mixture needs temperature");
+  if (mesh->nbnd_nodes > 0 && (!q_bnd || (mix && !t_bnd)))
+    return fail(MG_ERR_ARG, "boundary exterior state missing");
+  if (flux_kind < 0 || flux_kind > 2) return fail(MG_ERR_ARG, "flux_kind");
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.temp = temperature;
+  a.qb = q_bnd;
+  a.tb = t_bnd;
+  a.out = rhs;
+  a.flux_kind = flux_kind;
+  a.inviscid_on = 1;
+  return dispatch(mesh, a, gas->nspecies, mix, 8, static_cast<hipStream_t>(stream));
+}
+
 mg_status mg_ns_gradient(const mg_mesh* mesh, const mg_gas* gas, const double* q,
                          const double* temperature, const double* q_bnd,
                          const double* t_bnd, const mg_wall* wall, double* grad,

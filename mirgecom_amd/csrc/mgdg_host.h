@@ -250,6 +250,7 @@ mg_status dispatch_p(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
   }
   if constexpr (W == 6) return rhs_species<DIM, P, mgdg::kFlux, float>(a, nsp, mix, st);
   if constexpr (W == 7) return node_flux_species<DIM, P>(a, nsp, mix, st);
+  if constexpr (W == 8) return rhs_species<DIM, P, mgdg::kEuler, float>(a, nsp, mix, st);
   return fail(MG_ERR_ARG, "kernel selector");
 }
 
@@ -263,6 +264,7 @@ mg_status dispatch_p(const mgdg::KArgs& a, int nsp, int mix, hipStream_t st) {
                               int mix, hipStream_t st);
 MG_DISPATCH_DECL(0) MG_DISPATCH_DECL(1) MG_DISPATCH_DECL(2) MG_DISPATCH_DECL(3)
 MG_DISPATCH_DECL(4) MG_DISPATCH_DECL(5) MG_DISPATCH_DECL(6) MG_DISPATCH_DECL(7)
+MG_DISPATCH_DECL(8)
 #undef MG_DISPATCH_DECL
 #define MG_DEFINE_DISPATCH(W)                                                      \
   mg_status mgdg_dispatch_##W(const mg_mesh* m, const mgdg::KArgs& a, int nsp,     \
@@ -282,7 +284,7 @@ namespace {
 
 // what: 0 Euler RHS, 1 NS RHS from gradients, 2 NS RHS from nodal fluxes,
 // 3 gradients, 4 gradients + nodal fluxes, 5/6 the mixed-precision 4/2,
-// 7 nodal fluxes from stored gradients (mixture)
+// 7 nodal fluxes from stored gradients (mixture), 8 the mixed-precision 0
 inline mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, int mix,
                           int what, hipStream_t st) {
   switch (what) {
@@ -294,6 +296,7 @@ inline mg_status dispatch(const mg_mesh* m, const mgdg::KArgs& a, int nsp, int m
     case 5: return mgdg_dispatch_5(m, a, nsp, mix, st);
     case 6: return mgdg_dispatch_6(m, a, nsp, mix, st);
     case 7: return mgdg_dispatch_7(m, a, nsp, mix, st);
+    case 8: return mgdg_dispatch_8(m, a, nsp, mix, st);
   }
   return fail(MG_ERR_ARG, "kernel selector");
 }
@@ -383,8 +386,12 @@ inline mg_status set_wall(mgdg::KArgs& a, const mg_mesh* m, const mg_wall* w, bo
 template <int DIM, int NSP, int MIX, bool FUSED>
 mg_status launch_wall(const mgdg::KArgs& a, int shift, hipStream_t st) {
   const unsigned nblk = unsigned((a.nbnd + mgdg::kBlock - 1) / mgdg::kBlock);
-  hipLaunchKernelGGL((mgdg::wall_flux_kernel<DIM, NSP, MIX, FUSED>), dim3(nblk),
-                     dim3(mgdg::kBlock), 0, st, a, shift);
+  if (FUSED && shift)
+    hipLaunchKernelGGL((mgdg::wall_flux_kernel<DIM, NSP, MIX, FUSED, FUSED>), dim3(nblk),
+                       dim3(mgdg::kBlock), 0, st, a);
+  else
+    hipLaunchKernelGGL((mgdg::wall_flux_kernel<DIM, NSP, MIX, FUSED, false>), dim3(nblk),
+                       dim3(mgdg::kBlock), 0, st, a);
   return check_launch("wall_flux_kernel");
 }
 

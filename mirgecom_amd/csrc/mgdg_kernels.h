@@ -364,13 +364,25 @@ constexpr int odd_stride(int n) { return n | 1; }
 // node of the mesh.  Subtracting a constant p_ref from the momentum flux
 // diagonal does not change the DG divergence (the weak divergence of a
 // constant flux cancels its lift exactly), but it removes the large
-// low-Mach background from the values that are rounded to FP32.
+// low-Mach background from the values that are rounded to FP32.  Only the
+// pressure is formed (ideal gas: (gamma - 1) rho e; mixture: R T sum_k
+// rho Y_k / W_k), so that every thread can afford it.
 template <int DIM, int NSP, int MIX, typename A>
 __device__ __forceinline__ double reference_pressure(const A& a) {
-  Node<DIM, NSP> s;
-  load_q<DIM, NSP>(s, a.q, a.nnodes, 0);
-  node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[0] : 0.0);
-  return s.p;
+  const int64_t N = a.nnodes;
+  if constexpr (MIX == 0) {
+    const double rho = a.q[0];
+    double mm = 0.0;
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) mm += a.q[(2 + d) * N] * a.q[(2 + d) * N];
+    return (a.gas.gamma - 1.0) * (a.q[N] - 0.5 * mm / rho);
+  } else {
+    static_assert(NSP == MIX_NSP, "mixture path is the 7-species mechanism");
+    double rs = 0.0;
+#pragma unroll
+    for (int k = 0; k < MIX_NSP; ++k) rs += a.q[(2 + DIM + k) * N] * c_mech.inv_wt[k];
+    return c_mech.gas_constant * rs * a.temp[0];
+  }
 }
 
 // Mengaldo boundary nodes (boundary.py:391-793): gradient values CV_bc and
@@ -497,7 +509,7 @@ struct RhsCfg {
   // mixed-precision many-field kernels, which come out a few registers
   // above that otherwise; the compiler's choice elsewhere (forcing 5 on the
   // FP64 five-field kernels spills and runs slower)
-  static constexpr int WAVES = (sizeof(R) == 4 && NFLD > 6) ? 5 : 1;
+  static constexpr int WAVES = (sizeof(R) == 4 && NFLD > 6 && MODE == kFlux) ? 5 : 1;
   static_assert(EPB >= 1, "too many fields");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
@@ -516,7 +528,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   constexpr int FROW = C::FROW, SP = C::SP;
   constexpr bool VISC = MODE == kGrad;
   constexpr bool MIXED = sizeof(R) == 4;
-  static_assert(!MIXED || MODE == kFlux, "mixed precision is the fused NS path");
+  static_assert(!MIXED || MODE != kGrad, "mixed precision: Euler and the fused NS path");
   extern __shared__ double smem[];
   double* s_geo = smem;                                   // [EPB][DIM][DIM]
   double* s_ref = s_geo + C::SGEO;                        // [EPB][NODEF] (mixed)
@@ -535,11 +547,13 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   const int64_t nel = a.K - e0 < EPB ? a.K - e0 : EPB;
   const int64_t g_lo = e0 * NP, g_hi = (e0 + nel) * NP;
   const bool inv = MODE == kEuler || a.inviscid_on;
+  // FP32 fluxes exclude p_ref I (the stored net fluxes of the NS path, and
+  // the Euler fluxes formed here)
   const double p_ref = MIXED ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
   for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
   for (int t = threadIdx.x; t < EPB * DIM * DIM; t += blockDim.x)
     s_geo[t] = t < nel * DIM * DIM ? a.rx[e0 * DIM * DIM + t] : 0.0;
-  if constexpr (MIXED) {
+  if constexpr (MIXED && MODE == kFlux) {
     for (int t = threadIdx.x; t < EPB * NODEF; t += blockDim.x) {
       const int le = t / NODEF;
       const int k = t - le * NODEF;
@@ -600,10 +614,24 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
 #pragma unroll
       for (int f = 0; f < NFLD; ++f)
 #pragma unroll
-        for (int d = 0; d < DIM; ++d) fdst[f * DIM + d] = F[f][d];
+        for (int d = 0; d < DIM; ++d)
+          fdst[f * DIM + d] = R(F[f][d] - ((MIXED && f == 2 + d) ? p_ref : 0.0));
     }
   }
   __syncthreads();
+  // REL: s_f holds each element's fluxes relative to its first node (the
+  // stored FP32 net fluxes); the Euler kernel keeps its FP32 fluxes absolute
+  // and subtracts the first node's in the contraction (in FP32, where the
+  // values are already rounded).  Either way s_ref holds the reference.
+  constexpr bool REL = MIXED && MODE == kFlux;
+  if constexpr (MIXED && MODE == kEuler) {
+    for (int t = threadIdx.x; t < EPB * NODEF; t += blockDim.x) {
+      const int le = t / NODEF;
+      const int k = t - le * NODEF;
+      s_ref[t] = le < nel ? double(s_f[(le * NP) * NODEP + k]) : 0.0;
+    }
+    __syncthreads();
+  }
 
   // phase 2: numerical face fluxes (FP64).  The own side and neighbours
   // inside this workgroup come from LDS; other neighbours are gathered here.
@@ -641,7 +669,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   #pragma unroll
         for (int d = 0; d < DIM; ++d) {
           const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
-          c += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
+          c += (double(s_f[lm * NODEP + k * DIM + d]) + (REL ? r0 : 0.0)) * n[d];
           sh += r0 * n[d];
         }
         fnm[k] = c;
@@ -666,7 +694,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   #pragma unroll
           for (int d = 0; d < DIM; ++d)
             c += (double(s_f[lp * NODEP + k * DIM + d]) +
-                  (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
+                  (REL ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
           fnp[k] = c;
         }
       } else {
@@ -702,6 +730,10 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
         } else {
           if (inv) {
             inviscid_flux_n<DIM, NSP>(sp, n, fnp);
+            if constexpr (MIXED) {
+  #pragma unroll
+              for (int d = 0; d < DIM; ++d) fnp[2 + d] -= p_ref * n[d];
+            }
           } else {
   #pragma unroll
             for (int k = 0; k < NFLD; ++k) fnp[k] = 0.0;
@@ -804,7 +836,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
 #pragma unroll
         for (int d = 0; d < DIM; ++d) {
           const double r0 = MIXED ? s_ref[le * NODEF + k * DIM + d] : 0.0;
-          fm += (double(s_f[lm * NODEP + k * DIM + d]) + r0) * n[d];
+          fm += (double(s_f[lm * NODEP + k * DIM + d]) + (REL ? r0 : 0.0)) * n[d];
           sh += r0 * n[d];
         }
         const double qmk = a.q[k * N + gm];
@@ -814,7 +846,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
 #pragma unroll
           for (int d = 0; d < DIM; ++d)
             fp += (double(s_f[lp * NODEP + k * DIM + d]) +
-                   (MIXED ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
+                   (REL ? s_ref[ple * NODEF + k * DIM + d] : 0.0)) * n[d];
         } else if constexpr (MODE == kFlux) {
           if (bnd) {
             fp = a.fnb[k * a.nbnd + b];
@@ -829,7 +861,7 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
             // component k of inviscid_flux_n (mgdg_physics.h)
             if (k == 0) fp = mnp;
             else if (k == 1) fp = vnp * (sp.q[1] + sp.p);
-            else if (k < 2 + DIM) fp = sp.q[0] * sp.v[k - 2] * vnp + sp.p * n[k - 2];
+            else if (k < 2 + DIM) fp = sp.q[0] * sp.v[k - 2] * vnp + (sp.p - p_ref) * n[k - 2];
             else fp = vnp * sp.q[k];
           }
           if constexpr (VISC) fp -= fv[k];
@@ -859,12 +891,16 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   R acc[RPW];
 #pragma unroll
   for (int rr = 0; rr < RPW; ++rr) acc[rr] = R(0);
+  R f0[DIM];   // this column's reference (absolute FP32 Euler fluxes)
+#pragma unroll
+  for (int d = 0; d < DIM; ++d)
+    f0[d] = (MIXED && !REL) ? s_f[(cle * NP) * NODEP + cf * DIM + d] : R(0);
 #pragma unroll 2
   for (int j = 0; j < NP; ++j) {
     const R* fj = s_f + (cle * NP + j) * NODEP + cf * DIM;
     R fd[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) fd[d] = fj[d];
+    for (int d = 0; d < DIM; ++d) fd[d] = fj[d] - f0[d];
 #pragma unroll
     for (int r = 0; r < DIM; ++r) {
       R gr = R(0);
@@ -2196,8 +2232,9 @@ __global__ __launch_bounds__(kBlock) void bnd_node_flux_kernel(KArgs a) {
 // for the mixed path) is written to fnb_out.  Otherwise the gradients come
 // from the full gradient array and the effective exterior viscous flux
 // 2 F_V(bc).n - F_V(q-).n is written to wwork for rhs_kernel<kGrad>.
-template <int DIM, int NSP, int MIX, bool FUSED>
-__global__ __launch_bounds__(kBlock) void wall_flux_kernel(KArgs a, int shift_pref) {
+// SHIFT (FUSED only): the exterior fluxes exclude p_ref n (mixed precision).
+template <int DIM, int NSP, int MIX, bool FUSED, bool SHIFT>
+__global__ __launch_bounds__(kBlock) void wall_flux_kernel(KArgs a) {
   constexpr int NFLD = 2 + DIM + NSP;
   const int64_t b = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (b >= a.nbnd) return;
@@ -2230,7 +2267,8 @@ __global__ __launch_bounds__(kBlock) void wall_flux_kernel(KArgs a, int shift_pr
 #pragma unroll
       for (int k = 0; k < NFLD; ++k) fi[k] = 0.0;
     }
-    const double p_ref = shift_pref ? reference_pressure<DIM, NSP, MIX>(a) : 0.0;
+    double p_ref = 0.0;
+    if constexpr (SHIFT) p_ref = reference_pressure<DIM, NSP, MIX>(a);
 #pragma unroll
     for (int k = 0; k < NFLD; ++k)
       a.fnb_out[k * a.nbnd + b] =

@@ -23,7 +23,7 @@ def euler_operator(dcoll, state, gas_model, boundaries, time=0.0,
                    quadrature_tag=DISCR_TAG_BASE, dd=DD_VOLUME_ALL,
                    comm_tag=None, use_esdg=False, operator_states_quad=None,
                    entropy_conserving_flux_func=None, limiter_func=None,
-                   entropy_min=None):
+                   entropy_min=None, precision="float64"):
     r"""RHS of the Euler equations, :math:`\dot{q} = -\nabla\cdot F`.
 
     Same arguments and errors as the reference (mirgecom/euler.py:313-372).
@@ -32,7 +32,17 @@ def euler_operator(dcoll, state, gas_model, boundaries, time=0.0,
     points of ``dcoll.quadrature``; with ``use_esdg=True`` as well, the
     entropy-stable flux-differencing operator (``mg_euler_rhs_esdg``,
     mirgecom/euler.py:108-310; ideal gas).  Limiters are outside the device
-    path and raise ``NotImplementedError``."""
+    path and raise ``NotImplementedError``.
+
+    ``precision`` (not in the reference signature): ``"float64"`` (default)
+    or ``"mixed"`` -- nodal fluxes held relative to each element's first node
+    and contracted in FP32, physics, face fluxes, state and RHS in FP64
+    (``mg_euler_rhs_mixed``; nodal operator only)."""
+    if precision not in ("float64", "mixed"):
+        raise ValueError("precision must be 'float64' or 'mixed'")
+    if precision == "mixed" and (use_esdg or quadrature_tag == DISCR_TAG_QUAD):
+        raise NotImplementedError("the mixed-precision Euler path is the nodal "
+                                  "operator")
     check_dd(dd)
     check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
                       limiter_func, entropy_min, allow_quad=True)
@@ -50,7 +60,8 @@ def euler_operator(dcoll, state, gas_model, boundaries, time=0.0,
         return _euler_quad(dcoll, state, gas_model, boundaries, time, kind)
     prep = Prepared(dcoll, state, gas_model, boundaries, time)
     rhs = torch.empty_like(prep.q)
-    _lib.call("mg_euler_rhs", prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
+    entry = "mg_euler_rhs_mixed" if precision == "mixed" else "mg_euler_rhs"
+    _lib.call(entry, prep.dev.mesh, prep.gas, _lib.ptr(prep.q),
               _lib.ptr(prep.temp), _lib.ptr(prep.qb), _lib.ptr(prep.tb), kind,
               _lib.ptr(rhs), prep.stream)
     return ConservedVars(state.dim, rhs)
