@@ -316,6 +316,91 @@ def config1_line(cores):
             "linf_vs_exact": linf}
 
 
+def measure_reacting_variants(args, wl, model=None):
+    """The reacting box in its Euler form (config of measure_reacting):
+    plain FP64, mixed precision (mg_euler_rhs_mixed) with the exact sources,
+    and mixed precision with the surrogate's sources; per-RHS times and the
+    per-field error of a march of args.march_steps RK4 steps at dt = 1e-8
+    against the FP64 plain march."""
+    import warnings
+
+    import torch
+
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import make_fluid_state
+    from mirgecom_amd.integrators import rk4_step
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+
+    gm, dcoll, dt = wl.gas_model, wl.dcoll, wl.dt
+    npe = dcoll.refel.nnodes
+
+    def make_rhs(precision, source):
+        tbox = [wl.temperature.clone()]
+
+        def rhs(t, cv):
+            st = make_fluid_state(cv, gm, temperature_seed=tbox[0])
+            tbox[0] = st.temperature
+            with warnings.catch_warnings():
+                warnings.simplefilter("ignore")
+                r = euler_operator(dcoll, st, gm, {}, precision=precision,
+                                   inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
+            if source == "exact":
+                return gm.eos.add_species_source_terms(cv, st.temperature, r)
+            return model.species_source(cv, st.temperature, dt, npe, rhs=r)
+        return rhs
+
+    variants = [("fp64_exact", "float64", "exact"), ("mixed_exact", "mixed", "exact")]
+    if model is not None:
+        variants.append(("mixed_surrogate", "mixed", "surrogate"))
+    out = {}
+    dofs = dcoll.mesh.nelements * npe * 12
+    for name, prec, src in variants:
+        f = make_rhs(prec, src)
+        for _ in range(args.warmup):
+            f(0.0, wl.cv)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            f(0.0, wl.cv)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        out[name] = {"ms_per_rhs": ms, "dof_rhs_s": dofs / (ms * 1e-3)}
+    finals = {}
+    for name, prec, src in variants:
+        f = make_rhs(prec, src)
+        cv = wl.cv
+        for k in range(args.march_steps):
+            cv = rk4_step(cv, k * dt, dt, f)
+        torch.cuda.synchronize()
+        finals[name] = cv.q
+    ref = finals["fp64_exact"]
+    eos = gm.eos
+    cref = ConservedVars(3, ref)
+    tref = eos.temperature(cref, temperature_seed=wl.temperature)
+    cmax = float(eos.sound_speed(cref, tref).max())
+    rhomax = float(ref[0].max())
+    for name in finals:
+        if name == "fp64_exact":
+            continue
+        q = finals[name]
+        cq = ConservedVars(3, q)
+        tq = eos.temperature(cq, temperature_seed=wl.temperature)
+        out[name]["march_rel_err"] = _rel_err_mix(q, ref, 3)
+        out[name]["march_max_abs_dT_K"] = float((tq - tref).abs().max())
+        out[name]["march_max_abs_dY"] = float(
+            (cq.species_mass_fractions - cref.species_mass_fractions).abs().max())
+        out[name]["march_momentum_err_over_rho_c"] = float(
+            (q[2:5] - ref[2:5]).abs().max()) / (rhomax * cmax)
+        out[name]["speedup_vs_fp64"] = (out["fp64_exact"]["ms_per_rhs"]
+                                        / out[name]["ms_per_rhs"])
+    res = {"workload": f"reacting box {dcoll.mesh.nelements} tets p=2, Euler + uiuc_7sp "
+                       "sources, Newton T seeded from the previous stage",
+           "dofs": dofs, "march": f"{args.march_steps} RK4 steps at dt={dt}", **out}
+    print(f"# reacting Euler variants: {json.dumps(res)}", file=sys.stderr, flush=True)
+    return res
+
+
 def _ns_reacting_model(eos):
     """Constant transport of a 1600 K ethylene-air mixture (mu ~ 5e-5 Pa s,
     kappa ~ 0.1 W/(m K), d ~ 2.3e-4 m^2/s)."""
@@ -875,6 +960,7 @@ def main():
         reacting = measure_reacting(args, wl3)
         if not args.no_surrogate:
             surrogate, model = measure_surrogate(args, wl3)
+        reacting["precision_variants"] = measure_reacting_variants(args, wl3, model)
         reacting_ns = measure_reacting_ns(args, wl3, model)
         del wl3
     if args.ens_cases:
