@@ -62,6 +62,10 @@ def parse():
                     help="RK4 steps of the reacting-box marches")
     ap.add_argument("--out-dir", default=None,
                     help="directory for the ensemble tuning table")
+    ap.add_argument("--mixture-kernels", type=int, default=0, metavar="REPS",
+                    help="only evaluate the reacting-box right-hand sides (Euler and "
+                         "Navier-Stokes, FP64 and mixed) REPS times each and exit "
+                         "(a short run for kernel profiling)")
     return ap.parse_args()
 
 
@@ -811,6 +815,38 @@ def _rel_err_cv(a, b, dim):
     return out
 
 
+def mixture_kernels(args):
+    """The reacting box's right-hand sides (config 3 state, 32^3 x 6 p = 2):
+    Euler FP64 and mixed, Navier-Stokes with constant transport FP64 and
+    mixed, each evaluated args.mixture_kernels times."""
+    import warnings
+
+    import torch
+
+    from mirgecom_amd.euler import euler_operator
+    from mirgecom_amd.gas_model import make_fluid_state
+    from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
+    from mirgecom_amd.navierstokes import ns_operator
+    from mirgecom_amd.workloads import reacting_box
+
+    torch.cuda.set_device(0)
+    wl = reacting_box(n=args.reacting_n or 32, order=2)
+    gm_ns = _ns_reacting_model(wl.gas_model.eos)
+    ste = make_fluid_state(wl.cv, wl.gas_model, temperature_seed=wl.temperature)
+    stn = make_fluid_state(wl.cv, gm_ns, temperature_seed=wl.temperature)
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        for prec in ("float64", "mixed"):
+            for _ in range(args.mixture_kernels):
+                euler_operator(wl.dcoll, ste, wl.gas_model, {}, precision=prec,
+                               inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
+            for _ in range(args.mixture_kernels):
+                ns_operator(wl.dcoll, gm_ns, stn, {}, precision=prec)
+    torch.cuda.synchronize()
+    print(f"# mixture kernels: {args.mixture_kernels} evaluations per operator and "
+          "precision", file=sys.stderr, flush=True)
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -818,6 +854,9 @@ def main():
               "(the multi-GPU path is a design note, not built)",
               file=sys.stderr)
         sys.exit(2)
+    if args.mixture_kernels:
+        mixture_kernels(args)
+        return
     import math
 
     import torch
