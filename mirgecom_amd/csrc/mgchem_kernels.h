@@ -472,10 +472,14 @@ constexpr int kRows = 128;
 constexpr int kLdsStride = kH + 8;
 
 // One hidden layer of the fused inference, in place on the activations
-// act [128][kLdsStride] (BF16, LDS): wave w owns output columns
-// [64 w, 64 w + 64) of every row, 8 x 4 accumulator fragments; the weight
-// fragments of the next K step are fetched while the current one's 32
-// matrix-core products run (two register sets).
+// act [128][kLdsStride] (BF16, LDS), formed transposed: D = W X^T with the
+// weights W [out][k] as the A operand and the activations as the B operand,
+// so that each lane ends with 4 consecutive output features of one row, which
+// go to LDS as one 8-byte store after the bias, GELU and a packed BF16
+// conversion.  Wave w owns output features [64 w, 64 w + 64) of every row:
+// 4 x 8 accumulator fragments; the weight fragments of the next K step are
+// fetched while the current one's 32 matrix-core products run (two register
+// sets).
 template <int KDIM>
 __device__ __forceinline__ void hidden_layer(__bf16* act, const __bf16* __restrict__ wt,
                                              const float* __restrict__ bias, int w,
@@ -483,51 +487,54 @@ __device__ __forceinline__ void hidden_layer(__bf16* act, const __bf16* __restri
   constexpr int NKS = KDIM / 32;
   const int r16 = lane & 15, kq = (lane >> 4) * 8;
   const int c0 = w * 64;
-  floatx4 acc[8][4];
+  floatx4 acc[4][8];   // [out tile j][row tile i]: D[out][row]
 #pragma unroll
-  for (int i = 0; i < 8; ++i)
+  for (int j = 0; j < 4; ++j)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-  auto load_b = [&](bf16x8* fb, int k0) {
+    for (int i = 0; i < 8; ++i) acc[j][i] = floatx4{0.f, 0.f, 0.f, 0.f};
+  auto load_w = [&](bf16x8* fw, int k0) {
 #pragma unroll
     for (int j = 0; j < 4; ++j)
-      fb[j] = *reinterpret_cast<const bf16x8*>(wt + int64_t(c0 + 16 * j + r16) * KDIM + k0 +
+      fw[j] = *reinterpret_cast<const bf16x8*>(wt + int64_t(c0 + 16 * j + r16) * KDIM + k0 +
                                                kq);
   };
-  auto mma = [&](const bf16x8* fb, int k0) {
+  auto mma = [&](const bf16x8* fw, int k0) {
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const bf16x8 fa =
+      const bf16x8 fx =
           *reinterpret_cast<const bf16x8*>(act + (16 * i + r16) * kLdsStride + k0 + kq);
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fa, fb[j], acc[i][j], 0, 0, 0);
+        acc[j][i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fw[j], fx, acc[j][i], 0, 0, 0);
     }
   };
-  bf16x8 fb0[4], fb1[4];
-  load_b(fb0, 0);
+  bf16x8 fw0[4], fw1[4];
+  load_w(fw0, 0);
   if constexpr (NKS == 1) {
-    mma(fb0, 0);
+    mma(fw0, 0);
   } else {
 #pragma unroll 1
     for (int ks = 0; ks < NKS; ks += 2) {
-      load_b(fb1, (ks + 1) * 32);
-      mma(fb0, ks * 32);
-      if (ks + 2 < NKS) load_b(fb0, (ks + 2) * 32);
-      mma(fb1, (ks + 1) * 32);
+      load_w(fw1, (ks + 1) * 32);
+      mma(fw0, ks * 32);
+      if (ks + 2 < NKS) load_w(fw0, (ks + 2) * 32);
+      mma(fw1, (ks + 1) * 32);
     }
   }
   __syncthreads();   // every wave has read this layer's input
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const int col = c0 + 16 * j + r16;
-    const float b = bias[col];
+    const int o = c0 + 16 * j + 4 * (lane >> 4);   // this lane's 4 output features
+    float b[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) b[e] = bias[o + e];
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
-      const int row0 = 16 * i + (lane >> 4) * 4;
+      floatx4 v;
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        act[(row0 + e) * kLdsStride + col] = __bf16(gelu(acc[i][j][e] + b));
+      for (int e = 0; e < 4; ++e) v[e] = gelu(acc[j][i][e] + b[e]);
+      *reinterpret_cast<bf16x4*>(act + (16 * i + r16) * kLdsStride + o) =
+          __builtin_convertvector(v, bf16x4);
     }
   }
   __syncthreads();
