@@ -303,6 +303,38 @@ mg_status mg_euler_rhs_quad(const mg_mesh* mesh, const mg_gas* gas,
                             const double* t_bnd_quad, int32_t niter, int32_t flux_kind,
                             double* rhs, void* stream);
 
+/* DG gradients of the conserved fields and of T on the quadrature
+ * discretization (grad_cv_operator + grad_t_operator with
+ * quadrature_tag=DISCR_TAG_QUAD, mirgecom/navierstokes.py:129-347): the
+ * state interpolated to the volume and face points, T from the EOS there
+ * (mixtures: niter Newton steps seeded with the interpolated nodal
+ * temperature), the central value at interior face points and the exterior
+ * state at boundary points (prescribed / dummy boundaries: q_bnd_quad,
+ * t_bnd_quad as for mg_euler_rhs_quad), integrated with the quadrature
+ * weights.  grad [F+1][dim][N] (T last), the layout of mg_ns_gradient.
+ * The gas needs transport; nspecies 0 or 7 (the mixture: 7). */
+mg_status mg_ns_gradient_quad(const mg_mesh* mesh, const mg_gas* gas,
+                              const mg_quad* quad, const double* q,
+                              const double* temperature, const double* q_bnd_quad,
+                              const double* t_bnd_quad, int32_t niter, double* grad,
+                              void* stream);
+
+/* Navier-Stokes RHS div(F_V - F_I) on the quadrature discretization
+ * (ns_operator with quadrature_tag=DISCR_TAG_QUAD,
+ * mirgecom/navierstokes.py:350-575): the viscous and (inviscid_on != 0)
+ * inviscid fluxes at the volume points from the point states and the nodal
+ * gradients grad [F+1][dim][N] interpolated there (op.project); at the face
+ * points the numerical inviscid flux (flux_kind) and the central viscous
+ * flux of both sides' states and gradient traces (boundary points: the
+ * exterior state with the interior gradients, viscous.py:431-544); weak
+ * divergence and lift with the quadrature weights.  Arguments as for
+ * mg_ns_gradient_quad. */
+mg_status mg_ns_rhs_quad(const mg_mesh* mesh, const mg_gas* gas, const mg_quad* quad,
+                         const double* q, const double* temperature,
+                         const double* grad, const double* q_bnd_quad,
+                         const double* t_bnd_quad, int32_t niter, int32_t flux_kind,
+                         int32_t inviscid_on, double* rhs, void* stream);
+
 /* Hybridized summation-by-parts operators of the entropy-stable path (on
  * the quadrature of an mg_quad; nh = nq + nfaces*nfq hybridized points:
  * volume points, then each face's points), device arrays:

@@ -165,6 +165,12 @@ SIGNATURES = {
     "mg_euler_rhs_quad": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  ctypes.POINTER(MgQuad), _P, _P, _P, _P, _I32,
                                  _I32, _P, _P]),
+    "mg_ns_gradient_quad": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                                   ctypes.POINTER(MgQuad), _P, _P, _P, _P, _I32,
+                                   _P, _P]),
+    "mg_ns_rhs_quad": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
+                              ctypes.POINTER(MgQuad), _P, _P, _P, _P, _P, _I32,
+                              _I32, _I32, _P, _P]),
     "mg_euler_rhs_esdg": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  ctypes.POINTER(MgQuad), ctypes.POINTER(MgEsdg),
                                  _P, _P, _I32, _P, _P, _P]),

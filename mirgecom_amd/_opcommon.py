@@ -30,12 +30,11 @@ def check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
                       allow_quad=False):
     if use_esdg and not (allow_quad and quadrature_tag == DISCR_TAG_QUAD):
         raise NotImplementedError("the entropy-stable (ESDG) operator is built "
-                                  "for euler_operator with "
-                                  "quadrature_tag=DISCR_TAG_QUAD")
+                                  "with quadrature_tag=DISCR_TAG_QUAD")
     if quadrature_tag not in ((DISCR_TAG_BASE, DISCR_TAG_QUAD) if allow_quad
                               else (DISCR_TAG_BASE,)):
-        raise NotImplementedError("overintegration is built for euler_operator "
-                                  "only")
+        raise NotImplementedError(f"quadrature_tag {quadrature_tag!r} is not "
+                                  "supported here")
     if operator_states_quad is not None:
         raise NotImplementedError("operator_states_quad is not supported")
     if limiter_func is not None or entropy_min is not None:

@@ -1643,6 +1643,460 @@ __global__ __launch_bounds__(kBlock) void euler_quad_kernel(KArgs a, QuadArgs qa
 
 // }}}
 
+// {{{ overintegrated Navier-Stokes (ns_operator with a quadrature tag)
+//
+// ns_grad_quad_kernel: grad_cv_operator / grad_t_operator on the points.
+// The conserved state and T at the volume points and at both sides' face
+// points (T from the EOS there; mixtures: Newton from the interpolated
+// nodal T), the central face value (boundaries: the exterior state), then,
+// per (field, element) column, the DIM reference-direction volume
+// contractions and the NF face contractions on the matrix cores, combined
+// per lane with -rx and n into the nodal gradient [NS][DIM][N] (NS = F + 1,
+// T last).  LDS: s_q [F][epb][NP], s_p [F][epb][NF*NFP] the neighbours'
+// face-node values, s_u [NS*epb][gs] the point values (gs >= nq), s_s
+// [NS*epb][fs] fscale times the face values, s_o [NS*DIM][epb][NP] the
+// result, s_t / s_tp the nodal temperatures (mixtures).
+//
+// ns_div_quad_kernel: the layout of euler_quad_kernel with F_I - F_V at the
+// volume points (the nodal gradients interpolated there) and F_I* - F_V* at
+// the face points: the central viscous flux of both sides' states and
+// interpolated gradient traces (boundary points: the exterior state with
+// the interior gradients, which for a dummy boundary -- exterior state =
+// interior state -- is the one-sided flux).  Extra LDS: s_gr [NS*DIM][epb]
+// [NP] the element's gradients, s_gp [NS*DIM][epb][NF*NFP] the neighbours'
+// at this element's face nodes.
+
+template <int DIM>
+__device__ __forceinline__ double ideal_temperature(const double* q, const GasDev& g) {
+  const double irho = 1.0 / q[0];
+  double mm = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) mm += q[2 + d] * q[2 + d];
+  return ((g.gamma - 1.0) / g.gas_const) * (q[1] - 0.5 * mm * irho) * irho;
+}
+
+template <int DIM, int NSP, int MIX>
+__device__ __forceinline__ double quad_point_temperature(const double* q, double seed,
+                                                         const GasDev& g, int niter) {
+  if constexpr (MIX != 0)
+    return point_temperature<DIM, NSP>(q, seed, niter);
+  else
+    return ideal_temperature<DIM>(q, g);
+}
+
+// Phase 1 of both kernels: the elements' state and the neighbours' values
+// at this element's face nodes (interior faces), and, for mixtures, the
+// nodal temperatures of both.
+template <int DIM, int P, int NSP, int MIX>
+__device__ __forceinline__ void quad_load_states(const KArgs& a, const QuadArgs& qa,
+                                                 int64_t e0, int ne, int epb, double* s_q,
+                                                 double* s_p, double* s_t, double* s_tp) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + NSP;
+  const int64_t N = a.nnodes;
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int f = t / (ne * NP);
+    const int r = t - f * ne * NP;
+    s_q[f * epb * NP + r] = a.q[f * N + e0 * NP + r];
+  }
+  for (int t = threadIdx.x; t < ne * NF * NFP; t += blockDim.x) {
+    const int le = t / (NF * NFP);
+    const int fn = t - le * NF * NFP;
+    const int32_t nb = a.vmap_p[(e0 + le) * NF * NFP + fn];
+    if (nb < 0) continue;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_p[(f * epb + le) * NF * NFP + fn] = a.q[f * N + nb];
+    if constexpr (MIX != 0) s_tp[le * NF * NFP + fn] = qa.temp[nb];
+  }
+  if constexpr (MIX != 0) {
+    for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) s_t[t] = qa.temp[e0 * NP + t];
+  }
+}
+
+template <int DIM, int P, int NSP, int MIX>
+__global__ __launch_bounds__(kBlock) void ns_grad_quad_kernel(KArgs a, QuadArgs qa) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + NSP, NS = NFLD + 1;
+  extern __shared__ double lds[];
+  const int nq = qa.nq, nfq = qa.nfq, epb = qa.epb, nfq_all = NF * qa.nfq;
+  const int gs = qa.gs, fs = qa.fs;
+  double* s_q = lds;
+  double* s_p = s_q + NFLD * epb * NP;
+  double* s_u = s_p + NFLD * epb * NF * NFP;
+  double* s_s = s_u + NS * epb * gs;
+  double* s_o = s_s + NS * epb * fs;
+  double* s_t = s_o + NS * DIM * epb * NP;
+  double* s_tp = s_t + epb * NP;
+  const int64_t e0 = int64_t(blockIdx.x) * epb;
+  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
+  const int64_t N = a.nnodes;
+  quad_load_states<DIM, P, NSP, MIX>(a, qa, e0, ne, epb, s_q, s_p, s_t, s_tp);
+  __syncthreads();
+  // 2: state and temperature at the volume points
+  for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
+    const int le = t / nq;
+    const int m = t - le * nq;
+    const double* w = qa.interp + m;
+    double u[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) u[f] = 0.0;
+    double tseed = 0.0;
+#pragma unroll
+    for (int i = 0; i < NP; ++i) {
+      const double wi = w[i * nq];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) u[f] += wi * s_q[(f * epb + le) * NP + i];
+      if constexpr (MIX != 0) tseed += wi * s_t[le * NP + i];
+    }
+    const double tq = quad_point_temperature<DIM, NSP, MIX>(u, tseed, a.gas, qa.niter);
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_u[(f * epb + le) * gs + m] = u[f];
+    s_u[(NFLD * epb + le) * gs + m] = tq;
+  }
+  // 3: the gradient flux value at the face points (central inside, the
+  // exterior value on boundaries), times the face Jacobian ratio
+  for (int t = threadIdx.x; t < ne * nfq_all; t += blockDim.x) {
+    const int le = t / nfq_all;
+    const int rem = t - le * nfq_all;
+    const int fc = rem / nfq;
+    const int m = rem - fc * nfq;
+    const int64_t e = e0 + le;
+    const double* w = qa.finterp + fc * NFP * nfq + m;
+    const int32_t v0 = a.vmap_p[(e * NF + fc) * NFP];
+    const int* fm = a.fmask + fc * NFP;
+    double um[NFLD], up[NFLD];
+    double seed_m = 0.0, seed_p = 0.0, tp;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) um[f] = 0.0;
+#pragma unroll
+    for (int j = 0; j < NFP; ++j) {
+      const double wj = w[j * nfq];
+      const int ij = fm[j];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) um[f] += wj * s_q[(f * epb + le) * NP + ij];
+      if constexpr (MIX != 0) seed_m += wj * s_t[le * NP + ij];
+    }
+    const double tm = quad_point_temperature<DIM, NSP, MIX>(um, seed_m, a.gas, qa.niter);
+    const double sc = a.fscale[e * NF + fc];
+    if (v0 >= 0) {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) up[f] = 0.0;
+#pragma unroll
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f)
+          up[f] += wj * s_p[(f * epb + le) * NF * NFP + fc * NFP + j];
+        if constexpr (MIX != 0) seed_p += wj * s_tp[le * NF * NFP + fc * NFP + j];
+      }
+      tp = quad_point_temperature<DIM, NSP, MIX>(up, seed_p, a.gas, qa.niter);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+        s_s[(f * epb + le) * fs + rem] = sc * ((um[f] + up[f]) / 2);
+      s_s[(NFLD * epb + le) * fs + rem] = sc * ((tm + tp) / 2);
+    } else {
+      const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + m;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) up[f] = qa.qbq[f * qa.nbq + bq];
+      if constexpr (MIX != 0)
+        tp = qa.tbq[bq];
+      else
+        tp = ideal_temperature<DIM>(up, a.gas);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s_s[(f * epb + le) * fs + rem] = sc * up[f];
+      s_s[(NFLD * epb + le) * fs + rem] = sc * tp;
+    }
+  }
+  __syncthreads();
+  // 4: per column j = f * epb + le: D_r = Wq_r u (r < DIM) and L_c = Lq_c
+  // (fscale u*) (c < NF) on the matrix cores (operand layout as in
+  // euler_quad_kernel: each lane's accumulators hold its own column j), then
+  // grad_d = sum_r -rx[r][d] D_r + sum_c n_c[d] L_c with the element's
+  // factors read per lane.
+  {
+    constexpr int RT = (NP + 15) / 16;
+    const int ncol = NS * epb;
+    const int ntile = RT * ((ncol + 15) / 16);
+    const int li = threadIdx.x & 15, kk = (threadIdx.x >> 4) & 3;
+    for (int tile = threadIdx.x >> 6; tile < ntile; tile += blockDim.x >> 6) {
+      const int rt = tile % RT, ct = tile / RT;
+      const int row = rt * 16 + li;
+      const int j = ct * 16 + li;
+      const int lej = j % epb;
+      const bool cok = j < ncol && lej < ne;
+      const int jc = j < ncol ? j : 0;
+      const int r0 = row < NP ? row : 0;
+      const int64_t ej = e0 + (cok ? lej : 0);
+      doublex4 g[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) g[d] = doublex4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) {
+        doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+        quad_mfma_panel<NP>(qa.wdiff + r * nq * NP + r0, s_u + jc * gs, nq, kk, acc0, acc1);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const double c = -a.rx[ej * DIM * DIM + r * DIM + d];
+          g[d] += c * (acc0 + acc1);
+        }
+      }
+#pragma unroll
+      for (int fc = 0; fc < NF; ++fc) {
+        doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+        quad_mfma_panel<NP>(qa.lift + fc * nfq * NP + r0, s_s + jc * fs + fc * nfq, nfq, kk,
+                            acc0, acc1);
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          const double c = a.normals[(ej * NF + fc) * DIM + d];
+          g[d] += c * (acc0 + acc1);
+        }
+      }
+      if (cok) {
+        const int f = j / epb;
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const int i = rt * 16 + 4 * q4 + kk;
+          if (i < NP) {
+#pragma unroll
+            for (int d = 0; d < DIM; ++d)
+              s_o[((f * DIM + d) * epb + lej) * NP + i] = g[d][q4];
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NS * DIM * ne * NP; t += blockDim.x) {
+    const int k = t / (ne * NP);
+    const int r = t - k * ne * NP;
+    a.out[k * N + e0 * NP + r] = s_o[k * epb * NP + r];
+  }
+}
+
+template <int DIM, int P, int NSP, int MIX>
+__global__ __launch_bounds__(kBlock) void ns_div_quad_kernel(KArgs a, QuadArgs qa) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + NSP, NS = NFLD + 1, NSD = NS * DIM;
+  extern __shared__ double lds[];
+  const int nq = qa.nq, nfq = qa.nfq, epb = qa.epb, nfq_all = NF * qa.nfq;
+  const int gs = qa.gs, fs = qa.fs;
+  double* s_q = lds;
+  double* s_g = s_q + NFLD * epb * NP;
+  double* s_f = s_g + NFLD * epb * gs;
+  double* s_p = s_f + NFLD * epb * fs;
+  double* s_gr = s_p + NFLD * epb * NF * NFP;
+  double* s_gp = s_gr + NSD * epb * NP;
+  double* s_t = s_gp + NSD * epb * NF * NFP;
+  double* s_tp = s_t + epb * NP;
+  const int64_t e0 = int64_t(blockIdx.x) * epb;
+  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
+  const int64_t N = a.nnodes;
+  quad_load_states<DIM, P, NSP, MIX>(a, qa, e0, ne, epb, s_q, s_p, s_t, s_tp);
+  for (int t = threadIdx.x; t < NSD * ne * NP; t += blockDim.x) {
+    const int k = t / (ne * NP);
+    const int r = t - k * ne * NP;
+    s_gr[k * epb * NP + r] = a.grad[k * N + e0 * NP + r];
+  }
+  for (int t = threadIdx.x; t < ne * NF * NFP; t += blockDim.x) {
+    const int le = t / (NF * NFP);
+    const int fn = t - le * NF * NFP;
+    const int32_t nb = a.vmap_p[(e0 + le) * NF * NFP + fn];
+    if (nb < 0) continue;
+#pragma unroll
+    for (int k = 0; k < NSD; ++k) s_gp[(k * epb + le) * NF * NFP + fn] = a.grad[k * N + nb];
+  }
+  __syncthreads();
+  // 2: F_I - F_V at the volume points, in reference directions (the node
+  // loops stay rolled: unrolled, the interpolation's LDS reads are hoisted
+  // and spill)
+  for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
+    const int le = t / nq;
+    const int m = t - le * nq;
+    const double* w = qa.interp + m;
+    Node<DIM, NSP> s;
+    double gacc[NSD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s.q[f] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NSD; ++k) gacc[k] = 0.0;
+    double tseed = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < NP; ++i) {
+      const double wi = w[i * nq];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s.q[f] += wi * s_q[(f * epb + le) * NP + i];
+#pragma unroll
+      for (int k = 0; k < NSD; ++k) gacc[k] += wi * s_gr[(k * epb + le) * NP + i];
+      if constexpr (MIX != 0) tseed += wi * s_t[le * NP + i];
+    }
+    double tq = 0.0;
+    if constexpr (MIX != 0) tq = point_temperature<DIM, NSP>(s.q, tseed, qa.niter);
+    node_state<DIM, NSP, MIX>(s, a.gas, tq);
+    double F[NFLD][DIM];
+    viscous_flux<DIM, NSP, MIX>(s, reinterpret_cast<const double(*)[DIM]>(gacc),
+                                gacc + NFLD * DIM, F);
+    if (a.inviscid_on) {
+      double FI[NFLD][DIM];
+      inviscid_flux<DIM, NSP>(s, FI);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) F[f][d] = FI[f][d] - F[f][d];
+    } else {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) F[f][d] = -F[f][d];
+    }
+    const double* rx = a.rx + (e0 + le) * DIM * DIM;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f)
+#pragma unroll
+      for (int r = 0; r < DIM; ++r) {
+        double g = 0.0;
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) g += rx[r * DIM + d] * F[f][d];
+        s_g[(f * epb + le) * gs + r * nq + m] = g;
+      }
+  }
+  // 3: F_I* - F_V* at the face points: the own side's state and viscous
+  // flux first, then the other side's, so that one gradient set is live
+  for (int t = threadIdx.x; t < ne * nfq_all; t += blockDim.x) {
+    const int le = t / nfq_all;
+    const int rem = t - le * nfq_all;
+    const int fc = rem / nfq;
+    const int m = rem - fc * nfq;
+    const int64_t e = e0 + le;
+    const double* w = qa.finterp + fc * NFP * nfq + m;
+    const int32_t v0 = a.vmap_p[(e * NF + fc) * NFP];
+    const int* fm = a.fmask + fc * NFP;
+    double n[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
+    Node<DIM, NSP> sm, sp;
+    double fstar[NFLD], fv[NFLD];
+    {
+      double seed_m = 0.0, tm = 0.0;
+      double gacc[NSD];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) sm.q[f] = 0.0;
+#pragma unroll
+      for (int k = 0; k < NSD; ++k) gacc[k] = 0.0;
+#pragma unroll 1
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+        const int ij = fm[j];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) sm.q[f] += wj * s_q[(f * epb + le) * NP + ij];
+#pragma unroll
+        for (int k = 0; k < NSD; ++k) gacc[k] += wj * s_gr[(k * epb + le) * NP + ij];
+        if constexpr (MIX != 0) seed_m += wj * s_t[le * NP + ij];
+      }
+      if constexpr (MIX != 0) tm = point_temperature<DIM, NSP>(sm.q, seed_m, qa.niter);
+      node_state<DIM, NSP, MIX>(sm, a.gas, tm);
+      viscous_flux_n<DIM, NSP, MIX>(sm, reinterpret_cast<const double(*)[DIM]>(gacc),
+                                    gacc + NFLD * DIM, n, fv);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) fstar[f] = -0.5 * fv[f];
+    }
+    double tp = 0.0;
+    if (v0 >= 0) {
+      double seed_p = 0.0;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) sp.q[f] = 0.0;
+#pragma unroll 1
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f)
+          sp.q[f] += wj * s_p[(f * epb + le) * NF * NFP + fc * NFP + j];
+        if constexpr (MIX != 0) seed_p += wj * s_tp[le * NF * NFP + fc * NFP + j];
+      }
+      if constexpr (MIX != 0) tp = point_temperature<DIM, NSP>(sp.q, seed_p, qa.niter);
+    } else {
+      const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + m;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) sp.q[f] = qa.qbq[f * qa.nbq + bq];
+      if constexpr (MIX != 0) tp = qa.tbq[bq];
+    }
+    node_state<DIM, NSP, MIX>(sp, a.gas, tp);
+    if (a.inviscid_on) {
+      inviscid_numflux<DIM, NSP>(sm, sp, n, a.flux_kind, fv);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) fstar[f] += fv[f];
+    }
+    {
+      // the other side's gradient trace: the neighbour's (interior faces)
+      // or this element's own (boundary faces)
+      double gacc[NSD];
+#pragma unroll
+      for (int k = 0; k < NSD; ++k) gacc[k] = 0.0;
+      if (v0 >= 0) {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+          const double wj = w[j * nfq];
+#pragma unroll
+          for (int k = 0; k < NSD; ++k)
+            gacc[k] += wj * s_gp[(k * epb + le) * NF * NFP + fc * NFP + j];
+        }
+      } else {
+#pragma unroll 1
+        for (int j = 0; j < NFP; ++j) {
+          const double wj = w[j * nfq];
+          const int ij = fm[j];
+#pragma unroll
+          for (int k = 0; k < NSD; ++k) gacc[k] += wj * s_gr[(k * epb + le) * NP + ij];
+        }
+      }
+      viscous_flux_n<DIM, NSP, MIX>(sp, reinterpret_cast<const double(*)[DIM]>(gacc),
+                                    gacc + NFLD * DIM, n, fv);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) fstar[f] -= 0.5 * fv[f];
+    }
+    const double sc = a.fscale[e * NF + fc];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_f[(f * epb + le) * fs + rem] = -sc * fstar[f];
+  }
+  __syncthreads();
+  // 4: the contraction of euler_quad_kernel
+  {
+    constexpr int RT = (NP + 15) / 16;
+    const int ncol = NFLD * epb;
+    const int ntile = RT * ((ncol + 15) / 16);
+    const int li = threadIdx.x & 15, kk = (threadIdx.x >> 4) & 3;
+    for (int tile = threadIdx.x >> 6; tile < ntile; tile += blockDim.x >> 6) {
+      const int rt = tile % RT, ct = tile / RT;
+      const int row = rt * 16 + li;
+      const int j = ct * 16 + li;
+      const int lej = j % epb;
+      const bool rok = row < NP, cok = j < ncol && lej < ne;
+      doublex4 acc0 = {0.0, 0.0, 0.0, 0.0}, acc1 = acc0;
+      const int jc = j < ncol ? j : 0;
+      const int r0 = rok ? row : 0;
+      quad_mfma_panel<NP>(qa.wdiff + r0, s_g + jc * gs, DIM * nq, kk, acc0, acc1);
+      quad_mfma_panel<NP>(qa.lift + r0, s_f + jc * fs, nfq_all, kk, acc0, acc1);
+      if (cok) {
+#pragma unroll
+        for (int q4 = 0; q4 < 4; ++q4) {
+          const int i = rt * 16 + 4 * q4 + kk;
+          if (i < NP) s_q[j * NP + i] = acc0[q4] + acc1[q4];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int f = t / (ne * NP);
+    const int r = t - f * ne * NP;
+    a.out[f * N + e0 * NP + r] = s_q[f * epb * NP + r];
+  }
+}
+
+// }}}
+
 // {{{ entropy-stable Euler RHS (flux differencing, ideal gas)
 //
 // esdg_project_kernel: projected entropy variables v~ = P_q v(V_q u) per

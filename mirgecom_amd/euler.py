@@ -67,6 +67,25 @@ def euler_operator(dcoll, state, gas_model, boundaries, time=0.0,
     return ConservedVars(state.dim, rhs)
 
 
+def entropy_stable_euler_operator(
+        dcoll, gas_model, state, boundaries, time=0.0,
+        inviscid_numerical_flux_func=None, entropy_conserving_flux_func=None,
+        operator_states_quad=None, dd=DD_VOLUME_ALL, quadrature_tag=None,
+        comm_tag=None, entropy_min=None, limiter_func=None):
+    """mirgecom/euler.py:108-310: the flux-differencing RHS, i.e.
+    ``euler_operator(..., use_esdg=True)`` (``mg_euler_rhs_esdg``); built for
+    the ideal gas with ``quadrature_tag=DISCR_TAG_QUAD``."""
+    if quadrature_tag is None:
+        quadrature_tag = DISCR_TAG_BASE
+    return euler_operator(
+        dcoll, state, gas_model, boundaries, time=time,
+        inviscid_numerical_flux_func=inviscid_numerical_flux_func,
+        quadrature_tag=quadrature_tag, dd=dd, comm_tag=comm_tag, use_esdg=True,
+        operator_states_quad=operator_states_quad,
+        entropy_conserving_flux_func=entropy_conserving_flux_func,
+        limiter_func=limiter_func, entropy_min=entropy_min)
+
+
 def _euler_quad(dcoll, state, gas_model, boundaries, time, kind):
     from mirgecom_amd._opcommon import quad_boundary_states
     from mirgecom_amd.device import get_device_discretization
