@@ -719,7 +719,11 @@ def measure_ensemble(args, model):
 def measure_overintegration(args, dcoll, gm, cv):
     """Euler RHS of the same state on the nodal and on the quadrature
     (degree 2p) discretization, and the entropy-stable flux-differencing
-    operator on that quadrature, FP64."""
+    operator on that quadrature; the Navier-Stokes RHS on the quadrature
+    (gradients + divergence) and its entropy-stable split (the viscous
+    terms on the quadrature plus the flux-differencing Euler RHS); FP64."""
+    import warnings
+
     import torch
 
     from mirgecom_amd import _lib
@@ -728,10 +732,18 @@ def measure_overintegration(args, dcoll, gm, cv):
     from mirgecom_amd.gas_model import make_fluid_state
     from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
     from mirgecom_amd.inviscid import entropy_stable_inviscid_facial_flux_rusanov
+    from mirgecom_amd.navierstokes import ns_operator
     st = make_fluid_state(cv, _inviscid_model(gm))
+    st_ns = make_fluid_state(cv, gm)
     out = {}
-    for tag in (DISCR_TAG_BASE, DISCR_TAG_QUAD, "esdg"):
+    for tag in (DISCR_TAG_BASE, DISCR_TAG_QUAD, "esdg", "ns_quad", "ns_esdg"):
         def go():
+            if tag in ("ns_quad", "ns_esdg"):
+                with warnings.catch_warnings():
+                    warnings.simplefilter("ignore")
+                    return ns_operator(dcoll, gm, st_ns, {},
+                                       use_esdg=tag == "ns_esdg",
+                                       quadrature_tag=DISCR_TAG_QUAD)
             if tag == "esdg":
                 return euler_operator(
                     dcoll, st, st.gas_model, {}, use_esdg=True,
@@ -759,7 +771,15 @@ def measure_overintegration(args, dcoll, gm, cv):
            "quad_ms": out[DISCR_TAG_QUAD].get("mg_euler_rhs_quad"),
            "esdg_ms": out["esdg"].get("mg_euler_rhs_esdg"),
            "esdg_workload": "entropy-stable flux differencing on the same "
-                            "quadrature (entropy-stable Rusanov faces)"}
+                            "quadrature (entropy-stable Rusanov faces)",
+           "ns_quad_ms": {k: out["ns_quad"].get(k) for k in
+                          ("mg_ns_gradient_quad", "mg_ns_rhs_quad")},
+           "ns_esdg_ms": {k: out["ns_esdg"].get(k) for k in
+                          ("mg_ns_gradient_quad", "mg_ns_rhs_quad",
+                           "mg_euler_rhs_esdg")},
+           "ns_workload": "Navier-Stokes RHS of the same state on the "
+                          "quadrature (Rusanov); the entropy-stable split: "
+                          "viscous terms on the quadrature + flux differencing"}
     print(f"# overintegration: {json.dumps(res)}", file=sys.stderr, flush=True)
     return res
 

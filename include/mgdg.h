@@ -377,6 +377,31 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
                             const double* q_bnd_quad, int32_t dissipation,
                             double* ev_work, double* rhs, void* stream);
 
+/* Entropy-stable Euler RHS for the uiuc_7sp mixture (euler_operator with
+ * use_esdg=True and quadrature_tag=DISCR_TAG_QUAD, mirgecom/euler.py:108-310,
+ * with Renac's entropy-conserving two-point flux, mirgecom/inviscid.py:
+ * 521-598).  Volume points: the interpolated state and its temperature by
+ * niter Newton steps from the interpolated nodal temperature [N]; with
+ * es_volume != 0 taken through its entropy variables (with the state's own
+ * p and gamma) and back and made again (project_fluid_state(...,
+ * entropy_stable=True), gas_model.py:494-588; euler_operator's path), else
+ * as it is (the states ns_operator hands to its inviscid operator).  Its
+ * entropy variables are L2-projected into ev_work [F][N].  Hybridized points:
+ * conserved variables from the projected variables and the interpolated
+ * nodal gamma(cv, T) (gas_model.py:1000-1023), T by Newton from the
+ * interpolated nodal T; interior faces: the neighbour's projected variables
+ * with this element's gamma and the neighbour's T as the seed; boundary
+ * faces: the entropy-stable projected interior state and q_bnd_quad with its
+ * temperatures t_bnd_quad [nbnd_faces*nfq].  Face flux: Renac's flux . n,
+ * with dissipation != 0 minus alpha/2 (q+ - q-), alpha the larger |u| + c.
+ * 2-D p = 1..4, 3-D p = 1..3. */
+mg_status mg_euler_rhs_esdg_mixture(const mg_mesh* mesh, const mg_gas* gas,
+                                    const mg_quad* quad, const mg_esdg* esdg,
+                                    const double* q, const double* temperature,
+                                    const double* q_bnd_quad, const double* t_bnd_quad,
+                                    int32_t niter, int32_t dissipation, int32_t es_volume,
+                                    double* ev_work, double* rhs, void* stream);
+
 /* Allocation-free variant of mg_euler_rhs_esdg.  No counterpart in the
  * reference: it is for callers that cannot allocate device memory (code under
  * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).

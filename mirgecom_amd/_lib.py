@@ -174,6 +174,11 @@ SIGNATURES = {
     "mg_euler_rhs_esdg": (_I32, [ctypes.POINTER(MgMesh), ctypes.POINTER(MgGas),
                                  ctypes.POINTER(MgQuad), ctypes.POINTER(MgEsdg),
                                  _P, _P, _I32, _P, _P, _P]),
+    "mg_euler_rhs_esdg_mixture": (_I32, [ctypes.POINTER(MgMesh),
+                                         ctypes.POINTER(MgGas),
+                                         ctypes.POINTER(MgQuad),
+                                         ctypes.POINTER(MgEsdg), _P, _P, _P, _P,
+                                         _I32, _I32, _I32, _P, _P, _P]),
     "mg_euler_rhs_esdg_noalloc": (_I32, [ctypes.POINTER(MgMesh),
                                          ctypes.POINTER(MgGas),
                                          ctypes.POINTER(MgQuad),

@@ -1,5 +1,6 @@
 // Quadrature (overintegration) and entropy-stable Euler entry points of
-// libmgdg (include/mgdg.h): mg_euler_rhs_quad, mg_euler_rhs_esdg.
+// libmgdg (include/mgdg.h): mg_euler_rhs_quad, mg_euler_rhs_esdg,
+// mg_euler_rhs_esdg_mixture, mg_euler_rhs_esdg_noalloc.
 #include <algorithm>
 #include <mutex>
 #include <string>
@@ -189,6 +190,95 @@ mg_status esdg_run(const mg_mesh* mesh, const mg_gas* gas, const mg_quad* quad,
   return fail(MG_ERR_UNSUPPORTED, "dim/order");
 }
 
+// The uiuc_7sp mixture (Renac's two-point flux): esdg_project_mix_kernel
+// then esdg_mix_kernel (row-by-row phase 4).
+mg_status esdg_mix_run(const mg_mesh* mesh, const mg_gas* gas, const mg_quad* quad,
+                       const mg_esdg* esdg, const double* q, const double* temperature,
+                       const double* q_bnd_quad, const double* t_bnd_quad, int32_t niter,
+                       int32_t dissipation, int32_t es_volume, double* ev_work, double* rhs,
+                       void* stream) {
+  mg_status s = check_mesh(mesh);
+  if (s) return s;
+  if ((s = check_gas(gas, false))) return s;
+  if (gas->eos_kind != 1)
+    return fail(MG_ERR_ARG, "mg_euler_rhs_esdg_mixture needs the mixture EOS");
+  if ((s = ensure_mech())) return s;
+  if (!quad || !quad->interp || !quad->face_interp || quad->nq < 1 || quad->nfq < 1)
+    return fail(MG_ERR_ARG, "mg_quad incomplete");
+  if (!esdg || !esdg->proj || !esdg->vh || !esdg->skew || !esdg->ext || !esdg->ext_col ||
+      !esdg->apply || !esdg->face_weights ||
+      esdg->nh != quad->nq + mesh->nfaces * quad->nfq)
+    return fail(MG_ERR_ARG, "mg_esdg incomplete or nh != nq + nfaces*nfq");
+  if (!q || !rhs || !temperature || !ev_work)
+    return fail(MG_ERR_ARG, "q/temperature/ev_work/rhs NULL");
+  if (niter < 0) return fail(MG_ERR_ARG, "niter < 0");
+  if (mesh->nbnd_nodes > 0 && (!q_bnd_quad || !t_bnd_quad))
+    return fail(MG_ERR_ARG, "boundary exterior state / temperature missing");
+  if (mesh->nbnd_nodes % mesh->nfp) return fail(MG_ERR_ARG, "nbnd_nodes % nfp");
+  const int nfld = 2 + mesh->dim + gas->nspecies;
+  mgdg::KArgs a = make_args(mesh, gas);
+  a.q = q;
+  a.temp = temperature;
+  a.out = rhs;
+  mgdg::EsdgArgs ea{};
+  ea.nq = quad->nq;
+  ea.nfq = quad->nfq;
+  ea.nh = esdg->nh;
+  ea.interp = quad->interp;
+  ea.finterp = quad->face_interp;
+  ea.proj = esdg->proj;
+  ea.vh = esdg->vh;
+  ea.skew = esdg->skew;
+  ea.ext = esdg->ext;
+  ea.ext_col = esdg->ext_col;
+  ea.apply = esdg->apply;
+  ea.wf = esdg->face_weights;
+  ea.qbq = q_bnd_quad;
+  ea.tbq = t_bnd_quad;
+  ea.nbq = mesh->nbnd_nodes / mesh->nfp * quad->nfq;
+  ea.ev = ev_work;
+  ea.dissipation = dissipation ? 1 : 0;
+  ea.niter = niter;
+  ea.es_volume = es_volume ? 1 : 0;
+  ea.paired = 0;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  constexpr size_t kBudget = 128 * 1024;
+  const size_t nft = size_t(mesh->nfaces) * quad->nfq;
+  const size_t nfn = size_t(mesh->nfaces) * mesh->nfp;
+#define MG_MCASE(D, P)                                                                  \
+  if (mesh->dim == D && mesh->order == P) {                                             \
+    const size_t per_p = sizeof(double) * nfld * quad->nq;                              \
+    if (per_p > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
+    int epb_p = int(std::max<size_t>(1, std::min<size_t>(8, 256 / quad->nq)));        \
+    ea.epb = epb_p;                                                                     \
+    auto kp = mgdg::esdg_project_mix_kernel<D, P>;                                      \
+    if ((s = set_lds(kp, kBudget))) return s;                                           \
+    hipLaunchKernelGGL(kp, dim3(unsigned((a.K + epb_p - 1) / epb_p)),                   \
+                       dim3(mgdg::kBlock), per_p * epb_p, st, a, ea);                  \
+    if ((s = check_launch("esdg_project_mix_kernel"))) return s;                        \
+    const size_t nchunk = (size_t(esdg->nh) + quad->nq - 1) / quad->nq;               \
+    const size_t nacc = nchunk * quad->nq + nft;                                        \
+    const size_t per_e = sizeof(mgdg::RnPair<D>) * esdg->nh +                           \
+        sizeof(double) * (nfld * (2 * size_t(mesh->np) + nfn + nft + nacc) +            \
+                          2 * size_t(mesh->np) + nfn + 2 * nft);                        \
+    if (per_e > kBudget) return fail(MG_ERR_UNSUPPORTED, "quadrature too large for LDS"); \
+    int epb = int(std::max<size_t>(1, std::min<size_t>(16, (64 * 1024) / per_e)));     \
+    ea.epb = epb;                                                                       \
+    auto kern = mgdg::esdg_mix_kernel<D, P>;                                            \
+    if ((s = set_lds(kern, kBudget))) return s;                                         \
+    hipLaunchKernelGGL(kern, dim3(unsigned((a.K + epb - 1) / epb)), dim3(mgdg::kBlock), \
+                       per_e * epb, st, a, ea);                                         \
+    return check_launch("esdg_mix_kernel");                                             \
+  }
+  if (gas->nspecies != 7) return fail(MG_ERR_UNSUPPORTED, "the mixture has 7 species");
+  MG_MCASE(2, 1) MG_MCASE(2, 2) MG_MCASE(2, 3) MG_MCASE(2, 4)
+  MG_MCASE(3, 1) MG_MCASE(3, 2) MG_MCASE(3, 3)
+#undef MG_MCASE
+  if (mesh->dim == 3 && mesh->order == 4)
+    return fail(MG_ERR_UNSUPPORTED, "entropy-stable path: 3-D p = 4 is not supported");
+  return fail(MG_ERR_UNSUPPORTED, "dim/order");
+}
+
 // {{{ allocation-free variant: the projected entropy variables in `scratch`
 //
 // `scratch` is a file-scope device array: each GPU that loads the library
@@ -239,6 +329,16 @@ mg_status mg_euler_rhs_esdg(const mg_mesh* mesh, const mg_gas* gas,
                             double* ev_work, double* rhs, void* stream) {
   return esdg_run(mesh, gas, quad, esdg, q, q_bnd_quad, dissipation, ev_work, rhs,
                   stream);
+}
+
+mg_status mg_euler_rhs_esdg_mixture(const mg_mesh* mesh, const mg_gas* gas,
+                                    const mg_quad* quad, const mg_esdg* esdg,
+                                    const double* q, const double* temperature,
+                                    const double* q_bnd_quad, const double* t_bnd_quad,
+                                    int32_t niter, int32_t dissipation, int32_t es_volume,
+                                    double* ev_work, double* rhs, void* stream) {
+  return esdg_mix_run(mesh, gas, quad, esdg, q, temperature, q_bnd_quad, t_bnd_quad,
+                      niter, dissipation, es_volume, ev_work, rhs, stream);
 }
 
 mg_status mg_euler_rhs_esdg_noalloc(const mg_mesh* mesh, const mg_gas* gas,

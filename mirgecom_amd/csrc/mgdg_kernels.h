@@ -2129,6 +2129,13 @@ struct EsdgArgs {
   int paired;
   const double* skew_diag;  // [DIM][nq(s)][nq(h)]: (Q_r - Q_r^T)[h][(h+s) % nq]
   const double* ext_diag;   // [nq(s)][NF*nfq(m)]: E[m][(m+s) % nq]
+  // mixtures: Newton steps per point, exterior temperatures at the
+  // boundary points [nbq]; es_volume: the volume points' state is the
+  // entropy-stable projected one (euler_operator) rather than the plain
+  // projected one (the states ns_operator hands to its inviscid operator)
+  int niter;
+  const double* tbq;
+  int es_volume;
 };
 
 template <int DIM, int P, int NSP>
@@ -2351,6 +2358,141 @@ __device__ __forceinline__ void esdg_face_flux(const KArgs& a, const EsdgArgs& e
   for (int f = 0; f < NFLD; ++f) out[f] = sc * fs[f];
 }
 
+// Phase 4 of the entropy-stable kernels (row by row): each work item sums
+// one row piece of (Q_h - Q_h^T) o F_S with the two-point flux `flux`
+// (called as flux(point_h, point_j, direction, acc)); face rows add the
+// face flux s_fs.  Slots in s_acc as described in esdg_kernel.
+template <int DIM, int NFLD, typename Pair, typename Flux>
+__device__ __forceinline__ void esdg_rows(const KArgs& a, const EsdgArgs& ea,
+                                          const Pair* s_prim, const double* s_fs,
+                                          double* s_acc, int64_t e0, int ne, int nacc,
+                                          int C, Flux flux) {
+  constexpr int NF = DIM + 1;
+  const int nq = ea.nq, nfq = ea.nfq, nh = ea.nh, epb = ea.epb, nft = NF * ea.nfq;
+  const int nitems = nq * C + nft;
+  for (int t = threadIdx.x; t < ne * nitems; t += blockDim.x) {
+    const int le = t / nitems;
+    const int it = t - le * nitems;
+    int h, c, j0, j1;
+    if (it < nq * C) {
+      c = it / nq;
+      h = it - c * nq;
+      j0 = c * nq;
+      j1 = j0 + nq < nh ? j0 + nq : nh;
+    } else {
+      c = 0;
+      h = nq + (it - nq * C);
+      j0 = 0;
+      j1 = nq;
+    }
+    const int64_t e = e0 + le;
+    const Pair ph = s_prim[le * nh + h];
+    const Pair* pj = s_prim + le * nh;
+    double acc[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
+    if (h < nq) {
+      double rx[DIM * DIM];
+#pragma unroll
+      for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
+      const int jv = j1 < nq ? j1 : nq;
+      for (int j = j0; j < jv; ++j) {
+        double sr[DIM], sd[DIM];
+#pragma unroll
+        for (int r = 0; r < DIM; ++r) sr[r] = ea.skew[(r * nq + j) * nq + h];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) {
+          double v = 0.0;
+#pragma unroll
+          for (int r = 0; r < DIM; ++r) v += rx[r * DIM + d] * sr[r];
+          sd[d] = v;
+        }
+        flux(ph, pj[j], sd, acc);
+      }
+      for (int j = (j0 > nq ? j0 : nq); j < j1; ++j) {
+        const int m = j - nq;
+        const int fc = m / nfq;
+        const double b = ea.ext[m * nq + h] * a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
+        double sd[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sd[d] = b * a.normals[(e * NF + fc) * DIM + d];
+        flux(ph, pj[j], sd, acc);
+      }
+    } else {
+      const int m = h - nq;
+      const int fc = m / nfq;
+      const double bsc = -a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
+      double nrm[DIM];
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) nrm[d] = bsc * a.normals[(e * NF + fc) * DIM + d];
+      for (int j = 0; j < nq; ++j) {
+        const double ej = ea.ext_col[j * nft + m];
+        double sd[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; ++d) sd[d] = ej * nrm[d];
+        flux(ph, pj[j], sd, acc);
+      }
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) acc[f] += s_fs[(f * epb + le) * nft + m];
+    }
+    const int slot = h < nq ? c * nq + h : C * nq + (h - nq);
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_acc[(f * epb + le) * nacc + slot] = acc[f];
+  }
+}
+
+// Phase 5 of the entropy-stable kernels: dQ/dt = -M^-1 V_h^T acc.
+template <int DIM, int P, int NFLD>
+__device__ __forceinline__ void esdg_apply(const KArgs& a, const EsdgArgs& ea,
+                                           double* s_acc, int64_t e0, int ne,
+                                           int nacc, int C) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NF = E::NF;
+  const int nq = ea.nq, epb = ea.epb, nft = NF * ea.nfq;
+  const int64_t N = a.nnodes;
+  // the volume rows' pieces summed in place,
+  // then one (field, element, node) output per thread
+  if (C > 1) {
+    for (int t = threadIdx.x; t < NFLD * ne * nq; t += blockDim.x) {
+      const int fl = t / nq;
+      const int h = t - fl * nq;
+      const int f = fl / ne, le = fl - f * ne;
+      double* sa = s_acc + (f * epb + le) * nacc + h;
+      double v = sa[0];
+      for (int cc = 1; cc < C; ++cc) v += sa[cc * nq];
+      sa[0] = v;
+    }
+    __syncthreads();
+  }
+  for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
+    const int fl = t / NP;
+    const int i = t - fl * NP;
+    const int f = fl / ne, le = fl - f * ne;
+    const double* sa = s_acc + (f * epb + le) * nacc;
+    const double* w = ea.apply + i;
+    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
+    int h = 0;
+    for (; h + 4 <= nq; h += 4) {
+      o0 += w[h * NP] * sa[h];
+      o1 += w[(h + 1) * NP] * sa[h + 1];
+      o2 += w[(h + 2) * NP] * sa[h + 2];
+      o3 += w[(h + 3) * NP] * sa[h + 3];
+    }
+    for (; h < nq; ++h) o0 += w[h * NP] * sa[h];
+    const double* sf = sa + C * nq;
+    const double* wf = w + nq * NP;
+    int m = 0;
+    for (; m + 4 <= nft; m += 4) {
+      o0 += wf[m * NP] * sf[m];
+      o1 += wf[(m + 1) * NP] * sf[m + 1];
+      o2 += wf[(m + 2) * NP] * sf[m + 2];
+      o3 += wf[(m + 3) * NP] * sf[m + 3];
+    }
+    for (; m < nft; ++m) o0 += wf[m * NP] * sf[m];
+    a.out[f * N + (e0 + le) * NP + i] = -((o0 + o1) + (o2 + o3));
+  }
+}
+
 template <int DIM, int P, int NSP>
 __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   using E = Elem<DIM, P>;
@@ -2442,119 +2584,281 @@ __global__ __launch_bounds__(kBlock) void esdg_kernel(KArgs a, EsdgArgs ea) {
   if (ea.paired) {
     esdg_pairs_64<DIM, NSP>(a, ea, s_prim, fkeep, s_acc, e0, nacc, gamma);
   } else {
-  const int nitems = nq * C + nft;
-  for (int t = threadIdx.x; t < ne * nitems; t += blockDim.x) {
-    const int le = t / nitems;
-    const int it = t - le * nitems;
-    int h, c, j0, j1;
-    if (it < nq * C) {
-      c = it / nq;
-      h = it - c * nq;
-      j0 = c * nq;
-      j1 = j0 + nq < nh ? j0 + nq : nh;
-    } else {
-      c = 0;
-      h = nq + (it - nq * C);
-      j0 = 0;
-      j1 = nq;
-    }
-    const int64_t e = e0 + le;
-    const Pair ph = s_prim[le * nh + h];
-    const Pair* pj = s_prim + le * nh;
-    double acc[NFLD];
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) acc[f] = 0.0;
-    if (h < nq) {
-      double rx[DIM * DIM];
-#pragma unroll
-      for (int k = 0; k < DIM * DIM; ++k) rx[k] = a.rx[e * DIM * DIM + k];
-      const int jv = j1 < nq ? j1 : nq;
-      for (int j = j0; j < jv; ++j) {
-        double sr[DIM], sd[DIM];
-#pragma unroll
-        for (int r = 0; r < DIM; ++r) sr[r] = ea.skew[(r * nq + j) * nq + h];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) {
-          double v = 0.0;
-#pragma unroll
-          for (int r = 0; r < DIM; ++r) v += rx[r * DIM + d] * sr[r];
-          sd[d] = v;
-        }
-        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
-      }
-      for (int j = (j0 > nq ? j0 : nq); j < j1; ++j) {
-        const int m = j - nq;
-        const int fc = m / nfq;
-        const double b = ea.ext[m * nq + h] * a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
-        double sd[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) sd[d] = b * a.normals[(e * NF + fc) * DIM + d];
-        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
-      }
-    } else {
-      const int m = h - nq;
-      const int fc = m / nfq;
-      const double bsc = -a.fscale[e * NF + fc] * ea.wf[m - fc * nfq];
-      double nrm[DIM];
-#pragma unroll
-      for (int d = 0; d < DIM; ++d) nrm[d] = bsc * a.normals[(e * NF + fc) * DIM + d];
-      for (int j = 0; j < nq; ++j) {
-        const double ej = ea.ext_col[j * nft + m];
-        double sd[DIM];
-#pragma unroll
-        for (int d = 0; d < DIM; ++d) sd[d] = ej * nrm[d];
-        ec_flux_dot<DIM, NSP>(ph, pj[j], sd, gamma, acc);
-      }
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) acc[f] += s_fs[(f * epb + le) * nft + m];
-    }
-    const int slot = h < nq ? c * nq + h : C * nq + (h - nq);
-#pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_acc[(f * epb + le) * nacc + slot] = acc[f];
-  }
+    esdg_rows<DIM, NFLD>(a, ea, s_prim, s_fs, s_acc, e0, ne, nacc, C,
+                         [gamma](const Pair& x, const Pair& y, const double* sd,
+                                 double* acc) {
+                           ec_flux_dot<DIM, NSP>(x, y, sd, gamma, acc);
+                         });
   }
   __syncthreads();
-  // 5: dQ/dt = -M^-1 V_h^T acc: the volume rows' pieces summed in place,
-  // then one (field, element, node) output per thread
-  if (C > 1) {
-    for (int t = threadIdx.x; t < NFLD * ne * nq; t += blockDim.x) {
-      const int fl = t / nq;
-      const int h = t - fl * nq;
-      const int f = fl / ne, le = fl - f * ne;
-      double* sa = s_acc + (f * epb + le) * nacc + h;
-      double v = sa[0];
-      for (int cc = 1; cc < C; ++cc) v += sa[cc * nq];
-      sa[0] = v;
+  // 5: dQ/dt = -M^-1 V_h^T acc
+  esdg_apply<DIM, P, NFLD>(a, ea, s_acc, e0, ne, nacc, C);
+}
+
+// }}}
+
+// {{{ entropy-stable Euler RHS for the uiuc_7sp mixture (Renac's flux)
+//
+// The reference's mixture path (mirgecom/euler.py:108-310 with
+// project_fluid_state(entropy_stable=True), gas_model.py:494-588, and
+// make_entropy_projected_fluid_state, gas_model.py:1000-1023):
+// esdg_project_mix_kernel: at the volume points the interpolated state, its
+// entropy variables with the state's own p and gamma and back (the
+// "entropy-stable" projected state: energy p/(gamma-1) + rho|u|^2/2), the
+// state made again; its entropy variables L2-projected to ev.
+// esdg_mix_kernel: at the hybridized points the conserved state from the
+// projected variables and the interpolated nodal gamma(cv, T), T by Newton
+// from the interpolated nodal T; the neighbours' face states from their
+// projected variables with this element's interpolated gamma and their
+// nodal T as the seed; boundary faces the entropy-stable projected interior
+// state and the condition's exterior state; Renac's two-point flux; phases
+// 4-5 as esdg_kernel (row by row).
+
+// The "entropy-stable" projected state of conserved values u with Newton
+// seed `seed` (project_fluid_state(..., entropy_stable=True)).
+template <int DIM>
+__device__ __forceinline__ void es_projected_state(const double* u, double seed,
+                                                   const GasDev& g, int niter,
+                                                   Node<DIM, MIX_NSP>& s) {
+  constexpr int NFLD = 2 + DIM + MIX_NSP;
+  Node<DIM, MIX_NSP> s0;
+#pragma unroll
+  for (int f = 0; f < NFLD; ++f) s0.q[f] = u[f];
+  const double t0 = point_temperature<DIM, MIX_NSP>(s0.q, seed, niter);
+  node_state<DIM, MIX_NSP, 1, false>(s0, g, t0);
+  double v[NFLD];
+  cons_to_entropy_p<DIM, MIX_NSP>(s0.q, s0.p, s0.gamma, v);
+  entropy_to_cons<DIM, MIX_NSP>(v, s0.gamma, s.q);
+  const double t1 = point_temperature<DIM, MIX_NSP>(s.q, seed, niter);
+  node_state<DIM, MIX_NSP, 1, false>(s, g, t1);
+}
+
+template <int DIM, int P>
+__global__ __launch_bounds__(kBlock) void esdg_project_mix_kernel(KArgs a, EsdgArgs ea) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP;
+  constexpr int NFLD = 2 + DIM + MIX_NSP;
+  extern __shared__ double lds[];
+  const int nq = ea.nq, epb = ea.epb;
+  double* s_e = lds;  // [F][epb][nq]
+  const int64_t e0 = int64_t(blockIdx.x) * epb;
+  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
+  const int64_t N = a.nnodes;
+  for (int t = threadIdx.x; t < ne * nq; t += blockDim.x) {
+    const int le = t / nq;
+    const int m = t - le * nq;
+    const int64_t base = (e0 + le) * NP;
+    double u[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) u[f] = 0.0;
+    double seed = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < NP; ++i) {
+      const double w = ea.interp[i * nq + m];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) u[f] += w * a.q[f * N + base + i];
+      seed += w * a.temp[base + i];
     }
-    __syncthreads();
+    Node<DIM, MIX_NSP> s;
+    if (ea.es_volume) {
+      es_projected_state<DIM>(u, seed, a.gas, ea.niter, s);
+    } else {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s.q[f] = u[f];
+      const double t0 = point_temperature<DIM, MIX_NSP>(s.q, seed, ea.niter);
+      node_state<DIM, MIX_NSP, 1, false>(s, a.gas, t0);
+    }
+    double v[NFLD];
+    cons_to_entropy_p<DIM, MIX_NSP>(s.q, s.p, s.gamma, v);
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_e[(f * epb + le) * nq + m] = v[f];
   }
+  __syncthreads();
+  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
+    const int le = t / NP;
+    const int i = t - le * NP;
+    double v[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
+    for (int m = 0; m < nq; ++m) {
+      const double w = ea.proj[m * NP + i];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) v[f] += w * s_e[(f * epb + le) * nq + m];
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) ea.ev[f * N + e0 * NP + t] = v[f];
+  }
+}
+
+template <int DIM, int P>
+__global__ __launch_bounds__(kBlock) void esdg_mix_kernel(KArgs a, EsdgArgs ea) {
+  using E = Elem<DIM, P>;
+  constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF;
+  constexpr int NFLD = 2 + DIM + MIX_NSP;
+  using Pair = RnPair<DIM>;
+  extern __shared__ double lds[];
+  const int nq = ea.nq, nfq = ea.nfq, nh = ea.nh, epb = ea.epb, nft = NF * ea.nfq;
+  Pair* s_prim = reinterpret_cast<Pair*>(lds);                 // [epb][nh]
+  double* s_v = reinterpret_cast<double*>(s_prim + epb * nh);  // [F][epb][NP]
+  double* s_q = s_v + NFLD * epb * NP;                         // [F][epb][NP]
+  double* s_vp = s_q + NFLD * epb * NP;                        // [F][epb][NF*NFP]
+  double* s_tn = s_vp + NFLD * epb * NF * NFP;                 // [epb][NP] nodal T
+  double* s_gb = s_tn + epb * NP;                              // [epb][NP] gamma(cv, T)
+  double* s_tp = s_gb + epb * NP;                              // [epb][NF*NFP]
+  double* s_gf = s_tp + epb * NF * NFP;                        // [epb][nft] gamma_h
+  double* s_cf = s_gf + epb * nft;                             // [epb][nft] |u| + c
+  const int C = (nh + nq - 1) / nq;
+  const int nacc = C * nq + nft;
+  double* s_fs = s_cf + epb * nft;                             // [F][epb][nft]
+  double* s_qf = s_fs + NFLD * epb * nft;                      // [F][epb][nft], 2-3
+  double* s_acc = s_qf;                                        // [F][epb][nacc], 4-5
+  const int64_t e0 = int64_t(blockIdx.x) * epb;
+  const int ne = a.K - e0 < epb ? int(a.K - e0) : epb;
+  const int64_t N = a.nnodes;
+  // 1: own v~, q, T and gamma(cv, T); the neighbours' v~ and T at this
+  // element's face nodes
   for (int t = threadIdx.x; t < NFLD * ne * NP; t += blockDim.x) {
-    const int fl = t / NP;
-    const int i = t - fl * NP;
-    const int f = fl / ne, le = fl - f * ne;
-    const double* sa = s_acc + (f * epb + le) * nacc;
-    const double* w = ea.apply + i;
-    double o0 = 0.0, o1 = 0.0, o2 = 0.0, o3 = 0.0;
-    int h = 0;
-    for (; h + 4 <= nq; h += 4) {
-      o0 += w[h * NP] * sa[h];
-      o1 += w[(h + 1) * NP] * sa[h + 1];
-      o2 += w[(h + 2) * NP] * sa[h + 2];
-      o3 += w[(h + 3) * NP] * sa[h + 3];
-    }
-    for (; h < nq; ++h) o0 += w[h * NP] * sa[h];
-    const double* sf = sa + C * nq;
-    const double* wf = w + nq * NP;
-    int m = 0;
-    for (; m + 4 <= nft; m += 4) {
-      o0 += wf[m * NP] * sf[m];
-      o1 += wf[(m + 1) * NP] * sf[m + 1];
-      o2 += wf[(m + 2) * NP] * sf[m + 2];
-      o3 += wf[(m + 3) * NP] * sf[m + 3];
-    }
-    for (; m < nft; ++m) o0 += wf[m * NP] * sf[m];
-    a.out[f * N + (e0 + le) * NP + i] = -((o0 + o1) + (o2 + o3));
+    const int f = t / (ne * NP);
+    const int r = t - f * ne * NP;
+    s_v[f * epb * NP + r] = ea.ev[f * N + e0 * NP + r];
+    s_q[f * epb * NP + r] = a.q[f * N + e0 * NP + r];
   }
+  for (int t = threadIdx.x; t < ne * NP; t += blockDim.x) {
+    const int64_t g = e0 * NP + t;
+    Node<DIM, MIX_NSP> s;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s.q[f] = a.q[f * N + g];
+    node_state<DIM, MIX_NSP, 1, false>(s, a.gas, a.temp[g]);
+    s_tn[t] = a.temp[g];
+    s_gb[t] = s.gamma;
+  }
+  for (int t = threadIdx.x; t < ne * NF * NFP; t += blockDim.x) {
+    const int le = t / (NF * NFP);
+    const int fn = t - le * NF * NFP;
+    const int32_t nb = a.vmap_p[(e0 + le) * NF * NFP + fn];
+    if (nb < 0) continue;
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f)
+      s_vp[(f * epb + le) * NF * NFP + fn] = ea.ev[f * N + nb];
+    s_tp[le * NF * NFP + fn] = a.temp[nb];
+  }
+  __syncthreads();
+  // 2: the state at the hybridized points
+  for (int t = threadIdx.x; t < ne * nh; t += blockDim.x) {
+    const int le = t / nh;
+    const int h = t - le * nh;
+    double v[NFLD];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) v[f] = 0.0;
+    double gh = 0.0, seed = 0.0;
+#pragma unroll 1
+    for (int i = 0; i < NP; ++i) {
+      const double w = ea.vh[i * nh + h];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) v[f] += w * s_v[(f * epb + le) * NP + i];
+      gh += w * s_gb[le * NP + i];
+      seed += w * s_tn[le * NP + i];
+    }
+    Node<DIM, MIX_NSP> s;
+    entropy_to_cons<DIM, MIX_NSP>(v, gh, s.q);
+    const double th = point_temperature<DIM, MIX_NSP>(s.q, seed, ea.niter);
+    node_state<DIM, MIX_NSP, 1, false>(s, a.gas, th);
+    RnPrim<DIM> pr;
+    rn_prim<DIM>(s, pr);
+    s_prim[le * nh + h] = pr;
+    if (h >= nq) {
+      const int m = h - nq;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) s_qf[(f * epb + le) * nft + m] = s.q[f];
+      s_gf[le * nft + m] = gh;
+      s_cf[le * nft + m] = pr.c;
+    }
+  }
+  __syncthreads();
+  // 3: entropy-stable face flux at the face points, times fscale * w
+  for (int t = threadIdx.x; t < ne * nft; t += blockDim.x) {
+    const int le = t / nft;
+    const int m = t - le * nft;
+    const int fc = m / nfq;
+    const int mm = m - fc * nfq;
+    const int64_t e = e0 + le;
+    const double* w = ea.finterp + fc * NFP * nfq + mm;
+    const int32_t v0 = a.vmap_p[(e * NF + fc) * NFP];
+    double qm[NFLD], qp[NFLD];
+    RnPrim<DIM> pm, pp;
+    Node<DIM, MIX_NSP> sp;
+    if (v0 >= 0) {
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) qm[f] = s_qf[(f * epb + le) * nft + m];
+      static_cast<Pair&>(pm) = s_prim[le * nh + nq + m];
+      pm.c = s_cf[le * nft + m];
+      double vp[NFLD];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) vp[f] = 0.0;
+      double seed = 0.0;
+#pragma unroll 1
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f)
+          vp[f] += wj * s_vp[(f * epb + le) * NF * NFP + fc * NFP + j];
+        seed += wj * s_tp[le * NF * NFP + fc * NFP + j];
+      }
+      entropy_to_cons<DIM, MIX_NSP>(vp, s_gf[le * nft + m], sp.q);
+      const double tp = point_temperature<DIM, MIX_NSP>(sp.q, seed, ea.niter);
+      node_state<DIM, MIX_NSP, 1, false>(sp, a.gas, tp);
+    } else {
+      // boundary: the entropy-stable projected interior state and the
+      // boundary condition's exterior state
+      double ui[NFLD];
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) ui[f] = 0.0;
+      double seed = 0.0;
+#pragma unroll 1
+      for (int j = 0; j < NFP; ++j) {
+        const double wj = w[j * nfq];
+        const int ij = a.fmask[fc * NFP + j];
+#pragma unroll
+        for (int f = 0; f < NFLD; ++f) ui[f] += wj * s_q[(f * epb + le) * NP + ij];
+        seed += wj * s_tn[le * NP + ij];
+      }
+      Node<DIM, MIX_NSP> sm;
+      es_projected_state<DIM>(ui, seed, a.gas, ea.niter, sm);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) qm[f] = sm.q[f];
+      rn_prim<DIM>(sm, pm);
+      const int64_t bq = (-int64_t(v0) - 1) / NFP * nfq + mm;
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) sp.q[f] = ea.qbq[f * ea.nbq + bq];
+      node_state<DIM, MIX_NSP, 1, false>(sp, a.gas, ea.tbq[bq]);
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) qp[f] = sp.q[f];
+    rn_prim<DIM>(sp, pp);
+    double n[DIM], fs[NFLD];
+#pragma unroll
+    for (int d = 0; d < DIM; ++d) n[d] = a.normals[(e * NF + fc) * DIM + d];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) fs[f] = 0.0;
+    rn_flux_dot<DIM>(pm, pp, n, fs);
+    if (ea.dissipation) {
+      const double alpha = fmax(pm.c, pp.c);
+#pragma unroll
+      for (int f = 0; f < NFLD; ++f) fs[f] -= 0.5 * alpha * (qp[f] - qm[f]);
+    }
+    const double sc = a.fscale[e * NF + fc] * ea.wf[mm];
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_fs[(f * epb + le) * nft + m] = sc * fs[f];
+  }
+  __syncthreads();
+  // 4: rows of (Q_h - Q_h^T) o F_S, plus the face flux on face rows
+  esdg_rows<DIM, NFLD>(a, ea, s_prim, s_fs, s_acc, e0, ne, nacc, C,
+                       [](const Pair& x, const Pair& y, const double* sd, double* acc) {
+                         rn_flux_dot<DIM>(x, y, sd, acc);
+                       });
+  __syncthreads();
+  // 5: dQ/dt = -M^-1 V_h^T acc
+  esdg_apply<DIM, P, NFLD>(a, ea, s_acc, e0, ne, nacc, C);
 }
 
 // }}}

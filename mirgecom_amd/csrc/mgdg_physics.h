@@ -642,6 +642,111 @@ __device__ __forceinline__ void ec_flux_dot(const EcPair<DIM, NSP>& a,
   }
 }
 
+// {{{ Renac's two-point flux (mixtures)
+
+// Mixture c_v (mass basis) at T: sum_k Y_k R / W_k (cp_k / R - 1).
+__device__ __forceinline__ double mix_cv(double t, const double* y) {
+  double sc = 0.0;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    const bool high = t > c_mech.t_mid[k];
+    const double cp_lo = horner5(c_poly.cp[0][k], t);
+    const double cp_hi = horner5(c_poly.cp[1][k], t);
+    sc += c_mech.inv_wt[k] * ((high ? cp_hi : cp_lo) - 1.0) * y[k];
+  }
+  return c_mech.gas_constant * sc;
+}
+
+// Entropy variables (mirgecom/gas_model.py:1026-1060) of a state with its
+// own pressure p and ratio gamma (for a mixture p = rho R T, not
+// (gamma - 1) rho e).
+template <int DIM, int NSP>
+__device__ __forceinline__ void cons_to_entropy_p(const double* q, double p,
+                                                  double gamma, double* v) {
+  const double rho = q[0];
+  const double irho = 1.0 / rho;
+  double u2 = 0.0;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) u2 += (q[2 + d] * irho) * (q[2 + d] * irho);
+  const double s = log(p) - gamma * log(rho);
+  const double rho_p = rho / p;
+  v[0] = (gamma - s) / (gamma - 1.0) - 0.5 * rho_p * u2;
+  v[1] = -rho_p;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) v[2 + d] = rho_p * (q[2 + d] * irho);
+#pragma unroll
+  for (int k = 0; k < NSP; ++k) v[2 + DIM + k] = q[2 + DIM + k] * irho / (gamma - 1.0);
+}
+
+// What Renac's flux reads of a point: rho, theta = 1/T (with their
+// logarithms for the logarithmic means), p/T, T, u/2, Y/2.
+template <int DIM>
+struct RnPair {
+  double rho, lrho, th, lth, pth, t;
+  double uh[DIM];
+  double yh[MIX_NSP];
+};
+
+// ... with the wavespeed |u| + c the face flux needs.
+template <int DIM>
+struct RnPrim : RnPair<DIM> {
+  double c;
+};
+
+template <int DIM>
+__device__ __forceinline__ void rn_prim(const Node<DIM, MIX_NSP>& s, RnPrim<DIM>& a) {
+  a.rho = s.q[0];
+  a.lrho = log(s.q[0]);
+  a.t = s.t;
+  a.th = 1.0 / s.t;
+  a.lth = -log(s.t);
+  a.pth = s.p * a.th;
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) a.uh[d] = 0.5 * s.v[d];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) a.yh[k] = 0.5 * s.y[k];
+  a.c = s.speed + s.c;
+}
+
+// sum_d s_d F_S,d(a, b) of Renac's entropy-conserving, kinetic-energy-
+// preserving flux (mirgecom/inviscid.py:521-598; Renac 2021, eq. 24), added
+// to acc: rho_mean = ln-mean(rho), T_mean = 1 / ln-mean(1/T),
+// p~ = (p_a/T_a + p_b/T_b)/2 (T_a + T_b)/2, u = (u_a + u_b)/2,
+// Y = (Y_a + Y_b)/2; mass rho_mean u.s, momentum rho_mean u (u.s) + s p~,
+// energy (c_v(T_mean, Y) T_mean + u_a.u_b / 2) rho_mean u.s + p~ u.s,
+// species rho_mean Y u.s.  (The reference's p_mean and gamma_avg enter only
+// its c_v evaluation's state, whose c_v depends on T and Y alone.)
+template <int DIM>
+__device__ __forceinline__ void rn_flux_dot(const RnPair<DIM>& a, const RnPair<DIM>& b,
+                                            const double* s, double* acc) {
+  double rn, rd, tn, td;
+  ln_mean_parts(a.rho, a.lrho, b.rho, b.lrho, rn, rd);
+  ln_mean_parts(a.th, a.lth, b.th, b.lth, tn, td);
+  const double rho_mean = rn * rcp_nr(rd);
+  const double t_mean = td * rcp_nr(tn);
+  const double pec = 0.25 * (a.pth + b.pth) * (a.t + b.t);
+  double un = 0.0, kin = 0.0, ua[DIM];
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
+    ua[d] = a.uh[d] + b.uh[d];
+    un += s[d] * ua[d];
+    kin += a.uh[d] * b.uh[d];
+  }
+  double y[MIX_NSP];
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) y[k] = a.yh[k] + b.yh[k];
+  const double cvm = mix_cv(t_mean, y);
+  const double mflux = rho_mean * un;
+  acc[0] += mflux;
+  acc[1] += (cvm * t_mean + 2.0 * kin) * mflux + pec * un;
+#pragma unroll
+  for (int i = 0; i < DIM; ++i) acc[2 + i] += mflux * ua[i] + s[i] * pec;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) acc[2 + DIM + k] += mflux * y[k];
+}
+
+// }}}
+
 // }}}
 
 // Numerical inviscid flux: 0 Rusanov, 1 HLL, 2 central.

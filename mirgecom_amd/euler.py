@@ -86,6 +86,14 @@ def entropy_stable_euler_operator(
         limiter_func=limiter_func, entropy_min=entropy_min)
 
 
+def _esdg_for_ns(dcoll, state, gas_model, boundaries, time):
+    """entropy_stable_euler_operator as ns_operator calls it
+    (mirgecom/navierstokes.py:564-570): with ns_operator's own operator
+    states, so the volume points take the plain projected state."""
+    return _euler_esdg(dcoll, state, gas_model, boundaries, time, None, None,
+                       es_volume=False)
+
+
 def _euler_quad(dcoll, state, gas_model, boundaries, time, kind):
     from mirgecom_amd._opcommon import quad_boundary_states
     from mirgecom_amd.device import get_device_discretization
@@ -108,19 +116,28 @@ def _euler_quad(dcoll, state, gas_model, boundaries, time, kind):
 
 
 def _euler_esdg(dcoll, state, gas_model, boundaries, time, numerical_flux_func,
-                entropy_conserving_flux_func):
+                entropy_conserving_flux_func, es_volume=True):
+    """The flux-differencing RHS: Chandrashekar's two-point flux for the
+    ideal gas (``mg_euler_rhs_esdg``), Renac's for the uiuc_7sp mixture
+    (``mg_euler_rhs_esdg_mixture``).  ``es_volume=False``: the volume points
+    take the plain projected state (what ns_operator hands its inviscid
+    operator, mirgecom/navierstokes.py:564-570) instead of the entropy-stable
+    projected one."""
     from mirgecom_amd._opcommon import quad_boundary_states
     from mirgecom_amd.device import get_device_discretization
     from mirgecom_amd.inviscid import (
-        entropy_conserving_flux_chandrashekar, entropy_stable_inviscid_facial_flux,
+        entropy_conserving_flux_chandrashekar, entropy_conserving_flux_renac,
+        entropy_stable_inviscid_facial_flux,
         entropy_stable_inviscid_facial_flux_rusanov)
-    if getattr(gas_model.eos, "is_mixture", False):
-        raise NotImplementedError("the entropy-stable operator is built for the "
-                                  "ideal gas")
-    if entropy_conserving_flux_func not in (
-            None, entropy_conserving_flux_chandrashekar):
-        raise NotImplementedError("entropy_conserving_flux_chandrashekar is the "
-                                  "built-in entropy-conserving flux")
+    mix = bool(getattr(gas_model.eos, "is_mixture", False))
+    builtin = entropy_conserving_flux_renac if mix else \
+        entropy_conserving_flux_chandrashekar
+    if entropy_conserving_flux_func is None:
+        warn("No entropy_conserving_flux_func was given for ESDG. Setting EC "
+             f"flux to {builtin.__name__}.", stacklevel=3)
+    elif entropy_conserving_flux_func is not builtin:
+        raise NotImplementedError(f"{builtin.__name__} is the built-in "
+                                  "entropy-conserving flux for this gas")
     if numerical_flux_func is None:
         warn("No inviscid_numerical_flux_func was given for ESDG. Automatically "
              "setting facial flux to entropy-stable Rusanov "
@@ -138,10 +155,22 @@ def _euler_esdg(dcoll, state, gas_model, boundaries, time, numerical_flux_func,
         raise TypeError("the state must be a float64 tensor on the GPU")
     dev = get_device_discretization(dcoll, q.device)
     q = q.contiguous()
-    qbq, _ = quad_boundary_states(dcoll, dev, q, state.dim, gas_model,
-                                  boundaries, time)
     ev = torch.empty_like(q)
     rhs = torch.empty_like(q)
+    if mix:
+        temp = state.temperature.contiguous()
+        qbq, tbq = quad_boundary_states(dcoll, dev, q, state.dim, gas_model,
+                                        boundaries, time, temp,
+                                        entropy_stable=True)
+        _lib.call("mg_euler_rhs_esdg_mixture", dev.mesh,
+                  gas_model.device_params(state.cv.nspecies, with_transport=False),
+                  dev.quad_ops(), dev.esdg_ops(), _lib.ptr(q), _lib.ptr(temp),
+                  _lib.ptr(qbq), _lib.ptr(tbq), gas_model.eos.temperature_niter,
+                  dissipation, 1 if es_volume else 0, _lib.ptr(ev), _lib.ptr(rhs),
+                  _lib.stream_handle())
+        return ConservedVars(state.dim, rhs)
+    qbq, _ = quad_boundary_states(dcoll, dev, q, state.dim, gas_model,
+                                  boundaries, time)
     _lib.call("mg_euler_rhs_esdg", dev.mesh,
               gas_model.device_params(state.cv.nspecies, with_transport=False),
               dev.quad_ops(), dev.esdg_ops(), _lib.ptr(q), _lib.ptr(qbq),

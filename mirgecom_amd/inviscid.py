@@ -111,13 +111,53 @@ def entropy_conserving_flux_chandrashekar(gas_model, state_ll, state_rr):
     return ConservedVars(dim, torch.stack(rows))
 
 
+def entropy_conserving_flux_renac(gas_model, state_ll, state_rr):
+    """mirgecom/inviscid.py:521-598: Renac's entropy-conserving,
+    kinetic-energy-preserving two-point flux ``(F, dim, ...)`` (Renac 2021,
+    eq. 24), the mixtures' flux of the entropy-stable operator."""
+    dim = state_ll.dim
+    t_l, t_r = state_ll.temperature, state_rr.temperature
+    p_l, p_r = state_ll.pressure, state_rr.pressure
+    g_l = gas_model.eos.gamma(state_ll.cv, t_l)
+    g_r = gas_model.eos.gamma(state_rr.cv, t_r)
+    th_l, th_r = 1.0 / t_l, 1.0 / t_r
+    t_avg = 0.5 * (t_l + t_r)
+    pot_avg = 0.5 * (p_l * th_l + p_r * th_r)
+    t_mean = 1.0 / _ln_mean(th_l, th_r)
+    pec_avg = pot_avg * t_avg
+    p_mean = _ln_mean(p_l, p_r)
+    rho_l, rho_r = state_ll.mass_density, state_rr.mass_density
+    u_l, u_r = state_ll.velocity, state_rr.velocity
+    kin_comb = 0.5 * (u_l * u_r).sum(dim=0)
+    rho_mean = _ln_mean(rho_l, rho_r)
+    y_avg = 0.5 * (state_ll.species_mass_fractions
+                   + state_rr.species_mass_fractions)
+    spec_mean = rho_mean * y_avg
+    u_avg = 0.5 * (u_l + u_r)
+    mass = rho_mean * u_avg
+    mom = torch.stack([mass[i] * u_avg + torch.stack(
+        [pec_avg if j == i else 0 * pec_avg for j in range(dim)])
+        for i in range(dim)])
+    gamma_avg = 0.5 * (g_l + g_r)
+    ener_es = p_mean / (gamma_avg - 1) + 0.5 * rho_mean * (u_avg * u_avg).sum(dim=0)
+    cv_es = ConservedVars(dim, torch.cat([rho_mean[None], ener_es[None],
+                                          rho_mean * u_avg, spec_mean]))
+    cvm = gas_model.eos.heat_capacity_cv(cv_es, t_mean)
+    energy = (cvm * t_mean + kin_comb) * mass + pec_avg * u_avg
+    rows = [mass, energy] + [mom[i] for i in range(dim)]
+    rows += [spec_mean[k] * u_avg for k in range(state_ll.nspecies)]
+    return ConservedVars(dim, torch.stack(rows))
+
+
 def entropy_stable_inviscid_facial_flux(state_pair, gas_model, normal,
                                         entropy_conserving_flux_func=None,
                                         alpha=None):
     """mirgecom/inviscid.py:601-662: ``F_S(q-, q+) . n``, penalised by
     ``alpha/2 (q+ - q-)`` when ``alpha`` is given."""
     if entropy_conserving_flux_func is None:
-        entropy_conserving_flux_func = entropy_conserving_flux_chandrashekar
+        entropy_conserving_flux_func = (
+            entropy_conserving_flux_renac if state_pair.int.is_mixture
+            else entropy_conserving_flux_chandrashekar)
     flux = entropy_conserving_flux_func(gas_model, state_pair.int,
                                         state_pair.ext).q
     if alpha is not None:

@@ -259,10 +259,18 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
     out = ConservedVars(state.dim, rhs)
     if inviscid_terms_on and inviscid_fluid_operator is not None:
         # mirgecom/navierstokes.py:564-570 (ESDG always takes this branch)
-        out = out + inviscid_fluid_operator(
-            dcoll, state=state, gas_model=gas_model, boundaries=boundaries,
-            time=time, dd=dd, comm_tag=comm_tag, quadrature_tag=quadrature_tag,
-            operator_states_quad=operator_states_quad)
+        if (inviscid_fluid_operator is entropy_stable_euler_operator
+                and quadrature_tag == DISCR_TAG_QUAD):
+            # the reference hands its own (plain projected) operator states
+            # to the entropy-stable operator
+            from mirgecom_amd.euler import _esdg_for_ns
+            out = out + _esdg_for_ns(dcoll, state, gas_model, boundaries, time)
+        else:
+            out = out + inviscid_fluid_operator(
+                dcoll, state=state, gas_model=gas_model, boundaries=boundaries,
+                time=time, dd=dd, comm_tag=comm_tag,
+                quadrature_tag=quadrature_tag,
+                operator_states_quad=operator_states_quad)
     if return_gradients:
         return out, ConservedVars(state.dim, grad[:-1]), grad[-1]
     return out

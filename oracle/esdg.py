@@ -223,3 +223,246 @@ def esdg_euler_operator(disc, box, cv, gamma, boundaries, time=0.0,
                      (fstar * (sj[:, :, None] * ho.wf[None, None, :])[None])
                      .reshape(nvar, K, nft))
     return -np.einsum("ij,fkj->fki", ho.minv, vol + face) / disc.J[None, :, None]
+
+
+# {{{ mixtures: Renac's two-point flux and the state-based pipeline
+
+def entropy_vars_state(q, p, gamma, dim):
+    """gas_model.py:1026-1060 with the state's own pressure ``p`` and ratio
+    ``gamma`` (for a mixture, ``p = rho R T`` from the Newton temperature,
+    not ``(gamma - 1) rho e``)."""
+    rho = q[0]
+    u = q[2:2 + dim] / rho
+    u2 = np.sum(u * u, axis=0)
+    s = np.log(p) - gamma * np.log(rho)
+    rho_p = rho / p
+    ev = [(gamma - s) / (gamma - 1) - 0.5 * rho_p * u2, -rho_p]
+    ev += [rho_p * u[d] for d in range(dim)]
+    ev += [q[k] / rho / (gamma - 1) for k in range(2 + dim, q.shape[0])]
+    return np.stack(ev)
+
+
+def renac(sl, sr, eos, dim):
+    """inviscid.py:521-598: the entropy-conserving, kinetic-energy-preserving
+    two-point flux of Renac (2021, eq. 24) between states ``sl`` and ``sr``
+    (``oracle.physics.State``), ``(F, dim, ...)``.  The energy flux takes the
+    mixture's c_v at the logarithmic-mean temperature and the averaged mass
+    fractions."""
+    tl, tr = sl.temperature, sr.temperature
+    pl, pr = sl.pressure, sr.pressure
+    gl, gr = eos.gamma(sl.cv, tl), eos.gamma(sr.cv, tr)
+    thl, thr = 1.0 / tl, 1.0 / tr
+    t_avg = 0.5 * (tl + tr)
+    pot_avg = 0.5 * (pl * thl + pr * thr)
+    t_mean = 1.0 / ln_mean(thl, thr)
+    pec_avg = pot_avg * t_avg
+    p_mean = ln_mean(pl, pr)
+    rl, rr = sl.mass, sr.mass
+    ul, ur = sl.velocity, sr.velocity
+    yl, yr = sl.species_mass_fractions, sr.species_mass_fractions
+    kin_comb = 0.5 * np.sum(ul * ur, axis=0)
+    rho_mean = ln_mean(rl, rr)
+    spec_mean = rho_mean * 0.5 * (yl + yr)
+    u_avg = 0.5 * (ul + ur)
+    mass = rho_mean * u_avg                                    # (dim, ...)
+    mom = mass[:, None] * u_avg[None, :] + np.eye(dim).reshape(
+        (dim, dim) + (1,) * (mass.ndim - 1)) * pec_avg         # [i][d]
+    gamma_avg = 0.5 * (gl + gr)
+    ener_es = p_mean / (gamma_avg - 1) + 0.5 * rho_mean * np.sum(u_avg * u_avg,
+                                                                 axis=0)
+    cv_es = np.concatenate([rho_mean[None], ener_es[None], rho_mean * u_avg,
+                            spec_mean])
+    cvm = eos.heat_capacity_cv(cv_es, t_mean)
+    energy = (cvm * t_mean + kin_comb) * mass + pec_avg * u_avg
+    spec = [spec_mean[k] * u_avg for k in range(spec_mean.shape[0])]
+    return np.stack([mass, energy] + [mom[i] for i in range(dim)] + spec)
+
+
+def _es_projected_state(u, seed, gas_model, dim):
+    """project_fluid_state(..., entropy_stable=True) at a set of points
+    (gas_model.py:494-588): the state made from ``u``, its entropy variables
+    and back with the state's own ratio, and the state made again from
+    those conserved variables (mixtures: both Newton solves seeded with
+    ``seed``)."""
+    eos = gas_model.eos
+    s0 = ph.make_fluid_state(u, gas_model, dim, temperature_seed=seed)
+    g0 = eos.gamma(s0.cv, s0.temperature)
+    cv_mod = conserved_vars(entropy_vars_state(s0.cv, s0.pressure, g0, dim), g0,
+                            dim)
+    return ph.make_fluid_state(cv_mod, gas_model, dim, temperature_seed=seed)
+
+
+def _flat_state(s, shape):
+    """A state of ``(F, ...)`` arrays reshaped to ``(F,) + shape``."""
+    def r(a):
+        return None if a is None else a.reshape(a.shape[:1] + shape) \
+            if a.ndim > len(s.temperature.shape) else a.reshape(shape)
+    return ph.State(cv=s.cv.reshape(s.cv.shape[:1] + shape), dim=s.dim,
+                    temperature=s.temperature.reshape(shape),
+                    pressure=s.pressure.reshape(shape),
+                    speed_of_sound=s.speed_of_sound.reshape(shape),
+                    species_enthalpies=(None if s.species_enthalpies is None else
+                                        s.species_enthalpies.reshape(
+                                            s.species_enthalpies.shape[:1] + shape)))
+
+
+def esdg_operator(disc, box, cv, gas_model, boundaries, time=0.0,
+                  dissipation=True, degree=None, temperature=None,
+                  ec_flux=None, es_volume=True):
+    """Entropy-stable Euler RHS ``(F, K, Np)`` through fluid states
+    (mirgecom/euler.py:108-310 step by step), for the ideal gas and, with
+    the nodal ``temperature (K, Np)``, the uiuc_7sp mixture:
+
+    * volume points: ``project_fluid_state(entropy_stable=True)`` (with
+      ``es_volume=False`` the plain projected state, which is what
+      ns_operator hands its inviscid operator, mirgecom/navierstokes.py:
+      564-570); entropy variables with the state's pressure and ratio
+      gamma_q, L2-projected;
+    * hybridized points: the projected variables and the nodal ratio
+      ``gamma(cv, T)`` interpolated, ``entropy_to_conservative_vars``, the
+      state made there (Newton seeded with the interpolated nodal T);
+    * interior faces: the neighbour's projected variables at the face
+      points turned into conserved variables with this element's
+      interpolated ratio, seeded with the neighbour's nodal T;
+    * boundary faces: ``project_fluid_state(entropy_stable=True)`` of the
+      interior and the condition's exterior state from it;
+    * two-point flux ``ec_flux(sl, sr)``: Renac's for mixtures, else
+      Chandrashekar's; faces ``F_S . n - alpha/2 (q+ - q-)`` (Rusanov,
+      ``alpha`` the larger of ``|u| + c``) or no penalty.
+    """
+    el = disc.el
+    dim, nf, nfp, K = disc.dim, disc.nf, el.nfp, disc.K
+    degree = 2 * el.order if degree is None else degree
+    eos = gas_model.eos
+    mix = temperature is not None
+    if ec_flux is None:
+        if mix:
+            def ec_flux(a, b):
+                return renac(a, b, eos, dim)
+        else:
+            def ec_flux(a, b):
+                return chandrashekar(a.cv, b.cv, eos.gamma(), dim)
+    ho = _HybridOps(el, degree)
+    nq, nfq, nft = ho.nq, ho.nfq, nf * ho.nfq
+    nh = nq + nft
+    nvar = cv.shape[0]
+
+    def interp(mat, a):           # (n, Np) x (..., K, Np) -> (..., K, n)
+        return np.einsum("hn,...kn->...kh", mat, a)
+
+    seed = (lambda a: interp(a, temperature).reshape(-1)) if mix else \
+        (lambda a: None)
+    # volume points: projected entropy variables
+    if es_volume:
+        sq = _es_projected_state(interp(ho.vq, cv).reshape(nvar, -1),
+                                 seed(ho.vq), gas_model, dim)
+    else:
+        sq = ph.make_fluid_state(interp(ho.vq, cv).reshape(nvar, -1), gas_model,
+                                 dim, temperature_seed=seed(ho.vq))
+    gq = eos.gamma(sq.cv, sq.temperature)
+    ev_q = entropy_vars_state(sq.cv, sq.pressure, gq, dim).reshape(nvar, K, nq)
+    ev_nodal = np.einsum("nq,fkq->fkn", ho.pq, ev_q)
+    # hybridized points
+    g_base = eos.gamma(cv, temperature) * np.ones((K, el.np))
+    gh = interp(ho.vh, g_base)                                    # (K, nh)
+    uh = conserved_vars(interp(ho.vh, ev_nodal), gh, dim)
+    sh = ph.make_fluid_state(uh.reshape(nvar, -1), gas_model, dim,
+                             temperature_seed=seed(ho.vh))
+    sh = _flat_state(sh, (K, nh))
+    # volume flux differencing, element by element
+    vol = np.zeros((nvar, K, el.np))
+    sj = disc.sJ
+
+    def pick(s, k, sl):
+        """The state at ``[k] + sl`` of its ``(K, nh)`` point axes."""
+        idx = (k,) + (sl if isinstance(sl, tuple) else (sl,))
+        full = (slice(None),) + idx
+        return ph.State(cv=s.cv[full], dim=dim,
+                        temperature=s.temperature[idx],
+                        pressure=s.pressure[idx],
+                        speed_of_sound=s.speed_of_sound[idx],
+                        species_enthalpies=(None if s.species_enthalpies is None
+                                            else s.species_enthalpies[full]))
+
+    for k in range(K):
+        a = pick(sh, k, (slice(None), None))
+        b = pick(sh, k, (None, slice(None)))
+        fmat = ec_flux(a, b)                                      # (F, d, h, j)
+        for d in range(dim):
+            qx = disc.J[k] * np.einsum("r,rij->ij", disc.drdx[k][:, d], ho.qr)
+            bx = np.repeat(disc.normal[k, :, d] * sj[k], nfq) * np.tile(ho.wf, nf)
+            qh = np.zeros((nh, nh))
+            qh[:nq, :nq] = qx - 0.5 * ho.e.T @ (bx[:, None] * ho.e)
+            qh[:nq, nq:] = 0.5 * ho.e.T * bx[None, :]
+            qh[nq:, :nq] = -0.5 * bx[:, None] * ho.e
+            qh[nq:, nq:] = 0.5 * np.diag(bx)
+            skew = qh - qh.T
+            vol[:, k] += np.einsum("hn,fhj->fn", ho.vh, skew[None] * fmat[:, d])
+    # faces: own states from the hybridized face points
+    sm = pick(sh, slice(None), slice(nq, None))                   # (.., K, nft)
+    sm = _flat_state(sm, (K, nf, nfq))
+    # boundary faces (no neighbour) take their own values here; their flux
+    # is replaced below
+    ev_ext = disc.exterior(ev_nodal)                              # (F, K, Nf, Nfp)
+    ev_ext = np.where(np.isnan(ev_ext), disc.trace(ev_nodal), ev_ext)
+    t_ext = None
+    if mix:
+        t_ext = disc.exterior(temperature[None])[0]
+        t_ext = np.where(np.isnan(t_ext), disc.trace(temperature[None])[0], t_ext)
+    evp = np.zeros((nvar, K, nf, nfq))
+    tp = np.zeros((K, nf, nfq))
+    for f in range(nf):
+        tb = ho.vf[f * nfq:(f + 1) * nfq][:, el.face_mask[f]]    # (nfq, Nfp)
+        evp[:, :, f] = np.einsum("mj,fkj->fkm", tb, ev_ext[:, :, f])
+        if mix:
+            tp[:, f] = np.einsum("mj,kj->km", tb, t_ext[:, f])
+    gface = gh[:, nq:].reshape(K, nf, nfq)
+    up = conserved_vars(evp, gface, dim)
+    sp = ph.make_fluid_state(up.reshape(nvar, -1), gas_model, dim,
+                             temperature_seed=tp.reshape(-1) if mix else None)
+    sp = _flat_state(sp, (K, nf, nfq))
+    normal = np.moveaxis(np.repeat(disc.normal[:, :, None, :], nfq, axis=2),
+                         -1, 0)                                   # (d, K, Nf, q)
+
+    def face_flux(a, b, n):
+        f = np.einsum("fd...,d...->f...", ec_flux(a, b), n)
+        if dissipation:
+            alpha = np.maximum(a.wavespeed, b.wavespeed)
+            f = f - 0.5 * alpha * (b.cv - a.cv)
+        return f
+
+    fstar = face_flux(sm, sp, normal)
+    if boundaries:
+        trace_q = interp(ho.vf, cv).reshape(nvar, K, nf, nfq)
+        trace_t = interp(ho.vf, temperature).reshape(K, nf, nfq) if mix else None
+        xf, _ = face_rule(dim, degree)
+        xf_ref = []
+        for fv in FACES[dim]:
+            v = REF_VERTS[dim][list(fv)]
+            xf_ref.append(v[0] + 0.5 * (1 + xf) @ (v[1:] - v[0]))
+        xphys = (disc.v[:, None, None, 0, :]
+                 + np.einsum("fmi,kid->kfmd", (1 + np.array(xf_ref)) / 2,
+                             disc.v[:, 1:, :] - disc.v[:, :1, :]))
+        for bdry, idx in _boundary_groups(disc, box, boundaries):
+            faces = np.unique(idx // nfp)
+            kk, ff = faces // nf, faces % nf
+            nb = len(faces)
+            smb = _es_projected_state(
+                trace_q[:, kk, ff].reshape(nvar, -1), gas_model=gas_model,
+                seed=trace_t[kk, ff].reshape(-1) if mix else None, dim=dim)
+            if bdry.kind == "dummy":
+                spb = smb
+            elif bdry.kind == "prescribed":
+                spb = bdry.state_func(np.moveaxis(xphys[kk, ff], -1, 0)
+                                      .reshape(dim, -1), smb, time)
+            else:
+                raise NotImplementedError("the entropy-stable oracle covers "
+                                          "prescribed and dummy boundaries")
+            n = normal[:, kk, ff].reshape(dim, -1)
+            fstar[:, kk, ff] = face_flux(smb, spb, n).reshape(nvar, nb, nfq)
+    face = np.einsum("hn,fkh->fkn", ho.vf,
+                     (fstar * (sj[:, :, None] * ho.wf[None, None, :])[None])
+                     .reshape(nvar, K, nft))
+    return -np.einsum("ij,fkj->fki", ho.minv, vol + face) / disc.J[None, :, None]
+
+# }}}
