@@ -838,11 +838,16 @@ def _rel_err_cv(a, b, dim):
 def mixture_kernels(args):
     """The reacting box's right-hand sides (config 3 state, 32^3 x 6 p = 2):
     Euler FP64 and mixed, Navier-Stokes with constant transport FP64 and
-    mixed, each evaluated args.mixture_kernels times."""
+    mixed, each evaluated args.mixture_kernels times; then, on the quadrature
+    of degree 4, the overintegrated Navier-Stokes RHS and the entropy-stable
+    Euler RHS (Renac's flux), each args.mixture_kernels times, with the
+    per-entry-point HIP-event averages printed."""
     import warnings
 
     import torch
 
+    from mirgecom_amd import _lib
+    from mirgecom_amd.dof_desc import DISCR_TAG_QUAD
     from mirgecom_amd.euler import euler_operator
     from mirgecom_amd.gas_model import make_fluid_state
     from mirgecom_amd.inviscid import inviscid_facial_flux_rusanov
@@ -862,9 +867,20 @@ def mixture_kernels(args):
                                inviscid_numerical_flux_func=inviscid_facial_flux_rusanov)
             for _ in range(args.mixture_kernels):
                 ns_operator(wl.dcoll, gm_ns, stn, {}, precision=prec)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        _lib.timing = {}
+        for _ in range(args.mixture_kernels):
+            ns_operator(wl.dcoll, gm_ns, stn, {}, quadrature_tag=DISCR_TAG_QUAD)
+        for _ in range(args.mixture_kernels):
+            euler_operator(wl.dcoll, ste, wl.gas_model, {}, use_esdg=True,
+                           quadrature_tag=DISCR_TAG_QUAD)
+        torch.cuda.synchronize()
+    t = {k: round(sum(a.elapsed_time(b) for a, b in v) / len(v), 4)
+         for k, v in _lib.timing.items()}
+    _lib.timing = None
     print(f"# mixture kernels: {args.mixture_kernels} evaluations per operator and "
-          "precision", file=sys.stderr, flush=True)
+          f"precision; quadrature-degree-4 entry points (ms): {json.dumps(t)}",
+          file=sys.stderr, flush=True)
 
 
 def main():
