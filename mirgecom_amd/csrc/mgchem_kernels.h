@@ -470,6 +470,8 @@ static __global__ __launch_bounds__(256) void adam_kernel(float* __restrict__ pr
 
 constexpr int kRows = 128;
 constexpr int kLdsStride = kH + 8;
+constexpr int kOutStride = kLdsStride / 2;   // output rows over activation rows (floats)
+static_assert(kOut <= kOutStride, "an output row fits an activation row");
 
 // One hidden layer of the fused inference, in place on the activations
 // act [128][kLdsStride] (BF16, LDS), formed transposed: D = W X^T with the
@@ -566,8 +568,11 @@ template <typename A>
 __device__ __forceinline__ void infer_body(const A& a, int64_t blk) {
   extern __shared__ __align__(16) unsigned char smem_raw[];
   __bf16* act = reinterpret_cast<__bf16*>(smem_raw);                  // [128][264]
-  float* outp = reinterpret_cast<float*>(act + kRows * kLdsStride);    // [128][32]
-  int* eflag = reinterpret_cast<int*>(outp + kRows * kOut);            // [128]
+  // the output layer's results overlay the activations: row r at the start
+  // of activation row r (each wave reads and writes only its own 32 rows
+  // there), so that two workgroups fit a CU's LDS
+  float* outp = reinterpret_cast<float*>(smem_raw);                    // [128][132]
+  int* eflag = reinterpret_cast<int*>(act + kRows * kLdsStride);       // [128]
   const int lane = threadIdx.x & 63;
   const int w = threadIdx.x >> 6;
   const int epb = kRows / a.np;
@@ -646,7 +651,8 @@ __device__ __forceinline__ void infer_body(const A& a, int64_t blk) {
       for (int i = 0; i < 2; ++i) {
         const int row0 = 16 * (2 * w + i) + (lane >> 4) * 4;
 #pragma unroll
-        for (int e = 0; e < 4; ++e) outp[(row0 + e) * kOut + 16 * j + r16] = acc[i][j][e] + b;
+        for (int e = 0; e < 4; ++e)
+          outp[(row0 + e) * kOutStride + 16 * j + r16] = acc[i][j][e] + b;
       }
     }
   }
@@ -665,7 +671,8 @@ __device__ __forceinline__ void infer_body(const A& a, int64_t blk) {
     const double irdt = rho / a.dt;
 #pragma unroll
     for (int k = 0; k < MIX_NSP; ++k)
-      src[k] = (double(outp[rr * kOut + k]) * a.nm.out_scale[k] + a.nm.out_mean[k]) * irdt;
+      src[k] = (double(outp[rr * kOutStride + k]) * a.nm.out_scale[k] + a.nm.out_mean[k]) *
+               irdt;
     if (a.nm.sample_period > 0 && g % a.nm.sample_period == a.nm.sample_offset) {
       double ex[MIX_NSP];
       mgdg::mix_species_source(rho, a.temp[g], y, ex);
@@ -701,8 +708,7 @@ static __global__ __launch_bounds__(256, 2) void infer_kernel(InferArgs a) {
 }
 
 // LDS bytes of infer_body / infer_kernel
-constexpr size_t kInferLds = sizeof(__bf16) * kRows * kLdsStride + sizeof(float) * kRows * kOut +
-                             sizeof(int) * kRows;
+constexpr size_t kInferLds = sizeof(__bf16) * kRows * kLdsStride + sizeof(int) * kRows;
 
 // }}}
 
