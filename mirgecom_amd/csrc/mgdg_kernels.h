@@ -1376,7 +1376,13 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
 }
 
 template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
-__global__ __launch_bounds__(kBlock) void grad_kernel(KArgs a,
+// The mixed-precision few-field flux kernel aims at 5 waves per SIMD (96
+// VGPRs; its LDS allows 8 workgroups per CU): 0.114 -> 0.106 ms on the TGV
+// p = 3 workload.  6 waves, and 5 for the mixed divergence kernel, were
+// slower (spills); the FP64 flux kernel is held to 4 workgroups by LDS.
+__global__ __launch_bounds__(kBlock)
+__attribute__((amdgpu_waves_per_eu((FLUX && NSP == 0 && sizeof(R) == 4) ? 5 : 1)))
+void grad_kernel(KArgs a,
                                                       const double* __restrict__ ws,
                                                       double* __restrict__ out) {
   grad_body<DIM, P, NSP, MIX, FLUX, R>(a, ws, out, xcd_block());
