@@ -27,7 +27,7 @@ def check_dd(dd):
 
 def check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
                       limiter_func, entropy_min, comm_tag=None,
-                      allow_quad=False):
+                      allow_quad=False, state=None, boundaries=None):
     if use_esdg and not (allow_quad and quadrature_tag == DISCR_TAG_QUAD):
         raise NotImplementedError("the entropy-stable (ESDG) operator is built "
                                   "with quadrature_tag=DISCR_TAG_QUAD")
@@ -36,7 +36,14 @@ def check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
         raise NotImplementedError(f"quadrature_tag {quadrature_tag!r} is not "
                                   "supported here")
     if operator_states_quad is not None:
-        raise NotImplementedError("operator_states_quad is not supported")
+        # the kernels form the face states themselves: accept the states of
+        # make_operator_fluid_states made for this very call
+        from mirgecom_amd.gas_model import OperatorFluidStates
+        if not isinstance(operator_states_quad, OperatorFluidStates):
+            raise NotImplementedError(
+                "operator_states_quad must come from mirgecom_amd.gas_model."
+                "make_operator_fluid_states")
+        operator_states_quad.check(state, quadrature_tag, boundaries)
     if limiter_func is not None or entropy_min is not None:
         raise NotImplementedError("limiters are not part of the device path")
 

@@ -45,12 +45,18 @@ def euler_operator(dcoll, state, gas_model, boundaries, time=0.0,
                                   "operator")
     check_dd(dd)
     check_unsupported(quadrature_tag, use_esdg, operator_states_quad,
-                      limiter_func, entropy_min, allow_quad=True)
+                      limiter_func, entropy_min, allow_quad=True,
+                      state=state, boundaries=boundaries)
     warn_mixture_limiter(state, limiter_func)
     if use_esdg:
+        # the volume points take operator_states_quad's volume state when
+        # given (euler.py:159-171): entropy-stable projected only if those
+        # states were made with entropy_stable=True
+        es_volume = (operator_states_quad.entropy_stable
+                     if operator_states_quad is not None else True)
         return _euler_esdg(dcoll, state, gas_model, boundaries, time,
                            inviscid_numerical_flux_func,
-                           entropy_conserving_flux_func)
+                           entropy_conserving_flux_func, es_volume=es_volume)
     if inviscid_numerical_flux_func is None:
         warn("inviscid_numerical_flux_func unspecified, defaulting to "
              "inviscid_facial_flux_rusanov.", stacklevel=2)
@@ -86,12 +92,13 @@ def entropy_stable_euler_operator(
         limiter_func=limiter_func, entropy_min=entropy_min)
 
 
-def _esdg_for_ns(dcoll, state, gas_model, boundaries, time):
+def _esdg_for_ns(dcoll, state, gas_model, boundaries, time, es_volume=False):
     """entropy_stable_euler_operator as ns_operator calls it
-    (mirgecom/navierstokes.py:564-570): with ns_operator's own operator
-    states, so the volume points take the plain projected state."""
+    (mirgecom/navierstokes.py:564-570): with ns_operator's operator states,
+    whose volume state is the plain projected one unless the caller's
+    states were made with entropy_stable=True."""
     return _euler_esdg(dcoll, state, gas_model, boundaries, time, None, None,
-                       es_volume=False)
+                       es_volume=es_volume)
 
 
 def _euler_quad(dcoll, state, gas_model, boundaries, time, kind):

@@ -108,7 +108,8 @@ def grad_cv_operator(dcoll, gas_model, boundaries, state, *, time=0.0,
     check_dd(dd)
     check_unsupported(quadrature_tag, use_esdg,
                       operator_states_quad, limiter_func, entropy_min,
-                      allow_quad=True)
+                      allow_quad=True,
+                      state=state, boundaries=boundaries)
     _check_grad_flux(numerical_flux_func)
     _, grad = _gradients(dcoll, gas_model, boundaries, state, time,
                          quadrature_tag)
@@ -126,7 +127,8 @@ def grad_t_operator(dcoll, gas_model, boundaries, state, *, time=0.0,
     check_dd(dd)
     check_unsupported(quadrature_tag, use_esdg,
                       operator_states_quad, limiter_func, entropy_min,
-                      allow_quad=True)
+                      allow_quad=True,
+                      state=state, boundaries=boundaries)
     _check_grad_flux(numerical_flux_func)
     _, grad = _gradients(dcoll, gas_model, boundaries, state, time,
                          quadrature_tag)
@@ -160,7 +162,8 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
     check_dd(dd)
     check_unsupported(quadrature_tag, use_esdg,
                       operator_states_quad, limiter_func, entropy_min,
-                      allow_quad=True)
+                      allow_quad=True,
+                      state=state, boundaries=boundaries)
     from mirgecom_amd.euler import entropy_stable_euler_operator, euler_operator
     if inviscid_fluid_operator is None and use_esdg:
         inviscid_fluid_operator = entropy_stable_euler_operator
@@ -264,7 +267,10 @@ def ns_operator(dcoll, gas_model, state, boundaries, *, time=0.0,
             # the reference hands its own (plain projected) operator states
             # to the entropy-stable operator
             from mirgecom_amd.euler import _esdg_for_ns
-            out = out + _esdg_for_ns(dcoll, state, gas_model, boundaries, time)
+            out = out + _esdg_for_ns(
+                dcoll, state, gas_model, boundaries, time,
+                es_volume=(operator_states_quad is not None
+                           and operator_states_quad.entropy_stable))
         else:
             out = out + inviscid_fluid_operator(
                 dcoll, state=state, gas_model=gas_model, boundaries=boundaries,
