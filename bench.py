@@ -253,7 +253,7 @@ def measure_copy(dev):
             "datasheet_GB_s": HBM_PEAK_GBS, "frac_of_datasheet": gbs / HBM_PEAK_GBS}
 
 
-def config1_line(cores):
+def vortex_cpu_line(cores):
     """2-D Euler isentropic vortex, 2,048 triangles, p = 3: the FP64 oracle's
     RHS timed on the host (the CPU path), the GPU RHS timed, and the L-inf
     error of the GPU march to t = 1 against the exact advected vortex (the
@@ -836,7 +836,7 @@ def _rel_err_cv(a, b, dim):
 
 
 def mixture_kernels(args):
-    """The reacting box's right-hand sides (config 3 state, 32^3 x 6 p = 2):
+    """The reacting box's right-hand sides (32^3 x 6 tets, p = 2, uiuc_7sp):
     Euler FP64 and mixed, Navier-Stokes with constant transport FP64 and
     mixed, each evaluated args.mixture_kernels times; then, on the quadrature
     of degree 4, the overintegrated Navier-Stokes RHS and the entropy-stable
@@ -1041,11 +1041,11 @@ def main():
     if args.ens_cases:
         ensemble = measure_ensemble(args, model)
     overint = measure_overintegration(args, dcoll, gm, cv)
-    cpu = config1 = None
+    cpu = vortex_line = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(args.cpu_n, order)
-        config1 = config1_line(cpu["cores"])
-        print(f"# config1: {json.dumps(config1)}", file=sys.stderr, flush=True)
+        vortex_line = vortex_cpu_line(cpu["cores"])
+        print(f"# vortex 2d: {json.dumps(vortex_line)}", file=sys.stderr, flush=True)
 
     ms = ms32 if mixed_ok else ms64
     kern, ab = (k32, abytes32) if mixed_ok else (k64, abytes)
@@ -1101,7 +1101,7 @@ def main():
                    "reacting_ns": reacting_ns,
                    "surrogate": surrogate,
                    "ensemble": ensemble,
-                   "config1": config1,
+                   "vortex_2d": vortex_line,
                    "overintegration": overint},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

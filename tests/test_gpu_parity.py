@@ -39,8 +39,8 @@ def _flux_pair(name):
 @pytest.mark.parametrize("order", [1, 2, 3, 4])
 @pytest.mark.parametrize("flux", ["rusanov", "hll", "central"])
 def test_euler_vortex_prescribed(order, flux):
-    """2-D isentropic vortex with exact prescribed boundaries (config-1
-    setting on a coarser mesh)."""
+    """2-D isentropic vortex with exact prescribed boundaries (the
+    [0,10]x[-5,5] vortex workload on a coarser mesh)."""
     from mirgecom_amd.boundary import PrescribedFluidBoundary
     from mirgecom_amd.eos import IdealSingleGas
     from mirgecom_amd.euler import euler_operator
@@ -282,7 +282,7 @@ def test_uniform_state_zero_rhs():
 
 
 def test_rk4_vortex_march():
-    """Ten RK4 steps of the 2-D vortex (config-1 setting, coarser mesh):
+    """Ten RK4 steps of the 2-D vortex (the vortex workload on a coarser mesh):
     fused device RK4 vs oracle RK4."""
     from mirgecom_amd.boundary import PrescribedFluidBoundary
     from mirgecom_amd.eos import IdealSingleGas

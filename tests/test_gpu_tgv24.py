@@ -61,7 +61,7 @@ def test_tgv_fp64_against_oracle(n, flux):
     from mirgecom_amd.navierstokes import ns_operator
     from oracle import operators as oop, physics as ph
 
-    print(f"config-2 TGV vs oracle n={n} flux={flux}", flush=True)
+    print(f"TGV vs oracle n={n} flux={flux}", flush=True)
     pb = Problem(3, 3, (n,) * 3, 0.0, 2 * math.pi, (True,) * 3)
     gm = _gas()
     # the product's initializer gives the same state as the closed form
@@ -97,7 +97,7 @@ def test_tgv_fp64_against_oracle(n, flux):
 @pytest.fixture(scope="module")
 def full_size():
     from mirgecom_amd.workloads import taylor_green
-    print("config-2 TGV full size 24^3 x 6 tets, p=3", flush=True)
+    print("TGV full size 24^3 x 6 tets, p=3", flush=True)
     return taylor_green(n=24, order=3)
 
 

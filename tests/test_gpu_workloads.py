@@ -21,7 +21,7 @@ def _oracle_disc(dcoll):
     return od, (m.a, m.b, m.periodic)
 
 
-def test_config1_vortex_rk4():
+def test_vortex_2d_rk4_march():
     """2-D Euler vortex, 2,048 triangles, p = 3: the RHS against the oracle
     (1e-12), 10 RK4 steps at dt = 0.005 against the oracle's, and the march
     to t = 1 (200 steps) against the exact advected vortex (the FP64 oracle
@@ -57,13 +57,13 @@ def test_config1_vortex_rk4():
     cv = wl.cv
     q = cv.q.cpu().numpy()
     err0 = rel_err(rhs(0.0, cv).q.cpu().numpy(), orhs(0.0, q))
-    print("config1 RHS vs oracle", err0)
+    print("2-D vortex RHS vs oracle", err0)
     assert err0.max() < 1e-12
     for k in range(10):
         cv = rk4_step(cv, k * wl.dt, wl.dt, rhs)
         q = oop.rk4_step(q, k * wl.dt, wl.dt, orhs)
     err = rel_err(cv.q.cpu().numpy(), q)
-    print("config1 10 steps vs oracle", err)
+    print("2-D vortex 10 steps vs oracle", err)
     # 40 RHS evaluations, each within ~1e-13 of the oracle, accumulate
     assert err.max() < 1e-11
     nsteps = int(round(1.0 / wl.dt))
@@ -71,11 +71,11 @@ def test_config1_vortex_rk4():
         cv = rk4_step(cv, k * wl.dt, wl.dt, rhs)
     exact = vortex(wl.dcoll.nodes(), time=1.0).q
     linf = (cv.q - exact).abs().max().item()
-    print("config1 L-inf error vs exact at t=1:", linf)
+    print("2-D vortex L-inf error vs exact at t=1:", linf)
     assert abs(linf - 1.7679e-2) < 1e-4
 
 
-def test_config3_reacting_rk4_reduced():
+def test_reacting_box_rk4_reduced():
     """3-D reacting box (uiuc_7sp, p = 2) on 3^3 cells: RK4 at dt = 1e-8 with
     the temperature seeded from the previous stage, Euler + chemistry, vs
     the oracle."""
@@ -127,7 +127,7 @@ def test_config3_reacting_rk4_reduced():
     # the species change over three steps (chemistry-driven), which the
     # state error would hide; mass/energy/momentum barely move from rest
     derr = rel_err((qg - q0)[5:], (q - q0)[5:])
-    print("config3 3 steps", err, "increment", derr)
+    print("reacting box 3 steps", err, "increment", derr)
     # momentum starts at rest: after three steps it is all increment,
     # formed from pressure-gradient cancellation
     assert err[[0, 1] + list(range(5, 12))].max() < 1e-12
