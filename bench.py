@@ -257,7 +257,7 @@ def vortex_cpu_line(cores):
     """2-D Euler isentropic vortex, 2,048 triangles, p = 3: the FP64 oracle's
     RHS timed on the host (the CPU path), the GPU RHS timed, and the L-inf
     error of the GPU march to t = 1 against the exact advected vortex (the
-    GPU RHS equals the oracle's to 1e-12, tests/test_gpu_configs.py)."""
+    GPU RHS equals the oracle's to 1e-12, tests/test_gpu_workloads.py)."""
     import numpy as np
     import torch
 
