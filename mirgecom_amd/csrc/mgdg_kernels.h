@@ -553,14 +553,6 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   for (int t = threadIdx.x; t < NFN; t += blockDim.x) s_fmask[t] = a.fmask[t];
   for (int t = threadIdx.x; t < EPB * DIM * DIM; t += blockDim.x)
     s_geo[t] = t < nel * DIM * DIM ? a.rx[e0 * DIM * DIM + t] : 0.0;
-  if constexpr (MIXED && MODE == kFlux) {
-    for (int t = threadIdx.x; t < EPB * NODEF; t += blockDim.x) {
-      const int le = t / NODEF;
-      const int k = t - le * NODEF;
-      s_ref[t] = le < nel ? double(a.fnet32[k * N + (e0 + le) * NP]) : 0.0;
-    }
-    __syncthreads();
-  }
 
   // phase 1: net flux at the volume nodes (coalesced; fixed trip count)
 #pragma unroll
@@ -579,11 +571,20 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
       node_state<DIM, NSP, MIX, false>(s, a.gas, MIX ? a.temp[g] : 0.0);
       s_ws[t] = s.speed + s.c;
       if constexpr (MIXED) {
-        // relative to the element's first node (s_ref holds its FP32 values
-        // exactly)
-        const double* ref = s_ref + le * NODEF;
+        // relative to the element's first node: every thread of the element
+        // loads that node's values (cache hits after the first), node 0's
+        // thread keeps them in s_ref (exact in FP64) for the face phase,
+        // which reads them after the barrier below
+        const int64_t g0 = e * NP;
+        float r[NODEF];
 #pragma unroll
-        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet32[k * N + g] - float(ref[k]);
+        for (int k = 0; k < NODEF; ++k) r[k] = a.fnet32[k * N + g0];
+#pragma unroll
+        for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet32[k * N + g] - r[k];
+        if (i == 0) {
+#pragma unroll
+          for (int k = 0; k < NODEF; ++k) s_ref[le * NODEF + k] = double(r[k]);
+        }
       } else {
 #pragma unroll
         for (int k = 0; k < NODEF; ++k) fdst[k] = a.fnet[k * N + g];
@@ -1033,29 +1034,12 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
       v = u < DIM * DIM ? a.rx[e * DIM * DIM + u] : a.normals[e * NF * DIM + u - DIM * DIM];
     s_geo[t] = v;
   }
-  if constexpr (MIXED) {
-    // per-element reference values: q and T at the element's first node
-    for (int t = threadIdx.x; t < EPB; t += blockDim.x) {
-      double qv[NFLD];
-      if (t < nel) {
-        const int64_t g0 = (e0 + t) * NP;
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) qv[f] = a.q[f * N + g0];
-        s_ref[t * NS + NFLD] =
-            node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g0] : 0.0);
-      } else {
-#pragma unroll
-        for (int f = 0; f < NFLD; ++f) qv[f] = 0.0;
-        s_ref[t * NS + NFLD] = 0.0;
-      }
-#pragma unroll
-      for (int f = 0; f < NFLD; ++f) s_ref[t * NS + f] = qv[f];
-    }
-    __syncthreads();
-  }
-
   // phase 1: conserved fields and T at the volume nodes.  With one node per
   // thread the node's state stays in registers for the flux epilogue.
+  // Mixed precision stores them relative to the element's first node (q and
+  // T there): every thread of the element forms that reference itself (its
+  // loads hit the cache after the first), node 0's thread keeps it in s_ref
+  // for the face phase, which reads it after the barrier below.
   constexpr bool KEEPQ = FLUX && EPB * NP <= kBlock;
   double qkeep[KEEPQ ? NFLD : 1];
   for (int t = threadIdx.x; t < EPB * NP; t += blockDim.x) {
@@ -1072,10 +1056,23 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
       for (int f = 0; f < NFLD; ++f) qkeep[f] = qv[f];
     }
     const double tn = node_temperature<DIM, NSP, MIX>(qv, a.gas, MIX ? a.temp[g] : 0.0);
-    const double* ref = s_ref + le * NS;
+    double ref[NFLD + 1];
+    if constexpr (MIXED) {
+      const int64_t g0 = e * NP;
 #pragma unroll
-    for (int f = 0; f < NFLD; ++f) s_b[i * ROW + le * NS + f] = R(qv[f] - (MIXED ? ref[f] : 0.0));
-    s_b[i * ROW + le * NS + NFLD] = R(tn - (MIXED ? ref[NFLD] : 0.0));
+      for (int f = 0; f < NFLD; ++f) ref[f] = a.q[f * N + g0];
+      ref[NFLD] = node_temperature<DIM, NSP, MIX>(ref, a.gas, MIX ? a.temp[g0] : 0.0);
+      if (i == 0) {
+#pragma unroll
+        for (int f = 0; f <= NFLD; ++f) s_ref[le * NS + f] = ref[f];
+      }
+    } else {
+#pragma unroll
+      for (int f = 0; f <= NFLD; ++f) ref[f] = 0.0;
+    }
+#pragma unroll
+    for (int f = 0; f < NFLD; ++f) s_b[i * ROW + le * NS + f] = R(qv[f] - ref[f]);
+    s_b[i * ROW + le * NS + NFLD] = R(tn - ref[NFLD]);
   }
   __syncthreads();
 
