@@ -484,10 +484,25 @@ template <int DIM, int P, int NSP, int MIX, int MODE, typename R>
 struct RhsCfg {
   using E = Elem<DIM, P>;
   static constexpr int NFLD = 2 + DIM + NSP;
-  static constexpr int EPB = 64 / NFLD;           // columns fit one wave
-  static constexpr int COLS = EPB * NFLD;
   static constexpr int NODEF = NFLD * DIM;        // net flux
   static constexpr int NODEP = odd_stride(NODEF);  // LDS node stride
+  // LDS of a workgroup of epb elements (the few-field FP64 kernels alias the
+  // face buffer, below, and never take two column groups)
+  static constexpr size_t lds_bytes(int epb) {
+    return sizeof(R) * (size_t(epb) * E::NP * NODEP + size_t(E::NFN) * odd_stride(epb * NFLD)) +
+           sizeof(double) * (size_t(epb) * DIM * DIM + (sizeof(R) == 4 ? epb * NODEF : 0) +
+                             size_t(epb) * E::NP) +
+           sizeof(int) * E::NFN;
+  }
+  // (element, field) columns of the contraction: one per lane of a wave, or,
+  // for the mixed-precision 12-field mixture kernels, two per lane: twice
+  // the elements per workgroup, so the node and face phases keep twice the
+  // lanes busy, where the LDS still holds the five workgroups per CU that
+  // their registers allow (p <= 2; the FP64 buffers would halve them)
+  static constexpr int CG =
+      (NFLD > 6 && sizeof(R) == 4 && 5 * lds_bytes(128 / NFLD) <= 160 * 1024) ? 2 : 1;
+  static constexpr int EPB = 64 * CG / NFLD;
+  static constexpr int COLS = EPB * NFLD;
   static constexpr int FROW = odd_stride(COLS);    // LDS face-row stride
   static constexpr int SP = odd_stride(EPB * E::NP);  // staging plane stride
   static constexpr int SF = EPB * E::NP * NODEP;
@@ -511,6 +526,8 @@ struct RhsCfg {
   // FP64 five-field kernels spills and runs slower)
   static constexpr int WAVES = (sizeof(R) == 4 && NFLD > 6 && MODE == kFlux) ? 5 : 1;
   static_assert(EPB >= 1, "too many fields");
+  static_assert(COLS <= 64 * CG, "columns exceed the lanes' column groups");
+  static_assert(!(ALIAS && CG > 1), "aliased face buffer with two column groups");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
 
@@ -877,38 +894,53 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   // phase 3: rhs[i] = sum_j sum_r Dw_r[i][j] G_r[j] + sum_m LIFT[i][m] B[m]
   // per (element, field) column, G_r = sum_d rx[r][d] F_d; rows split over
   // the waves, operator entries wave-uniform
+  // (each lane holds CG columns, c and c + 64)
+  constexpr int CG = C::CG;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int c = threadIdx.x & 63;
-  const int cc = c < COLS ? c : 0;
-  const int cle = cc / NFLD;
-  const int cf = cc - cle * NFLD;
-  R rxl[DIM][DIM];
+  int cc[CG], cle[CG], cf[CG];
 #pragma unroll
-  for (int r = 0; r < DIM; ++r)
+  for (int g = 0; g < CG; ++g) {
+    cc[g] = c + 64 * g < COLS ? c + 64 * g : 0;
+    cle[g] = cc[g] / NFLD;
+    cf[g] = cc[g] - cle[g] * NFLD;
+  }
+  R rxl[CG][DIM][DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) rxl[r][d] = R(s_geo[cle * DIM * DIM + r * DIM + d]);
+  for (int g = 0; g < CG; ++g)
+#pragma unroll
+    for (int r = 0; r < DIM; ++r)
+#pragma unroll
+      for (int d = 0; d < DIM; ++d) rxl[g][r][d] = R(s_geo[cle[g] * DIM * DIM + r * DIM + d]);
   const R* DV = ops + w * (NP * DIM * RPW);
   const R* LF = ops + O::DVOL + w * (NFN * RPW);
-  R acc[RPW];
+  R acc[CG][RPW];
 #pragma unroll
-  for (int rr = 0; rr < RPW; ++rr) acc[rr] = R(0);
-  R f0[DIM];   // this column's reference (absolute FP32 Euler fluxes)
+  for (int g = 0; g < CG; ++g)
 #pragma unroll
-  for (int d = 0; d < DIM; ++d)
-    f0[d] = (MIXED && !REL) ? s_f[(cle * NP) * NODEP + cf * DIM + d] : R(0);
+    for (int rr = 0; rr < RPW; ++rr) acc[g][rr] = R(0);
+  R f0[CG][DIM];   // the columns' references (absolute FP32 Euler fluxes)
+#pragma unroll
+  for (int g = 0; g < CG; ++g)
+#pragma unroll
+    for (int d = 0; d < DIM; ++d)
+      f0[g][d] = (MIXED && !REL) ? s_f[(cle[g] * NP) * NODEP + cf[g] * DIM + d] : R(0);
 #pragma unroll 2
   for (int j = 0; j < NP; ++j) {
-    const R* fj = s_f + (cle * NP + j) * NODEP + cf * DIM;
-    R fd[DIM];
 #pragma unroll
-    for (int d = 0; d < DIM; ++d) fd[d] = fj[d] - f0[d];
+    for (int g = 0; g < CG; ++g) {
+      const R* fj = s_f + (cle[g] * NP + j) * NODEP + cf[g] * DIM;
+      R fd[DIM];
 #pragma unroll
-    for (int r = 0; r < DIM; ++r) {
-      R gr = R(0);
+      for (int d = 0; d < DIM; ++d) fd[d] = fj[d] - f0[g][d];
 #pragma unroll
-      for (int d = 0; d < DIM; ++d) gr += rxl[r][d] * fd[d];
+      for (int r = 0; r < DIM; ++r) {
+        R gr = R(0);
 #pragma unroll
-      for (int rr = 0; rr < RPW; ++rr) acc[rr] += DV[(j * DIM + r) * RPW + rr] * gr;
+        for (int d = 0; d < DIM; ++d) gr += rxl[g][r][d] * fd[d];
+#pragma unroll
+        for (int rr = 0; rr < RPW; ++rr) acc[g][rr] += DV[(j * DIM + r) * RPW + rr] * gr;
+      }
     }
   }
   if constexpr (ALIAS) {
@@ -923,18 +955,24 @@ __device__ __forceinline__ void rhs_body(const A& a, const double* __restrict__ 
   }
 #pragma unroll 4
   for (int m = 0; m < NFN; ++m) {
-    const R bv = s_face[m * FROW + cc];
 #pragma unroll
-    for (int rr = 0; rr < RPW; ++rr) acc[rr] += LF[m * RPW + rr] * bv;
+    for (int g = 0; g < CG; ++g) {
+      const R bv = s_face[m * FROW + cc[g]];
+#pragma unroll
+      for (int rr = 0; rr < RPW; ++rr) acc[g][rr] += LF[m * RPW + rr] * bv;
+    }
   }
   __syncthreads();
   // phase 4: stage in LDS (over s_f), write coalesced
   R* stage = s_f;                            // [NFLD][SP]
-  if (c < COLS) {
 #pragma unroll
-    for (int rr = 0; rr < RPW; ++rr) {
-      const int i = w * RPW + rr;
-      if (i < NP) stage[cf * SP + cle * NP + i] = acc[rr];
+  for (int g = 0; g < CG; ++g) {
+    if (c + 64 * g < COLS) {
+#pragma unroll
+      for (int rr = 0; rr < RPW; ++rr) {
+        const int i = w * RPW + rr;
+        if (i < NP) stage[cf[g] * SP + cle[g] * NP + i] = acc[g][rr];
+      }
     }
   }
   __syncthreads();
