@@ -151,7 +151,7 @@ mg_status launch_rhs(const mgdg::KArgs& a, hipStream_t st) {
 
 template <int DIM, int P, int NSP, int MIX, bool FLUX, typename R>
 mg_status launch_grad(const mgdg::KArgs& a, hipStream_t st) {
-  using C = mgdg::GradCfg<DIM, P, NSP, MIX, R>;
+  using C = mgdg::GradCfg<DIM, P, NSP, MIX, R, FLUX>;
   auto kern = mgdg::grad_kernel<DIM, P, NSP, MIX, FLUX, R>;
   static bool attr = false;
   if (!attr) {

@@ -1005,22 +1005,49 @@ void rhs_kernel(KArgs a,
 // runs in FP32, the physics in FP64 and the nodal fluxes are stored in FP32
 // with p_ref removed from the momentum-flux diagonal.
 
-template <int DIM, int P, int NSP, int MIX, typename R>
+template <int DIM, int P, int NSP, int MIX, typename R, bool FLUX = false>
 struct GradCfg {
   using E = Elem<DIM, P>;
   static constexpr int NS = 3 + DIM + NSP;  // conserved fields + T
-  static constexpr int EPB = 64 / NS;
-  static constexpr int COLS = 64;
-  static constexpr int ROW = odd_stride(COLS);         // LDS trace-row stride
-  static constexpr int SP = odd_stride(EPB * E::NP);   // gradient plane stride
   static constexpr int KG = E::NP + E::NFN;
   static constexpr int GEO = DIM * DIM + E::NF * DIM;  // rx and normals
+  // LDS of a workgroup of cg * 64 columns
+  static constexpr size_t lds_bytes(int cg) {
+    const int epb = 64 * cg / NS;
+    const size_t sb = size_t(KG) * odd_stride(64 * cg);
+    const size_t stage = size_t(NS) * DIM * odd_stride(epb * E::NP);
+    return sizeof(R) * (sb > stage ? sb : stage) +
+           sizeof(double) * (size_t(epb) * GEO + (sizeof(R) == 4 ? epb * NS : 0)) +
+           sizeof(int) * E::NFN;
+  }
+  // elements resident per CU: workgroups by LDS, at most those the
+  // registers allow (the mixed-precision gradient pass of the 13-scalar
+  // mixture with one 16-row tile, p <= 2 in 3-D: 60 VGPRs, eight waves per
+  // SIMD, with one column group; 80, six waves, with two.  With more tiles
+  // the second group's accumulators cost a half of the waves or more.)
+  static constexpr int resident(int cg) {
+    const int by_lds = int((160 * 1024) / lds_bytes(cg));
+    const int by_regs = cg == 1 ? 8 : 6;
+    return (64 * cg / NS) * (by_lds < by_regs ? by_lds : by_regs);
+  }
+  // (element, scalar) columns: 64, or 128 (two column groups per wave) for
+  // the mixed-precision mixture gradients where that keeps more elements
+  // resident: twice the elements per workgroup, so the node and face phases
+  // keep twice the lanes busy
+  static constexpr int CG = (NS > 7 && !FLUX && sizeof(R) == 4 && E::NP <= 16 &&
+                             resident(2) > resident(1)) ? 2 : 1;
+  static constexpr int EPB = 64 * CG / NS;
+  static constexpr int COLS = 64 * CG;
+  static constexpr int ROW = odd_stride(COLS);         // LDS trace-row stride
+  static constexpr int SP = odd_stride(EPB * E::NP);   // gradient plane stride
   static constexpr int STAGE = NS * DIM * SP;
   static constexpr int SB = KG * ROW > STAGE ? KG * ROW : STAGE;
   static constexpr int SREF = sizeof(R) == 4 ? EPB * NS : 0;
   static constexpr size_t LDS = sizeof(R) * SB + sizeof(double) * (EPB * GEO + SREF) +
                                 sizeof(int) * E::NFN;
   static_assert(EPB >= 1, "too many fields");
+  static_assert(CG == 1 || sizeof(R) == 4, "two column groups: matrix-core contraction only");
+  static_assert(LDS == lds_bytes(CG), "LDS layout");
   static_assert(LDS <= 160 * 1024, "LDS per workgroup exceeds 160 KiB");
 };
 
@@ -1044,7 +1071,7 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
                                           double* __restrict__ out, int64_t blk) {
   using E = Elem<DIM, P>;
   using O = Ops<DIM, P>;
-  using C = GradCfg<DIM, P, NSP, MIX, R>;
+  using C = GradCfg<DIM, P, NSP, MIX, R, FLUX>;
   constexpr int NP = E::NP, NFP = E::NFP, NF = E::NF, NFN = E::NFN;
   constexpr int NFLD = 2 + DIM + NSP, NS = C::NS, EPB = C::EPB, COLS = C::COLS;
   constexpr int ROW = C::ROW, SP = C::SP;
@@ -1176,33 +1203,42 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
   const int c = threadIdx.x & 63;
   if constexpr (MIXED) {
     // phase 3 on the matrix cores (v_mfma_f32_16x16x4f32): wave w owns the
-    // 16 columns [16w, 16w + 16); per 16-row tile of each output
-    // W_r = Dw_r u and L_f = LIFT_f (fs u*) the K dimension (nodes) runs in
-    // steps of 4.  Lane l supplies B[k = l / 16][col = l % 16] and holds
-    // D[4 (l / 16) + reg][l % 16] of each tile.
+    // 16 columns [16w, 16w + 16) of each group of 64 (CG groups); per
+    // 16-row tile of each output W_r = Dw_r u and L_f = LIFT_f (fs u*) the
+    // K dimension (nodes) runs in steps of 4.  Lane l supplies
+    // B[k = l / 16][col = l % 16] and holds D[4 (l / 16) + reg][l % 16] of
+    // each tile; the groups share the operator fragments.
     using O = Ops<DIM, P>;
-    constexpr int MT = O::MT, KSV = O::KSV, KSF = O::KSF;
+    constexpr int MT = O::MT, KSV = O::KSV, KSF = O::KSF, CG = C::CG;
     const float* mf = mfma_ops<DIM, P>(ws);
-    const int col = w * 16 + (c & 15);
+    const int col0 = w * 16 + (c & 15);
     const int kq = c >> 4;
-    floatx4 aw[DIM][MT], al[NF][MT];
+    floatx4 aw[CG][DIM][MT], al[CG][NF][MT];
 #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
+    for (int g = 0; g < CG; ++g)
 #pragma unroll
-      for (int r = 0; r < DIM; ++r) aw[r][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      for (int mt = 0; mt < MT; ++mt) {
 #pragma unroll
-      for (int f = 0; f < NF; ++f) al[f][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
-    }
+        for (int r = 0; r < DIM; ++r) aw[g][r][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int f = 0; f < NF; ++f) al[g][f][mt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      }
 #pragma unroll
     for (int st = 0; st < KSV; ++st) {
       const int j = st * 4 + kq;
-      const float bv = float(s_b[(j < NP ? j : NP - 1) * ROW + col]);  // A is 0 past NP
+      float bv[CG];
+#pragma unroll
+      for (int g = 0; g < CG; ++g)  // A is 0 past NP
+        bv[g] = float(s_b[(j < NP ? j : NP - 1) * ROW + 64 * g + col0]);
 #pragma unroll
       for (int r = 0; r < DIM; ++r)
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-          aw[r][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-              mf[((r * MT + mt) * KSV + st) * 64 + c], bv, aw[r][mt], 0, 0, 0);
+        for (int mt = 0; mt < MT; ++mt) {
+          const float av = mf[((r * MT + mt) * KSV + st) * 64 + c];
+#pragma unroll
+          for (int g = 0; g < CG; ++g)
+            aw[g][r][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[g], aw[g][r][mt], 0, 0, 0);
+        }
     }
     // faces innermost: NF * MT independent accumulations per step
 #pragma unroll
@@ -1211,40 +1247,49 @@ __device__ __forceinline__ void grad_body(const A& a, const double* __restrict__
       const int mm = m < NFP ? m : NFP - 1;
 #pragma unroll
       for (int f = 0; f < NF; ++f) {
-        const float bv = float(s_b[(NP + f * NFP + mm) * ROW + col]);
+        float bv[CG];
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
-          al[f][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-              mf[O::MFV + ((f * MT + mt) * KSF + st) * 64 + c], bv, al[f][mt], 0, 0, 0);
+        for (int g = 0; g < CG; ++g) bv[g] = float(s_b[(NP + f * NFP + mm) * ROW + 64 * g + col0]);
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const float av = mf[O::MFV + ((f * MT + mt) * KSF + st) * 64 + c];
+#pragma unroll
+          for (int g = 0; g < CG; ++g)
+            al[g][f][mt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, bv[g], al[g][f][mt], 0, 0, 0);
+        }
       }
     }
     __syncthreads();
     // phase 4: stage the gradients [NS][DIM][EPB*NP] (over the trace buffer)
-    if (col < EPB * NS) {
-      const int le = col / NS;
-      const int sc = col - le * NS;
-      const double* geo = s_geo + le * GEO;
-      // one direction at a time: only its 7 geometric factors are live
+#pragma unroll
+    for (int g = 0; g < CG; ++g) {
+      const int col = 64 * g + col0;
+      if (col < EPB * NS) {
+        const int le = col / NS;
+        const int sc = col - le * NS;
+        const double* geo = s_geo + le * GEO;
+        // one direction at a time: only its 7 geometric factors are live
 #pragma unroll 1
-      for (int d = 0; d < DIM; ++d) {
-        float rxd[DIM], nfd[NF];
+        for (int d = 0; d < DIM; ++d) {
+          float rxd[DIM], nfd[NF];
 #pragma unroll
-        for (int r = 0; r < DIM; ++r) rxd[r] = float(geo[r * DIM + d]);
+          for (int r = 0; r < DIM; ++r) rxd[r] = float(geo[r * DIM + d]);
 #pragma unroll
-        for (int f = 0; f < NF; ++f) nfd[f] = float(geo[DIM * DIM + f * DIM + d]);
+          for (int f = 0; f < NF; ++f) nfd[f] = float(geo[DIM * DIM + f * DIM + d]);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt)
+          for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
-          for (int reg = 0; reg < 4; ++reg) {
-            const int i = mt * 16 + 4 * kq + reg;
-            if (i >= NP) continue;
-            float vsum = 0.f, fsum = 0.f;
+            for (int reg = 0; reg < 4; ++reg) {
+              const int i = mt * 16 + 4 * kq + reg;
+              if (i >= NP) continue;
+              float vsum = 0.f, fsum = 0.f;
 #pragma unroll
-            for (int r = 0; r < DIM; ++r) vsum += rxd[r] * aw[r][mt][reg];
+              for (int r = 0; r < DIM; ++r) vsum += rxd[r] * aw[g][r][mt][reg];
 #pragma unroll
-            for (int f = 0; f < NF; ++f) fsum += nfd[f] * al[f][mt][reg];
-            stage[(sc * DIM + d) * SP + le * NP + i] = R(fsum - vsum);
-          }
+              for (int f = 0; f < NF; ++f) fsum += nfd[f] * al[g][f][mt][reg];
+              stage[(sc * DIM + d) * SP + le * NP + i] = R(fsum - vsum);
+            }
+        }
       }
     }
   } else {
