@@ -129,3 +129,43 @@ def test_reacting_box_march():
     print("reacting box march: mixed vs FP64", err, flush=True)
     assert np.all(np.isfinite(a))
     assert err.max() < 1e-6
+
+
+@pytest.mark.parametrize("dim,order,n", [(3, 1, (4, 4, 4)), (3, 2, (4, 4, 4)),
+                                         (3, 3, (3, 3, 4)), (2, 2, (7, 7)),
+                                         (2, 4, (7, 7))])
+def test_mixture_column_groups(dim, order, n):
+    """The mixed-precision mixture kernels take two contraction columns per
+    lane where the LDS allows (3-D p <= 2: 10 elements per RHS workgroup, 9
+    per gradient workgroup; 2-D: 11 and 10), one otherwise (3-D p = 3).  The
+    element counts here (384, 216, 98) are no multiple of those, so the last
+    workgroup is partial.  Euler and Navier-Stokes RHS against FP64 within
+    2e-5 (Euler; at p = 4 the FP32 contraction's rounding reaches 1.2e-5 on
+    a momentum component, bit for bit as with one column per lane) and 1e-3
+    (NS: FP32 gradients and nodal fluxes, the bound of the mixed mixture NS
+    tests) per field, with the Rusanov flux, and with HLL for Euler."""
+    from mirgecom_amd.eos import PyrometheusMixture
+    from mirgecom_amd.fluid import ConservedVars
+    from mirgecom_amd.gas_model import GasModel, make_fluid_state
+    from mirgecom_amd.navierstokes import ns_operator
+    from mirgecom_amd.transport import SimpleTransport
+    pb = Problem(dim, order, n, 0.0, 2 * np.pi, (True,) * dim)
+    q, t = mixture_state(pb.x, seed=41 + order)
+    gm = GasModel(eos=PyrometheusMixture())
+    st = make_fluid_state(ConservedVars(dim, _dev(q)), gm, temperature_seed=_dev(t))
+    for flux in ("rusanov", "hll"):
+        a, b = _both(pb.dcoll, st, gm, {}, flux)
+        err = rel_err_cv(a, b, dim)
+        print(f"column groups dim={dim} p={order} K={pb.dcoll.mesh.nelements} "
+              f"Euler {flux}: {err}", flush=True)
+        assert err.max() < 2e-5
+    gmn = GasModel(eos=PyrometheusMixture(), transport=SimpleTransport(
+        viscosity=5e-5, thermal_conductivity=0.1, species_diffusivity=np.full(7, 2.3e-4)))
+    stn = make_fluid_state(ConservedVars(dim, _dev(q)), gmn, temperature_seed=_dev(t))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")
+        r64 = ns_operator(pb.dcoll, gmn, stn, {})
+        r32 = ns_operator(pb.dcoll, gmn, stn, {}, precision="mixed")
+    err = rel_err(_host(r32.q), _host(r64.q))
+    print(f"column groups dim={dim} p={order} NS: {err}", flush=True)
+    assert err.max() < 1e-3
