@@ -657,9 +657,10 @@ def measure_surrogate(args, wl):
 
 def measure_ensemble(args, model):
     """48 (args.ens_cases) reacting Navier-Stokes cases of uneven size and
-    order in shared launches: the launch shape autotuned, graph-replayed RK4
-    steps against the same steps with per-case launches, and each case's
-    RHS error against its FP64 exact-chemistry RHS."""
+    order in shared launches: the launch shape autotuned, RK4 steps of the
+    shared launches (graph-replayed and eager, the faster counted) against
+    the same steps with per-case launches, and each case's RHS error against
+    its FP64 exact-chemistry RHS."""
     import warnings
 
     import torch

@@ -385,8 +385,10 @@ def autotune(ens, reps=10, write=None):
 
 
 def measure(ens, steps=5, sequential=True):
-    """Wall time per RK4 step: graph-replayed ensemble, and (optionally) the
-    same step with sequential per-case launches; DOF-RHS/s of each."""
+    """Wall time per RK4 step of the shared launches, graph-replayed and
+    eager (the faster of the two is the batched figure, both are listed),
+    and (optionally) the same step with sequential per-case launches;
+    DOF-RHS/s of each."""
     ens.capture()
     ens.step()
     torch.cuda.synchronize()
@@ -394,8 +396,18 @@ def measure(ens, steps=5, sequential=True):
     for _ in range(steps):
         ens.step()
     torch.cuda.synchronize()
-    ms_b = (time.perf_counter() - t0) / steps * 1e3
+    ms_g = (time.perf_counter() - t0) / steps * 1e3
+    ens.step_eager()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ens.step_eager()
+    torch.cuda.synchronize()
+    ms_e = (time.perf_counter() - t0) / steps * 1e3
+    ms_b = min(ms_g, ms_e)
     out = {"batched_ms_per_step": ms_b,
+           "batched_graph_ms_per_step": ms_g,
+           "batched_eager_ms_per_step": ms_e,
            "batched_dof_rhs_s": ens.dofs * 4 / (ms_b * 1e-3)}
     if sequential:
         ens.step_eager(sequential=True)
