@@ -70,6 +70,20 @@ struct PolyDev {
 // module); mg_load_mechanism / mg_load_transport fill every copy
 static __constant__ PolyDev c_poly;
 
+// Lowest and highest NASA7 switch temperature over the species: a wave whose
+// lanes all lie above the highest (below the lowest) needs only the high
+// (low) range, which the callers below evaluate alone in that case (the
+// same values as the per-lane selection).
+__device__ __forceinline__ void nasa7_tmid_range(double& lo, double& hi) {
+  lo = c_mech.t_mid[0];
+  hi = c_mech.t_mid[0];
+#pragma unroll
+  for (int k = 1; k < MIX_NSP; ++k) {
+    lo = fmin(lo, c_mech.t_mid[k]);
+    hi = fmax(hi, c_mech.t_mid[k]);
+  }
+}
+
 // The temperature range is chosen per lane, the species index is
 // wave-uniform: both ranges' polynomials are evaluated with scalar-operand
 // coefficients and the result is selected per lane (indexing the table by
@@ -143,12 +157,8 @@ __device__ __forceinline__ void mix_e_cv_range(double t, const double* y, double
 // common case), only that range's polynomials are evaluated.
 __device__ __forceinline__ double mix_temperature(double e_in, double t_seed,
                                                   const double* y, int niter) {
-  double tmid_lo = c_mech.t_mid[0], tmid_hi = c_mech.t_mid[0];
-#pragma unroll
-  for (int k = 1; k < MIX_NSP; ++k) {
-    tmid_lo = fmin(tmid_lo, c_mech.t_mid[k]);
-    tmid_hi = fmax(tmid_hi, c_mech.t_mid[k]);
-  }
+  double tmid_lo, tmid_hi;
+  nasa7_tmid_range(tmid_lo, tmid_hi);
   double t = t_seed;
   for (int it = 0; it < niter; ++it) {
     double e, cv;
@@ -398,6 +408,30 @@ __device__ __forceinline__ void node_transport(Node<DIM, NSP>& s, const GasDev& 
   }
 }
 
+// Mixture sums of a node at T: sum Y_k/W_k, sum cp_k/R Y_k/W_k and the
+// species enthalpies h_k (mass basis).  RANGE 0 / 1: only the low / high
+// NASA7 range (every active lane of the wave lies in it); -1: both ranges
+// with a per-lane select.
+template <int RANGE, int DIM, int NSP>
+__device__ __forceinline__ void mix_node_sums(Node<DIM, NSP>& s, double t, double& rsum,
+                                              double& cpsum) {
+  const double inv_t = 1.0 / t;
+#pragma unroll
+  for (int k = 0; k < MIX_NSP; ++k) {
+    double cp_r, h_rt;
+    if constexpr (RANGE < 0) {
+      nasa7_cp_h(k, t, inv_t, cp_r, h_rt);
+    } else {
+      const double* h = c_poly.h[RANGE][k];
+      cp_r = horner5(c_poly.cp[RANGE][k], t);
+      h_rt = h[0] + t * (h[1] + t * (h[2] + t * (h[3] + t * h[4]))) + h[5] * inv_t;
+    }
+    rsum += s.y[k] * c_mech.inv_wt[k];
+    cpsum += c_mech.inv_wt[k] * cp_r * s.y[k];
+    s.hk[k] = c_mech.gas_constant * c_mech.inv_wt[k] * t * h_rt;
+  }
+}
+
 // Fill the dependent variables of a node from q (and T for mixtures); the
 // transport coefficients only if TRANSPORT.
 template <int DIM, int NSP, int MIX, bool TRANSPORT = true>
@@ -426,16 +460,15 @@ __device__ __forceinline__ void node_state(Node<DIM, NSP>& s, const GasDev& g,
   } else {
     static_assert(NSP == MIX_NSP, "mixture path is the 7-species mechanism");
     s.t = t_mix;
+    double tmid_lo, tmid_hi;
+    nasa7_tmid_range(tmid_lo, tmid_hi);
     double rsum = 0.0, cpsum = 0.0;
-    const double inv_t = 1.0 / t_mix;
-#pragma unroll
-    for (int k = 0; k < MIX_NSP; ++k) {
-      double cp_r, h_rt;
-      nasa7_cp_h(k, t_mix, inv_t, cp_r, h_rt);
-      rsum += s.y[k] * c_mech.inv_wt[k];
-      cpsum += c_mech.inv_wt[k] * cp_r * s.y[k];
-      s.hk[k] = c_mech.gas_constant * c_mech.inv_wt[k] * t_mix * h_rt;
-    }
+    if (__all(t_mix > tmid_hi))
+      mix_node_sums<1>(s, t_mix, rsum, cpsum);
+    else if (__all(!(t_mix > tmid_lo)))
+      mix_node_sums<0>(s, t_mix, rsum, cpsum);
+    else
+      mix_node_sums<-1>(s, t_mix, rsum, cpsum);
     const double rmix = c_mech.gas_constant * rsum;
     const double cp = c_mech.gas_constant * cpsum;
     s.p = rho * rmix * t_mix;
@@ -644,17 +677,33 @@ __device__ __forceinline__ void ec_flux_dot(const EcPair<DIM, NSP>& a,
 
 // {{{ Renac's two-point flux (mixtures)
 
-// Mixture c_v (mass basis) at T: sum_k Y_k R / W_k (cp_k / R - 1).
-__device__ __forceinline__ double mix_cv(double t, const double* y) {
+// Mixture c_v (mass basis) at T: sum_k Y_k R / W_k (cp_k / R - 1); one
+// NASA7 range when every active lane of the wave lies in it.
+template <int RANGE>
+__device__ __forceinline__ double mix_cv_range(double t, const double* y) {
   double sc = 0.0;
 #pragma unroll
   for (int k = 0; k < MIX_NSP; ++k) {
-    const bool high = t > c_mech.t_mid[k];
-    const double cp_lo = horner5(c_poly.cp[0][k], t);
-    const double cp_hi = horner5(c_poly.cp[1][k], t);
-    sc += c_mech.inv_wt[k] * ((high ? cp_hi : cp_lo) - 1.0) * y[k];
+    double cp;
+    if constexpr (RANGE < 0) {
+      const bool high = t > c_mech.t_mid[k];
+      const double cp_lo = horner5(c_poly.cp[0][k], t);
+      const double cp_hi = horner5(c_poly.cp[1][k], t);
+      cp = high ? cp_hi : cp_lo;
+    } else {
+      cp = horner5(c_poly.cp[RANGE][k], t);
+    }
+    sc += c_mech.inv_wt[k] * (cp - 1.0) * y[k];
   }
   return c_mech.gas_constant * sc;
+}
+
+__device__ __forceinline__ double mix_cv(double t, const double* y) {
+  double tmid_lo, tmid_hi;
+  nasa7_tmid_range(tmid_lo, tmid_hi);
+  if (__all(t > tmid_hi)) return mix_cv_range<1>(t, y);
+  if (__all(!(t > tmid_lo))) return mix_cv_range<0>(t, y);
+  return mix_cv_range<-1>(t, y);
 }
 
 // Entropy variables (mirgecom/gas_model.py:1026-1060) of a state with its
