@@ -507,10 +507,9 @@ mg_status mg_ensemble_prepare(int32_t ncases, const mg_ens_case* cases, int32_t 
     plan->grid_div[p] = grid_for(plan->div_count[p], wg, cus);
   }
   plan->grid_pt = grid_for(plan->pt_count, 8, cus);
-  // surrogate: one workgroup per 128-row item (the dispatcher overlaps one
-  // workgroup's item lookup with the other resident workgroup's layers; a
-  // persistent loop over items at 2 per CU ran 15 % slower than the
-  // per-case launches); exact rates: persistent, 8 per CU
+  // surrogate: one workgroup per 128-row item (a persistent loop over the
+  // items at two workgroups per CU ran 15 % slower than the per-case
+  // launches); exact rates: persistent, 8 per CU
   plan->grid_src = surrogate ? int(std::min<int64_t>(std::max<int64_t>(plan->src_count, 1),
                                                      0x7fffffff))
                              : grid_for(plan->src_count, 8, cus);
